@@ -1,0 +1,199 @@
+/*
+ * mvs_hip.h -- C ABI of the MI355X (gfx950) dense multi-view-stereo library.
+ *
+ * Drop-in boundary for the reference `mvs` crate's dense path.  Every entry
+ * point names the reference item it replaces (paths relative to the
+ * reference repository root).  Conventions:
+ *   - plain C types and pointers only; every buffer crossing the boundary is
+ *     a HOST buffer owned by the caller (no device pointers cross it);
+ *   - return value 0 = ok, < 0 = error (MVS_ERR_*); the message of the last
+ *     failure on the calling thread is available from mvs_last_error();
+ *   - no C++ exception crosses the boundary;
+ *   - one context = one device + one HIP stream; calls on one context are
+ *     serialised internally, so the reference's concurrent per-view calls
+ *     (mvs/src/lib.rs:351-374, rayon) are safe; mvs_estimate_depth_batch is
+ *     the intended replacement for that loop.
+ */
+#ifndef MVS_HIP_H
+#define MVS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MVS_OK 0
+#define MVS_ERR_ARG (-1)     /* invalid argument (sizes, indices, NULL) */
+#define MVS_ERR_HIP (-2)     /* HIP runtime failure */
+#define MVS_ERR_NOMEM (-3)   /* device or host allocation failed */
+#define MVS_ERR_STATE (-4)   /* call out of order (e.g. no views uploaded) */
+#define MVS_ERR_LIMIT (-5)   /* input beyond a documented limit */
+
+typedef struct mvs_ctx mvs_ctx;
+
+/* MvsOptions (mvs/src/lib.rs:33-54); defaults from lib.rs:56-70. */
+typedef struct mvs_options {
+    uint32_t max_dim;         /* 900 (input stage; views arrive at working resolution) */
+    uint32_t num_sources;     /* 3 */
+    uint32_t sweeps;          /* 4: passes = 2 * max(sweeps, 1) */
+    uint32_t num_hyp;         /* 48: plane-sweep hypotheses, max(num_hyp, 2) used */
+    uint32_t min_consistent;  /* 2 */
+    uint32_t save_depth_vis;  /* 0 */
+    double depth_diff;        /* 0.03 */
+    double voxel_rel;         /* 0.0015 */
+    float cost_thr;           /* 0.38 */
+    uint32_t reserved;        /* 0 */
+} mvs_options_t;
+
+/* One registered view at working resolution: the fields of ViewData
+ * (mvs/src/lib.rs:86-101) that the dense path reads.  `gray` is the u8
+ * luma the reference divides by 255 (lib.rs:193-196); the census and the
+ * per-pixel rays are computed on the device.  Cam / Pose as in
+ * sfm-lite/src/geom.rs:13-53 (SIMPLE_RADIAL, world->camera, R row-major). */
+typedef struct mvs_view {
+    int32_t w, h;
+    double f, cx, cy, k;
+    double R[9];
+    double t[3];
+    const uint8_t *gray;      /* w*h */
+    const uint8_t *rgb;       /* 3*w*h, or NULL = gray replicated (to_rgb8 of luma) */
+} mvs_view_t;
+
+/* One reference view of a batched stereo run. */
+typedef struct mvs_job {
+    int32_t ref;              /* view index */
+    int32_t k;                /* number of sources */
+    const int32_t *srcs;      /* k view indices */
+    double zmin, zmax;        /* depth range (depth_range_for, lib.rs:253-282) */
+    float *depth;             /* optional host outputs (w*h, w*h, 3*w*h); NULL = keep on device */
+    float *cost;
+    float *normals;
+} mvs_job_t;
+
+/* Fused point: (x, y, z, [r, g, b]) of fuse_depth_maps (mvs/src/fuse.rs:71). */
+typedef struct mvs_point {
+    double x, y, z;
+    uint8_t r, g, b, pad[5];
+} mvs_point_t;
+
+/* Work counters and device timings of the last stereo / fusion call. */
+typedef struct mvs_stats {
+    int64_t n_eval_init;      /* plane_cost evaluations in the init sweep */
+    int64_t n_eval_pass;      /* plane_cost evaluations over all passes */
+    int64_t n_pixels;         /* reference pixels processed */
+    int64_t taps_per_eval_sum;/* sum over jobs of pixels * 25 * K (for byte accounting) */
+    double ms_prepare;        /* census + rays kernels of the last upload */
+    double ms_init;
+    double ms_pass;           /* all passes */
+    double ms_gate;
+    double ms_fuse;
+    int64_t n_candidates;     /* fusion candidates of the last fusion */
+    int64_t n_points;
+    int32_t n_pass_launches;
+    int32_t graph_used;
+} mvs_stats_t;
+
+/* MvsOptions::default (mvs/src/lib.rs:56-70). */
+void mvs_options_default(mvs_options_t *opts);
+
+/* Thread-local message of the last failed call on this thread ("" if none). */
+const char *mvs_last_error(void);
+
+/* Context: device index (as HIP numbers them) and one stream. */
+int mvs_ctx_create(int device, mvs_ctx **out);
+void mvs_ctx_destroy(mvs_ctx *ctx);
+
+/* census_map (mvs/src/lib.rs:119-142): 7x7 census, bit (dy+3)*7+(dx+3),
+ * clamp-to-edge, set when the neighbour is darker.  Host in, host out. */
+int mvs_census(mvs_ctx *ctx, const uint8_t *gray, int32_t w, int32_t h, uint64_t *out);
+
+/* load_views (mvs/src/lib.rs:144-218) after decode/resize: uploads n views,
+ * replacing any previous set, and computes each view's census and rays on
+ * the device (one launch for all views). */
+int mvs_upload_views(mvs_ctx *ctx, const mvs_view_t *views, int32_t n);
+
+/* ViewData.census of an uploaded view (w*h u64). */
+int mvs_view_census(mvs_ctx *ctx, int32_t view, uint64_t *out);
+
+/* stereo::estimate_depth (mvs/src/stereo.rs:234-394) for one reference
+ * view: DepthMap.depth (0 = invalid) and DepthMap.cost, plus the accepted
+ * unit normals (3 floats/px, camera frame; NULL to skip) that the reference
+ * keeps internal (stereo.rs:24-31). */
+int mvs_estimate_depth(mvs_ctx *ctx, int32_t ref, const int32_t *srcs, int32_t k, double zmin, double zmax,
+                       const mvs_options_t *opts, float *depth, float *cost, float *normals);
+
+/* As mvs_estimate_depth, plus: stop after `max_passes` passes (< 0 = all;
+ * the decay schedule still uses 2*sweeps), the pre-gate plane buffer
+ * (5 floats/px: nx, ny, nz, z, cost; NULL to skip) and per-stage evaluation
+ * counts n_eval[0] = init, n_eval[1+p] = pass p (NULL to skip). */
+int mvs_estimate_depth_ex(mvs_ctx *ctx, int32_t ref, const int32_t *srcs, int32_t k, double zmin, double zmax,
+                          const mvs_options_t *opts, int32_t max_passes, float *depth, float *cost,
+                          float *normals, float *planes, int64_t *n_eval);
+
+/* The per-view stereo loop of run_mvs (mvs/src/lib.rs:351-374) for all
+ * jobs at once: one launch per stage covers every job (uneven sizes and K),
+ * and the stage schedule is replayed from a captured HIP graph.  Results
+ * stay resident for mvs_fuse and are copied to the jobs' host pointers
+ * where those are non-NULL.  `mode`: 0 = auto, 1 = one launch per job and
+ * stage (unbatched reference schedule), 2 = batched tile grid,
+ * 3 = batched persistent tile queue. */
+int mvs_estimate_depth_batch(mvs_ctx *ctx, const mvs_job_t *jobs, int32_t njobs, const mvs_options_t *opts,
+                             int32_t mode);
+
+/* Download the resident depth / cost / normals of a view computed by the
+ * last batch (NULL pointers skip). */
+int mvs_get_depth(mvs_ctx *ctx, int32_t view, float *depth, float *cost, float *normals);
+
+/* fuse::fuse_depth_maps (mvs/src/fuse.rs:64-215) over the depth maps held
+ * on the device from the last stereo calls.  order_off has nviews+1
+ * entries; order[order_off[i] .. order_off[i+1]) is fusion_order[i]
+ * (lib.rs:398-418).  `has_depth[i]` != 0 marks views with a depth map
+ * (Option::Some).  `cell` is the absolute voxel size (see mvs_fusion_cell).
+ * Points come back in first-insertion order of their voxel (view-major
+ * raster order of the first candidate); the reference's HashMap order is
+ * unspecified.  *pts is allocated by the library: free with
+ * mvs_free_points. */
+int mvs_fuse(mvs_ctx *ctx, const int32_t *has_depth, const int32_t *order_off, const int32_t *order,
+             double cell, const mvs_options_t *opts, mvs_point_t **pts, int64_t *n);
+
+/* As mvs_fuse with caller-provided host depth maps (NULL entry = None),
+ * the exact signature shape of fuse_depth_maps. */
+int mvs_fuse_depth_maps(mvs_ctx *ctx, const float *const *depth_maps, const int32_t *order_off,
+                        const int32_t *order, double cell, const mvs_options_t *opts, mvs_point_t **pts,
+                        int64_t *n);
+
+void mvs_free_points(mvs_point_t *pts);
+
+/* Robust voxel cell of fuse_depth_maps (mvs/src/fuse.rs:72-88): per-axis
+ * 2 %/98 % extent of the sparse points -> diag * voxel_rel (>= 1e-9).
+ * Returns < 0 when n == 0 (the reference panics there). */
+double mvs_fusion_cell(const double *xyz, int64_t n, double voxel_rel);
+
+/* write_ply_binary (mvs/src/fuse.rs:221-239). */
+int mvs_write_ply(const char *path, const mvs_point_t *pts, int64_t n);
+
+/* Counters and device timings of the last call on this context. */
+int mvs_get_stats(mvs_ctx *ctx, mvs_stats_t *out);
+
+/* Instrumentation for parity tests: evaluates plane_cost
+ * (mvs/src/stereo.rs:158-228) on the device for n (pixel, plane, hint)
+ * queries of reference view `ref` with sources srcs.  q holds 7 floats per
+ * query: px, py, nx, ny, nz, d, hint.  Outputs per query: the returned
+ * cost, the integer Hamming sum and invalid-tap count over the full
+ * window, and the peak partial average (max over (source, row) prefixes). */
+int mvs_debug_plane_cost(mvs_ctx *ctx, int32_t ref, const int32_t *srcs, int32_t k, const float *q, int32_t n,
+                         float *cost, int32_t *ham, int32_t *ninv, float *peak);
+
+/* Per-pixel RNG of estimate_depth (stereo.rs:47-78) evaluated on the
+ * device: for n (x, y, salt) triples, the seed and the first `ndraw`
+ * uniform() values of its stream. */
+int mvs_debug_rng(mvs_ctx *ctx, const int32_t *xy, const uint64_t *salt, int32_t n, int32_t ndraw,
+                  uint64_t *seed_out, float *uniform_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MVS_HIP_H */

@@ -1,0 +1,903 @@
+// capi.hip -- context, host orchestration and the C ABI (include/mvs_hip.h).
+//
+// Host-side duties kept from the reference:
+//   * RefWorkspace::new warp parameters (mvs/src/stereo.rs:126-149) in f64
+//     with nalgebra's left-to-right products, cast to f32;
+//   * the decay schedule (stereo.rs:275-277) with libm powf;
+//   * the stage schedule of estimate_depth (init, 2*sweeps passes, gate),
+//     recorded once per batch shape into a HIP graph and replayed;
+//   * fusion inputs (fusion order, depth-map presence) and the robust voxel
+//     cell (fuse.rs:72-88); PLY writing (fuse.rs:221-239).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mvs_hip.h"
+#include "stereo_kernels.h"
+#include "tuner.h"
+
+using namespace mvs;
+
+// --------------------------------------------------------------------------
+// error plumbing
+// --------------------------------------------------------------------------
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HCHK(call)                                                                                       \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            return fail(MVS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+// --------------------------------------------------------------------------
+// context
+// --------------------------------------------------------------------------
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    hipError_t need(size_t n) {
+        if (p && n <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc((void**)&p, (n ? n : 1) * sizeof(T));
+        if (e == hipSuccess) cap = n;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct ViewState {
+    int w = 0, h = 0;
+    double f = 0, cx = 0, cy = 0, k = 0, R[9] = {0}, t[3] = {0};
+    bool has_rgb = false;
+    DBuf<uint8_t> gray, rgb;
+    DBuf<u64> census;
+    DBuf<float2> rays;
+    // stereo state (allocated when the view is a reference view)
+    DBuf<float4> plane[2];
+    DBuf<float> cost[2];
+    DBuf<float> depth, cost_out, normal;
+    bool has_depth = false;
+    void release() {
+        gray.release(); rgb.release(); census.release(); rays.release();
+        plane[0].release(); plane[1].release(); cost[0].release(); cost[1].release();
+        depth.release(); cost_out.release(); normal.release();
+        has_depth = false;
+    }
+};
+
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<uint64_t> sig;   // batch signature the graph was captured for
+    void release() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        sig.clear();
+    }
+};
+
+struct mvs_ctx {
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::vector<ViewState> views;
+    DBuf<PrepView> prep;
+    // batch descriptors
+    DBuf<JobDesc> jobs;
+    DBuf<SrcDesc> srcs;
+    DBuf<int4> tiles;
+    DBuf<u64> counters;   // [0] init, [1 + p] pass p
+    DBuf<int> queues;     // persistent-kernel queue heads, one per pass
+    GraphCache graph;
+    hipEvent_t ev[8] = {nullptr};
+    // fusion
+    FuseScratch* fuse = nullptr;
+    DBuf<FuseView> fviews;
+    DBuf<int32_t> forder;
+    DBuf<float> tmp_depth;   // host depth maps uploaded for mvs_fuse_depth_maps
+    // debug
+    DBuf<uint8_t> dbg;
+    mvs_stats_t stats;
+    Tuner tuner;
+};
+
+static int set_device(mvs_ctx* c) {
+    HCHK(hipSetDevice(c->device));
+    return MVS_OK;
+}
+
+extern "C" {
+
+void mvs_options_default(mvs_options_t* o) {
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->max_dim = 900;
+    o->num_sources = 3;
+    o->sweeps = 4;
+    o->num_hyp = 48;
+    o->min_consistent = 2;
+    o->depth_diff = 0.03;
+    o->cost_thr = 0.38f;
+    o->voxel_rel = 0.0015;
+    o->save_depth_vis = 0;
+}
+
+const char* mvs_last_error(void) { return g_last_error.c_str(); }
+
+int mvs_ctx_create(int device, mvs_ctx** out) {
+    if (!out) return fail(MVS_ERR_ARG, "mvs_ctx_create: out is NULL");
+    *out = nullptr;
+    try {
+        int n = 0;
+        HCHK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) return fail(MVS_ERR_ARG, "mvs_ctx_create: no such device");
+        mvs_ctx* c = new mvs_ctx();
+        c->device = device;
+        if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return fail(MVS_ERR_HIP, "mvs_ctx_create: stream creation failed");
+        }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+        for (auto& e : c->ev) (void)hipEventCreate(&e);
+        memset(&c->stats, 0, sizeof(c->stats));
+        *out = c;
+        return MVS_OK;
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_ctx_create: ") + e.what());
+    }
+}
+
+void mvs_ctx_destroy(mvs_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->graph.release();
+    for (auto& v : c->views) v.release();
+    c->prep.release(); c->jobs.release(); c->srcs.release(); c->tiles.release();
+    c->counters.release(); c->queues.release(); c->fviews.release(); c->forder.release();
+    c->tmp_depth.release(); c->dbg.release();
+    free_fuse_scratch(c->fuse);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// --------------------------------------------------------------------------
+// census / upload
+// --------------------------------------------------------------------------
+
+int mvs_census(mvs_ctx* c, const uint8_t* gray, int32_t w, int32_t h, uint64_t* out) {
+    if (!c || !gray || !out || w <= 0 || h <= 0) return fail(MVS_ERR_ARG, "mvs_census: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int r = set_device(c)) return r;
+    const size_t n = (size_t)w * h;
+    DBuf<uint8_t> g;
+    DBuf<u64> cen;
+    DBuf<PrepView> pv;
+    if (g.need(n) != hipSuccess || cen.need(n) != hipSuccess || pv.need(1) != hipSuccess) {
+        g.release(); cen.release(); pv.release();
+        return fail(MVS_ERR_NOMEM, "mvs_census: device allocation failed");
+    }
+    PrepView v{};
+    v.gray = g.p;
+    v.census = cen.p;
+    v.rays = nullptr;
+    v.w = w;
+    v.h = h;
+    const int nb = prepare_tiles(w, h, v.tiles_x);
+    v.tile_off = 0;
+    hipError_t e = hipMemcpyAsync(g.p, gray, n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(pv.p, &v, sizeof(v), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_prepare(pv.p, 1, nb, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, cen.p, n * sizeof(u64), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    g.release(); cen.release(); pv.release();
+    if (e != hipSuccess) return fail(MVS_ERR_HIP, std::string("mvs_census: ") + hipGetErrorString(e));
+    return MVS_OK;
+}
+
+int mvs_upload_views(mvs_ctx* c, const mvs_view_t* views, int32_t n) {
+    if (!c || !views || n < 1) return fail(MVS_ERR_ARG, "mvs_upload_views: bad arguments");
+    for (int i = 0; i < n; i++) {
+        if (views[i].w <= 0 || views[i].h <= 0 || !views[i].gray)
+            return fail(MVS_ERR_ARG, "mvs_upload_views: view " + std::to_string(i) + " has no image");
+        if ((int64_t)views[i].w * views[i].h > (int64_t)1 << 30)
+            return fail(MVS_ERR_LIMIT, "mvs_upload_views: view larger than 2^30 pixels");
+        if (!(views[i].f != 0.0)) return fail(MVS_ERR_ARG, "mvs_upload_views: focal length is 0");
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int r = set_device(c)) return r;
+    HCHK(hipStreamSynchronize(c->stream));
+    c->graph.release();
+    if ((int)c->views.size() > n) {
+        for (size_t i = n; i < c->views.size(); i++) c->views[i].release();
+    }
+    c->views.resize(n);
+    std::vector<PrepView> pv(n);
+    int tile_off = 0;
+    for (int i = 0; i < n; i++) {
+        const mvs_view_t& in = views[i];
+        ViewState& v = c->views[i];
+        const size_t np = (size_t)in.w * in.h;
+        if (v.w != in.w || v.h != in.h) {
+            v.release();
+        }
+        v.w = in.w; v.h = in.h; v.f = in.f; v.cx = in.cx; v.cy = in.cy; v.k = in.k;
+        memcpy(v.R, in.R, sizeof(v.R));
+        memcpy(v.t, in.t, sizeof(v.t));
+        v.has_depth = false;
+        v.has_rgb = in.rgb != nullptr;
+        HCHK(v.gray.need(np));
+        HCHK(v.census.need(np));
+        HCHK(v.rays.need(np));
+        HCHK(hipMemcpyAsync(v.gray.p, in.gray, np, hipMemcpyHostToDevice, c->stream));
+        if (in.rgb) {
+            HCHK(v.rgb.need(3 * np));
+            HCHK(hipMemcpyAsync(v.rgb.p, in.rgb, 3 * np, hipMemcpyHostToDevice, c->stream));
+        }
+        PrepView& p = pv[i];
+        p.gray = v.gray.p;
+        p.census = v.census.p;
+        p.rays = v.rays.p;
+        p.w = in.w; p.h = in.h; p.f = in.f; p.cx = in.cx; p.cy = in.cy; p.k = in.k;
+        const int nt = prepare_tiles(in.w, in.h, p.tiles_x);
+        p.tile_off = tile_off;
+        tile_off += nt;
+    }
+    HCHK(c->prep.need(n));
+    HCHK(hipMemcpyAsync(c->prep.p, pv.data(), n * sizeof(PrepView), hipMemcpyHostToDevice, c->stream));
+    HCHK(hipEventRecord(c->ev[0], c->stream));
+    HCHK(launch_prepare(c->prep.p, n, tile_off, c->stream));
+    HCHK(hipEventRecord(c->ev[1], c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
+    c->stats.ms_prepare = ms;
+    return MVS_OK;
+}
+
+int mvs_view_census(mvs_ctx* c, int32_t view, uint64_t* out) {
+    if (!c || !out) return fail(MVS_ERR_ARG, "mvs_view_census: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (view < 0 || view >= (int)c->views.size()) return fail(MVS_ERR_ARG, "mvs_view_census: no such view");
+    if (int r = set_device(c)) return r;
+    ViewState& v = c->views[view];
+    HCHK(hipMemcpyAsync(out, v.census.p, (size_t)v.w * v.h * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+    return MVS_OK;
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------------------------
+// stereo batch
+// --------------------------------------------------------------------------
+
+// SourceCtx of stereo.rs:127-148: rs = R_s R_r^T, ts = t_s - rs t_r in f64.
+static SrcDesc make_src(const ViewState& rv, const ViewState& sv) {
+    double rs64[9], ts64[3], rtt[3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            rs64[3 * i + j] = (sv.R[3 * i] * rv.R[3 * j] + sv.R[3 * i + 1] * rv.R[3 * j + 1]) + sv.R[3 * i + 2] * rv.R[3 * j + 2];
+    for (int i = 0; i < 3; i++) rtt[i] = (rs64[3 * i] * rv.t[0] + rs64[3 * i + 1] * rv.t[1]) + rs64[3 * i + 2] * rv.t[2];
+    for (int i = 0; i < 3; i++) ts64[i] = sv.t[i] - rtt[i];
+    SrcDesc s{};
+    s.census = sv.census.p;
+    s.w = sv.w;
+    s.h = sv.h;
+    for (int i = 0; i < 9; i++) s.rs[i] = (float)rs64[i];
+    for (int i = 0; i < 3; i++) s.ts[i] = (float)ts64[i];
+    s.f = (float)sv.f;
+    s.cx = (float)sv.cx;
+    s.cy = (float)sv.cy;
+    s.k = (float)sv.k;
+    return s;
+}
+
+struct BatchPlan {
+    std::vector<JobDesc> jobs;
+    std::vector<SrcDesc> srcs;
+    std::vector<int4> tiles;
+    std::vector<uint64_t> sig;
+    TileCfg cfg;
+    int n_passes = 0, run_passes = 0, nh = 0;
+    int grid_p = 0;   // persistent grid size
+    int64_t n_pixels = 0, taps = 0;
+};
+
+static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_options_t* o, int mode, int max_passes,
+                      BatchPlan& P) {
+    if (c->views.empty()) return fail(MVS_ERR_STATE, "estimate_depth: no views uploaded");
+    std::vector<char> seen(c->views.size(), 0);
+    P.nh = (int)std::max<uint32_t>(o->num_hyp, 2);
+    P.n_passes = (int)std::max<uint32_t>(o->sweeps, 1) * 2;
+    P.run_passes = (max_passes < 0 || max_passes > P.n_passes) ? P.n_passes : max_passes;
+    // launch shape: tuned per shape class unless the mode pins it
+    int maxw = 0, maxh = 0, maxk = 0;
+    for (int j = 0; j < njobs; j++) {
+        const mvs_job_t& jb = jobs[j];
+        if (jb.ref < 0 || jb.ref >= (int)c->views.size()) return fail(MVS_ERR_ARG, "estimate_depth: bad ref index");
+        if (seen[jb.ref]) return fail(MVS_ERR_ARG, "estimate_depth: a view is the reference of two jobs");
+        seen[jb.ref] = 1;
+        if (jb.k < 1 || !jb.srcs) return fail(MVS_ERR_ARG, "estimate_depth: job needs >= 1 source");
+        for (int s = 0; s < jb.k; s++)
+            if (jb.srcs[s] < 0 || jb.srcs[s] >= (int)c->views.size())
+                return fail(MVS_ERR_ARG, "estimate_depth: bad source index");
+        if (!(jb.zmin > 0.0) || !(jb.zmax > 0.0)) return fail(MVS_ERR_ARG, "estimate_depth: zmin/zmax must be > 0");
+        maxw = std::max(maxw, c->views[jb.ref].w);
+        maxh = std::max(maxh, c->views[jb.ref].h);
+        maxk = std::max(maxk, jb.k);
+    }
+    P.cfg = c->tuner.pick(maxw, maxh, maxk, njobs);
+    if ((mode & 3) == 2) P.cfg.persistent = 0;
+    if ((mode & 3) == 3) P.cfg.persistent = 1;
+    const int TW = P.cfg.tw, TH = P.cfg.th;
+    int tile_off = 0;
+    for (int j = 0; j < njobs; j++) {
+        const mvs_job_t& jb = jobs[j];
+        ViewState& rv = c->views[jb.ref];
+        const size_t np = (size_t)rv.w * rv.h;
+        for (int b = 0; b < 2; b++) {
+            HCHK(rv.plane[b].need(np));
+            HCHK(rv.cost[b].need(np));
+        }
+        HCHK(rv.depth.need(np));
+        HCHK(rv.cost_out.need(np));
+        HCHK(rv.normal.need(3 * np));
+        JobDesc J{};
+        J.census = rv.census.p;
+        J.rays = rv.rays.p;
+        J.plane[0] = rv.plane[0].p;
+        J.plane[1] = rv.plane[1].p;
+        J.cost[0] = rv.cost[0].p;
+        J.cost[1] = rv.cost[1].p;
+        J.depth_out = rv.depth.p;
+        J.cost_out = rv.cost_out.p;
+        J.normal_out = rv.normal.p;
+        J.w = rv.w;
+        J.h = rv.h;
+        J.nsrc = jb.k;
+        J.src_off = (int)P.srcs.size();
+        J.zmin = (float)jb.zmin;
+        J.zmax = (float)jb.zmax;
+        volatile float one = 1.0f;
+        J.inv_min = one / J.zmax;
+        J.inv_max = one / J.zmin;
+        J.tiles_x = (rv.w + TW - 1) / TW;
+        J.tiles_y = (rv.h + TH - 1) / TH;
+        J.tile_off = tile_off;
+        for (int s = 0; s < jb.k; s++) P.srcs.push_back(make_src(rv, c->views[jb.srcs[s]]));
+        for (int ty = 0; ty < J.tiles_y; ty++)
+            for (int tx = 0; tx < J.tiles_x; tx++) P.tiles.push_back(make_int4(j, tx * TW, ty * TH, 0));
+        tile_off += J.tiles_x * J.tiles_y;
+        P.jobs.push_back(J);
+        P.n_pixels += (int64_t)np;
+        P.taps += (int64_t)np * 25 * jb.k;
+        // signature: everything the captured launch sequence depends on
+        P.sig.push_back((uint64_t)jb.ref);
+        P.sig.push_back((uint64_t)(uintptr_t)rv.plane[0].p);
+        P.sig.push_back((uint64_t)jb.k);
+        for (int s = 0; s < jb.k; s++) P.sig.push_back((uint64_t)jb.srcs[s]);
+        uint64_t zz;
+        memcpy(&zz, &jb.zmin, 8);
+        P.sig.push_back(zz);
+        memcpy(&zz, &jb.zmax, 8);
+        P.sig.push_back(zz);
+    }
+    P.sig.push_back((uint64_t)P.nh);
+    P.sig.push_back((uint64_t)P.n_passes);
+    P.sig.push_back((uint64_t)P.run_passes);
+    uint32_t thr_bits;
+    memcpy(&thr_bits, &o->cost_thr, 4);
+    P.sig.push_back(thr_bits);
+    P.sig.push_back((uint64_t)(P.cfg.tw * 1000 + P.cfg.th * 10 + P.cfg.persistent));
+    P.grid_p = std::min((int)P.tiles.size(), c->n_cu * persistent_blocks_per_cu(P.cfg));
+    P.sig.push_back((uint64_t)P.grid_p);
+    P.sig.push_back((uint64_t)(uintptr_t)c->jobs.p);
+    return MVS_OK;
+}
+
+// Records the stage sequence of a batch on c->stream (captured into a graph
+// by the caller when requested).  Counters and queue heads are cleared by a
+// fill kernel, never by a memset node.
+static hipError_t enqueue_batch(mvs_ctx* c, const BatchPlan& P, float cost_thr, bool timed) {
+    hipStream_t st = c->stream;
+    const int ntiles = (int)P.tiles.size();
+    const int ncnt = 1 + P.n_passes;
+    hipError_t e = launch_fill_u64(c->counters.p, ncnt + (P.n_passes + 1) / 2 + 1, 0ULL, st);
+    if (e != hipSuccess) return e;
+    if (timed) (void)hipEventRecord(c->ev[0], st);
+    e = launch_init(P.cfg, c->jobs.p, c->srcs.p, c->tiles.p, ntiles, P.nh, c->counters.p, st);
+    if (e != hipSuccess) return e;
+    if (timed) (void)hipEventRecord(c->ev[1], st);
+    for (int p = 0; p < P.run_passes; p++) {
+        const float t = (float)p / (float)P.n_passes;
+        const float sig_inv = 0.02f * powf(0.28f, t * 4.0f);
+        const float sig_n = 0.30f * powf(0.42f, t * 4.0f);
+        e = launch_pass(P.cfg, c->jobs.p, c->srcs.p, c->tiles.p, ntiles, p & 1, (u64)p + 7, sig_inv, sig_n,
+                        c->counters.p + 1 + p, ((int*)(c->counters.p + ncnt)) + p, P.grid_p, st);
+        if (e != hipSuccess) return e;
+    }
+    if (timed) (void)hipEventRecord(c->ev[2], st);
+    e = launch_gate(P.cfg, c->jobs.p, c->tiles.p, ntiles, P.run_passes & 1, cost_thr, st);
+    if (timed) (void)hipEventRecord(c->ev[3], st);
+    return e;
+}
+
+static int run_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_options_t* o, int mode, int max_passes,
+                     float* planes_out, int64_t* n_eval_out) {
+    BatchPlan P;
+    if (int r = plan_batch(c, jobs, njobs, o, mode, max_passes, P)) return r;
+    const int ncnt = 1 + P.n_passes;
+    const int sched = mode & 3;
+    const bool use_graph = sched != 1 && !(mode & 4);
+    bool reuse = use_graph && c->graph.exec && c->graph.sig == P.sig;
+    if (!reuse) {
+        c->graph.release();
+        HCHK(c->jobs.need(P.jobs.size()));
+        HCHK(c->srcs.need(P.srcs.size()));
+        HCHK(c->tiles.need(P.tiles.size()));
+        HCHK(c->counters.need(ncnt + (P.n_passes + 1) / 2 + 1));
+        // the job array pointer is part of the signature
+        P.sig.back() = (uint64_t)(uintptr_t)c->jobs.p;
+        HCHK(hipMemcpyAsync(c->jobs.p, P.jobs.data(), P.jobs.size() * sizeof(JobDesc), hipMemcpyHostToDevice, c->stream));
+        HCHK(hipMemcpyAsync(c->srcs.p, P.srcs.data(), P.srcs.size() * sizeof(SrcDesc), hipMemcpyHostToDevice, c->stream));
+        HCHK(hipMemcpyAsync(c->tiles.p, P.tiles.data(), P.tiles.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+        HCHK(hipStreamSynchronize(c->stream));
+    }
+    if (sched == 1) {
+        // unbatched schedule: every stage launched per job (same kernels)
+        float ms_init = 0, ms_pass = 0, ms_gate = 0;
+        HCHK(launch_fill_u64(c->counters.p, ncnt + (P.n_passes + 1) / 2 + 1, 0ULL, c->stream));
+        int toff = 0;
+        for (int j = 0; j < njobs; j++) {
+            const int nt = P.jobs[j].tiles_x * P.jobs[j].tiles_y;
+            HCHK(hipEventRecord(c->ev[0], c->stream));
+            HCHK(launch_init(P.cfg, c->jobs.p, c->srcs.p, c->tiles.p + toff, nt, P.nh, c->counters.p, c->stream));
+            HCHK(hipEventRecord(c->ev[1], c->stream));
+            const int grid_p = std::min(nt, c->n_cu * persistent_blocks_per_cu(P.cfg));
+            for (int p = 0; p < P.run_passes; p++) {
+                const float t = (float)p / (float)P.n_passes;
+                const float sig_inv = 0.02f * powf(0.28f, t * 4.0f);
+                const float sig_n = 0.30f * powf(0.42f, t * 4.0f);
+                HCHK(launch_fill_u64(c->counters.p + ncnt, (P.n_passes + 1) / 2 + 1, 0ULL, c->stream));
+                HCHK(launch_pass(P.cfg, c->jobs.p, c->srcs.p, c->tiles.p + toff, nt, p & 1, (u64)p + 7, sig_inv, sig_n,
+                                 c->counters.p + 1 + p, ((int*)(c->counters.p + ncnt)) + p, grid_p, c->stream));
+            }
+            HCHK(hipEventRecord(c->ev[2], c->stream));
+            HCHK(launch_gate(P.cfg, c->jobs.p, c->tiles.p + toff, nt, P.run_passes & 1, o->cost_thr, c->stream));
+            HCHK(hipEventRecord(c->ev[3], c->stream));
+            HCHK(hipStreamSynchronize(c->stream));
+            float a = 0, b = 0, d = 0;
+            (void)hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+            (void)hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
+            (void)hipEventElapsedTime(&d, c->ev[2], c->ev[3]);
+            ms_init += a; ms_pass += b; ms_gate += d;
+            toff += nt;
+        }
+        c->stats.ms_init = ms_init;
+        c->stats.ms_pass = ms_pass;
+        c->stats.ms_gate = ms_gate;
+        c->stats.graph_used = 0;
+    } else if (!use_graph) {
+        // batched schedule launched directly, with per-stage events
+        HCHK(enqueue_batch(c, P, o->cost_thr, true));
+        HCHK(hipStreamSynchronize(c->stream));
+        float a = 0, b = 0, d = 0;
+        (void)hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+        (void)hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&d, c->ev[2], c->ev[3]);
+        c->stats.ms_init = a;
+        c->stats.ms_pass = b;
+        c->stats.ms_gate = d;
+        c->stats.graph_used = 0;
+    } else {
+        if (!reuse) {
+            HCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = enqueue_batch(c, P, o->cost_thr, false);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(MVS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            hipGraphExec_t ex = nullptr;
+            hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            if (e3 != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                return fail(MVS_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e3));
+            }
+            c->graph.graph = g;
+            c->graph.exec = ex;
+            c->graph.sig = P.sig;
+        }
+        HCHK(hipEventRecord(c->ev[4], c->stream));
+        HCHK(hipGraphLaunch(c->graph.exec, c->stream));
+        HCHK(hipEventRecord(c->ev[5], c->stream));
+        HCHK(hipStreamSynchronize(c->stream));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->ev[4], c->ev[5]);
+        // per-stage split of the graph replay is not observable from events
+        // inside it; the total is reported as passes + init.
+        c->stats.ms_init = 0;
+        c->stats.ms_pass = ms;
+        c->stats.ms_gate = 0;
+        c->stats.graph_used = 1;
+    }
+    std::vector<u64> cnt(ncnt);
+    HCHK(hipMemcpy(cnt.data(), c->counters.p, ncnt * sizeof(u64), hipMemcpyDeviceToHost));
+    c->stats.n_eval_init = (int64_t)cnt[0];
+    c->stats.n_eval_pass = 0;
+    for (int p = 0; p < P.run_passes; p++) c->stats.n_eval_pass += (int64_t)cnt[1 + p];
+    c->stats.n_pixels = P.n_pixels;
+    c->stats.taps_per_eval_sum = P.taps;
+    c->stats.n_pass_launches = P.run_passes;
+    if (n_eval_out) {
+        for (int p = 0; p < 1 + P.run_passes; p++) n_eval_out[p] = (int64_t)cnt[p];
+    }
+    for (int j = 0; j < njobs; j++) {
+        ViewState& rv = c->views[jobs[j].ref];
+        const size_t np = (size_t)rv.w * rv.h;
+        rv.has_depth = true;
+        if (jobs[j].depth) HCHK(hipMemcpyAsync(jobs[j].depth, rv.depth.p, np * 4, hipMemcpyDeviceToHost, c->stream));
+        if (jobs[j].cost) HCHK(hipMemcpyAsync(jobs[j].cost, rv.cost_out.p, np * 4, hipMemcpyDeviceToHost, c->stream));
+        if (jobs[j].normals)
+            HCHK(hipMemcpyAsync(jobs[j].normals, rv.normal.p, 3 * np * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (planes_out && njobs == 1) {
+        ViewState& rv = c->views[jobs[0].ref];
+        const size_t np = (size_t)rv.w * rv.h;
+        const int par = P.run_passes & 1;
+        std::vector<float4> pl(np);
+        std::vector<float> co(np);
+        HCHK(hipMemcpyAsync(pl.data(), rv.plane[par].p, np * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        HCHK(hipMemcpyAsync(co.data(), rv.cost[par].p, np * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HCHK(hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < np; i++) {
+            planes_out[5 * i] = pl[i].x;
+            planes_out[5 * i + 1] = pl[i].y;
+            planes_out[5 * i + 2] = pl[i].z;
+            planes_out[5 * i + 3] = pl[i].w;
+            planes_out[5 * i + 4] = co[i];
+        }
+    }
+    HCHK(hipStreamSynchronize(c->stream));
+    return MVS_OK;
+}
+
+extern "C" {
+
+int mvs_estimate_depth_ex(mvs_ctx* c, int32_t ref, const int32_t* srcs, int32_t k, double zmin, double zmax,
+                          const mvs_options_t* opts, int32_t max_passes, float* depth, float* cost, float* normals,
+                          float* planes, int64_t* n_eval) {
+    if (!c || !opts) return fail(MVS_ERR_ARG, "mvs_estimate_depth: NULL context or options");
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        mvs_job_t jb{};
+        jb.ref = ref;
+        jb.k = k;
+        jb.srcs = srcs;
+        jb.zmin = zmin;
+        jb.zmax = zmax;
+        jb.depth = depth;
+        jb.cost = cost;
+        jb.normals = normals;
+        return run_batch(c, &jb, 1, opts, 0, max_passes, planes, n_eval);
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_estimate_depth: ") + e.what());
+    }
+}
+
+int mvs_estimate_depth(mvs_ctx* c, int32_t ref, const int32_t* srcs, int32_t k, double zmin, double zmax,
+                       const mvs_options_t* opts, float* depth, float* cost, float* normals) {
+    return mvs_estimate_depth_ex(c, ref, srcs, k, zmin, zmax, opts, -1, depth, cost, normals, nullptr, nullptr);
+}
+
+int mvs_estimate_depth_batch(mvs_ctx* c, const mvs_job_t* jobs, int32_t njobs, const mvs_options_t* opts,
+                             int32_t mode) {
+    if (!c || !opts || !jobs || njobs < 1) return fail(MVS_ERR_ARG, "mvs_estimate_depth_batch: bad arguments");
+    if (mode < 0 || mode > 7) return fail(MVS_ERR_ARG, "mvs_estimate_depth_batch: mode must be 0..7");
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        return run_batch(c, jobs, njobs, opts, mode, -1, nullptr, nullptr);
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_estimate_depth_batch: ") + e.what());
+    }
+}
+
+int mvs_get_depth(mvs_ctx* c, int32_t view, float* depth, float* cost, float* normals) {
+    if (!c) return fail(MVS_ERR_ARG, "mvs_get_depth: NULL context");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (view < 0 || view >= (int)c->views.size()) return fail(MVS_ERR_ARG, "mvs_get_depth: no such view");
+    ViewState& v = c->views[view];
+    if (!v.has_depth) return fail(MVS_ERR_STATE, "mvs_get_depth: view has no depth map");
+    if (int r = set_device(c)) return r;
+    const size_t np = (size_t)v.w * v.h;
+    if (depth) HCHK(hipMemcpyAsync(depth, v.depth.p, np * 4, hipMemcpyDeviceToHost, c->stream));
+    if (cost) HCHK(hipMemcpyAsync(cost, v.cost_out.p, np * 4, hipMemcpyDeviceToHost, c->stream));
+    if (normals) HCHK(hipMemcpyAsync(normals, v.normal.p, 3 * np * 4, hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+    return MVS_OK;
+}
+
+// --------------------------------------------------------------------------
+// fusion
+// --------------------------------------------------------------------------
+
+static int fuse_common(mvs_ctx* c, const std::vector<const float*>& dmaps, const int32_t* order_off,
+                       const int32_t* order, double cell, const mvs_options_t* o, mvs_point_t** pts, int64_t* n) {
+    const int nv = (int)c->views.size();
+    if (!order_off || !pts || !n) return fail(MVS_ERR_ARG, "fuse: NULL argument");
+    if (!(cell > 0.0)) return fail(MVS_ERR_ARG, "fuse: cell must be > 0");
+    *pts = nullptr;
+    *n = 0;
+    if (order_off[0] != 0) return fail(MVS_ERR_ARG, "fuse: order_off[0] must be 0");
+    const int total_ord = order_off[nv];
+    for (int i = 0; i < nv; i++)
+        if (order_off[i + 1] < order_off[i]) return fail(MVS_ERR_ARG, "fuse: order_off must be non-decreasing");
+    for (int q = 0; q < total_ord; q++)
+        if (order[q] < 0 || order[q] >= nv) return fail(MVS_ERR_ARG, "fuse: bad view index in order");
+    std::vector<FuseView> fv(nv);
+    int64_t off = 0;
+    for (int i = 0; i < nv; i++) {
+        const ViewState& v = c->views[i];
+        FuseView& f = fv[i];
+        f.depth = dmaps[i];
+        f.rgb = v.has_rgb ? v.rgb.p : nullptr;
+        f.gray = v.gray.p;
+        f.w = v.w;
+        f.h = v.h;
+        f.f = v.f; f.cx = v.cx; f.cy = v.cy; f.k = v.k;
+        memcpy(f.R, v.R, sizeof(f.R));
+        memcpy(f.t, v.t, sizeof(f.t));
+        f.pix_off = off;
+        f.ord_off = order_off[i];
+        f.ord_n = order_off[i + 1] - order_off[i];
+        if (f.depth) off += (int64_t)v.w * v.h;
+    }
+    HCHK(c->fviews.need(nv));
+    HCHK(c->forder.need(total_ord > 0 ? total_ord : 1));
+    HCHK(hipMemcpyAsync(c->fviews.p, fv.data(), nv * sizeof(FuseView), hipMemcpyHostToDevice, c->stream));
+    if (total_ord > 0)
+        HCHK(hipMemcpyAsync(c->forder.p, order, total_ord * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    FuseArgs a;
+    a.d_views = c->fviews.p;
+    a.nviews = nv;
+    a.d_order = c->forder.p;
+    a.n_pix = off;
+    a.min_consistent = (int)o->min_consistent;
+    a.depth_diff = o->depth_diff;
+    a.cell = cell;
+    int64_t ncand = 0, npts = 0;
+    const void* dp = nullptr;
+    int lim = 0;
+    HCHK(hipEventRecord(c->ev[6], c->stream));
+    HCHK(run_fusion(c->fuse, a, fv.data(), c->stream, &ncand, &npts, &dp, &lim));
+    HCHK(hipEventRecord(c->ev[7], c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+    if (lim == 1) return fail(MVS_ERR_LIMIT, "fuse: more than 2^31-1 depth pixels");
+    if (lim == 2) return fail(MVS_ERR_LIMIT, "fuse: voxel grid extent exceeds 2^21 cells on an axis");
+    if (lim == 3) return fail(MVS_ERR_HIP, "fuse: voxel hash table overflow");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev[6], c->ev[7]);
+    c->stats.ms_fuse = ms;
+    c->stats.n_candidates = ncand;
+    c->stats.n_points = npts;
+    mvs_point_t* out = (mvs_point_t*)malloc((size_t)(npts > 0 ? npts : 1) * sizeof(mvs_point_t));
+    if (!out) return fail(MVS_ERR_NOMEM, "fuse: host allocation failed");
+    if (npts > 0) {
+        hipError_t e = hipMemcpyAsync(out, dp, (size_t)npts * sizeof(mvs_point_t), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            free(out);
+            return fail(MVS_ERR_HIP, std::string("fuse: ") + hipGetErrorString(e));
+        }
+    }
+    *pts = out;
+    *n = npts;
+    return MVS_OK;
+}
+
+int mvs_fuse(mvs_ctx* c, const int32_t* has_depth, const int32_t* order_off, const int32_t* order, double cell,
+             const mvs_options_t* opts, mvs_point_t** pts, int64_t* n) {
+    if (!c || !opts || !has_depth) return fail(MVS_ERR_ARG, "mvs_fuse: NULL argument");
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        std::vector<const float*> dm(c->views.size(), nullptr);
+        for (size_t i = 0; i < c->views.size(); i++) {
+            if (has_depth[i]) {
+                if (!c->views[i].has_depth) return fail(MVS_ERR_STATE, "mvs_fuse: view " + std::to_string(i) + " has no depth map");
+                dm[i] = c->views[i].depth.p;
+            }
+        }
+        return fuse_common(c, dm, order_off, order, cell, opts, pts, n);
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_fuse: ") + e.what());
+    }
+}
+
+int mvs_fuse_depth_maps(mvs_ctx* c, const float* const* depth_maps, const int32_t* order_off, const int32_t* order,
+                        double cell, const mvs_options_t* opts, mvs_point_t** pts, int64_t* n) {
+    if (!c || !opts || !depth_maps) return fail(MVS_ERR_ARG, "mvs_fuse_depth_maps: NULL argument");
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        const int nv = (int)c->views.size();
+        size_t total = 0;
+        for (int i = 0; i < nv; i++)
+            if (depth_maps[i]) total += (size_t)c->views[i].w * c->views[i].h;
+        HCHK(c->tmp_depth.need(total));
+        std::vector<const float*> dm(nv, nullptr);
+        size_t off = 0;
+        for (int i = 0; i < nv; i++) {
+            if (!depth_maps[i]) continue;
+            const size_t np = (size_t)c->views[i].w * c->views[i].h;
+            HCHK(hipMemcpyAsync(c->tmp_depth.p + off, depth_maps[i], np * 4, hipMemcpyHostToDevice, c->stream));
+            dm[i] = c->tmp_depth.p + off;
+            off += np;
+        }
+        return fuse_common(c, dm, order_off, order, cell, opts, pts, n);
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_fuse_depth_maps: ") + e.what());
+    }
+}
+
+void mvs_free_points(mvs_point_t* p) { free(p); }
+
+double mvs_fusion_cell(const double* xyz, int64_t n, double voxel_rel) {
+    if (!xyz || n <= 0) return -1.0;
+    std::vector<double> ax((size_t)n);
+    double ext[3];
+    for (int a = 0; a < 3; a++) {
+        for (int64_t i = 0; i < n; i++) ax[i] = xyz[3 * i + a];
+        std::sort(ax.begin(), ax.end());
+        // `(len as f64 * q) as usize`, clamped to len - 1
+        const double fl = (double)n * 0.02, fh = (double)n * 0.98;
+        uint64_t il = fl > 0 ? (uint64_t)fl : 0, ih = fh > 0 ? (uint64_t)fh : 0;
+        il = std::min<uint64_t>(il, (uint64_t)n - 1);
+        ih = std::min<uint64_t>(ih, (uint64_t)n - 1);
+        ext[a] = ax[ih] - ax[il];
+    }
+    const double diag = sqrt(ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2]);
+    const double cell = diag * voxel_rel;
+    return cell > 1e-9 ? cell : 1e-9;
+}
+
+int mvs_write_ply(const char* path, const mvs_point_t* pts, int64_t n) {
+    if (!path || (n > 0 && !pts)) return fail(MVS_ERR_ARG, "mvs_write_ply: bad arguments");
+    FILE* fp = fopen(path, "wb");
+    if (!fp) return fail(MVS_ERR_ARG, std::string("mvs_write_ply: cannot create ") + path);
+    fprintf(fp,
+            "ply\nformat binary_little_endian 1.0\ncomment rustcolmap dense MVS fusion\n"
+            "element vertex %lld\nproperty float x\nproperty float y\nproperty float z\n"
+            "property uchar red\nproperty uchar green\nproperty uchar blue\nend_header\n",
+            (long long)n);
+    std::vector<uint8_t> buf;
+    buf.reserve((size_t)std::min<int64_t>(n, 1 << 20) * 15);
+    for (int64_t i = 0; i < n; i++) {
+        const float xyz[3] = {(float)pts[i].x, (float)pts[i].y, (float)pts[i].z};
+        const size_t o = buf.size();
+        buf.resize(o + 15);
+        memcpy(&buf[o], xyz, 12);
+        buf[o + 12] = pts[i].r;
+        buf[o + 13] = pts[i].g;
+        buf[o + 14] = pts[i].b;
+        if (buf.size() >= ((size_t)1 << 20) * 15) {
+            fwrite(buf.data(), 1, buf.size(), fp);
+            buf.clear();
+        }
+    }
+    if (!buf.empty()) fwrite(buf.data(), 1, buf.size(), fp);
+    if (fclose(fp) != 0) return fail(MVS_ERR_ARG, "mvs_write_ply: write failed");
+    return MVS_OK;
+}
+
+int mvs_get_stats(mvs_ctx* c, mvs_stats_t* out) {
+    if (!c || !out) return fail(MVS_ERR_ARG, "mvs_get_stats: NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    *out = c->stats;
+    return MVS_OK;
+}
+
+// --------------------------------------------------------------------------
+// instrumentation
+// --------------------------------------------------------------------------
+
+int mvs_debug_plane_cost(mvs_ctx* c, int32_t ref, const int32_t* srcs, int32_t k, const float* q, int32_t n,
+                         float* cost, int32_t* ham, int32_t* ninv, float* peak) {
+    if (!c || !srcs || k < 1 || !q || n < 0 || !cost || !ham || !ninv || !peak)
+        return fail(MVS_ERR_ARG, "mvs_debug_plane_cost: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (ref < 0 || ref >= (int)c->views.size()) return fail(MVS_ERR_ARG, "mvs_debug_plane_cost: bad ref");
+    for (int s = 0; s < k; s++)
+        if (srcs[s] < 0 || srcs[s] >= (int)c->views.size()) return fail(MVS_ERR_ARG, "mvs_debug_plane_cost: bad source");
+    const ViewState& rv = c->views[ref];
+    for (int i = 0; i < n; i++) {
+        const int px = (int)q[7 * i], py = (int)q[7 * i + 1];
+        if (px < 0 || py < 0 || px >= rv.w || py >= rv.h) return fail(MVS_ERR_ARG, "mvs_debug_plane_cost: pixel outside view");
+    }
+    if (int r = set_device(c)) return r;
+    std::vector<SrcDesc> S;
+    for (int s = 0; s < k; s++) S.push_back(make_src(rv, c->views[srcs[s]]));
+    JobDesc J{};
+    J.census = rv.census.p;
+    J.rays = rv.rays.p;
+    J.w = rv.w;
+    J.h = rv.h;
+    J.nsrc = k;
+    const size_t bytes = S.size() * sizeof(SrcDesc) + (size_t)n * (7 * 4 + 16) + 256;
+    HCHK(c->dbg.need(bytes));
+    uint8_t* base = c->dbg.p;
+    SrcDesc* dS = (SrcDesc*)base;
+    size_t o = (S.size() * sizeof(SrcDesc) + 255) & ~(size_t)255;
+    float* dq = (float*)(base + o);
+    o += (size_t)n * 7 * 4;
+    float* dc = (float*)(base + o);
+    o += (size_t)n * 4;
+    int* dh = (int*)(base + o);
+    o += (size_t)n * 4;
+    int* di = (int*)(base + o);
+    o += (size_t)n * 4;
+    float* dp = (float*)(base + o);
+    HCHK(hipMemcpyAsync(dS, S.data(), S.size() * sizeof(SrcDesc), hipMemcpyHostToDevice, c->stream));
+    HCHK(hipMemcpyAsync(dq, q, (size_t)n * 7 * 4, hipMemcpyHostToDevice, c->stream));
+    HCHK(launch_debug_cost(J, dS, dq, n, dc, dh, di, dp, c->stream));
+    HCHK(hipMemcpyAsync(cost, dc, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipMemcpyAsync(ham, dh, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipMemcpyAsync(ninv, di, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipMemcpyAsync(peak, dp, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+    return MVS_OK;
+}
+
+int mvs_debug_rng(mvs_ctx* c, const int32_t* xy, const uint64_t* salt, int32_t n, int32_t ndraw, uint64_t* seed_out,
+                  float* uniform_out) {
+    if (!c || !xy || !salt || n < 0 || ndraw < 0 || !seed_out || (ndraw > 0 && !uniform_out))
+        return fail(MVS_ERR_ARG, "mvs_debug_rng: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int r = set_device(c)) return r;
+    const size_t bytes = (size_t)n * (8 + 8 + 8) + (size_t)n * ndraw * 4 + 64;
+    HCHK(c->dbg.need(bytes));
+    uint8_t* base = c->dbg.p;
+    int* dxy = (int*)base;
+    u64* ds = (u64*)(base + (((size_t)n * 8 + 7) & ~(size_t)7));
+    u64* dseed = ds + n;
+    float* du = (float*)(dseed + n);
+    HCHK(hipMemcpyAsync(dxy, xy, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    HCHK(hipMemcpyAsync(ds, salt, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    HCHK(launch_debug_rng(dxy, ds, n, ndraw, dseed, du, c->stream));
+    HCHK(hipMemcpyAsync(seed_out, dseed, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (ndraw > 0) HCHK(hipMemcpyAsync(uniform_out, du, (size_t)n * ndraw * 4, hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+    return MVS_OK;
+}
+
+}  // extern "C"

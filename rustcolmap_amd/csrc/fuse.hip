@@ -1,0 +1,402 @@
+// fuse.hip -- geometric-consistency fusion and voxel dedup on gfx950.
+//
+// Reference: mvs/src/fuse.rs:32-215 (sample_depth, fuse_depth_maps).
+//
+// Stages (all on the context's stream):
+//   1. candidates: one thread per pixel of every view that has a depth map
+//      (global index = view-major raster order, the reference's merge
+//      order); f64 back-projection, projection into every view of its
+//      fusion order, f32 bilinear depth sample, relative agreement test,
+//      average.  Writes a flag, the f64 point and the colour per pixel.
+//   2. compaction of flagged pixels (order preserving).
+//   3. voxel keys floor(p / cell) as i64 and their per-axis range.
+//   4. integer voxel hash: 63-bit packed keys (21 bits per axis, relative to
+//      the per-axis minimum), open addressing with 64-bit atomicCAS; each
+//      candidate learns its slot.
+//   5. stable radix sort of (slot, candidate) pairs: candidates of one
+//      voxel become contiguous and stay in reference order, so the f64
+//      coordinate sums and the wrapping u16 colour sums are accumulated in
+//      exactly the reference's sequence (fuse.rs:178-201).
+//   6. one thread per voxel run: sums, averages, round() as u8.
+//   7. compaction of voxel results by their first candidate index: output
+//      in first-insertion order.
+#include <hipcub/hipcub.hpp>
+
+#include "stereo_kernels.h"
+
+namespace mvs {
+
+struct DevPoint {
+    double x, y, z;
+    uint8_t r, g, b, pad[5];
+};
+
+static constexpr u64 EMPTY_KEY = ~0ULL;
+
+// fuse.rs:32-62
+__device__ __forceinline__ double sample_depth(const float* __restrict__ depth, int w, int h, double u, double v) {
+    const int32_t u0 = f64_to_i32_sat(floor(u));
+    const int32_t v0 = f64_to_i32_sat(floor(v));
+    if (u0 < 0 || v0 < 0 || (long long)u0 + 1 >= w || (long long)v0 + 1 >= h) return 0.0;
+    const float fx = (float)(u - (double)u0);
+    const float fy = (float)(v - (double)v0);
+    const size_t r0 = (size_t)v0 * w + u0, r1 = (size_t)(v0 + 1) * w + u0;
+    const float d00 = depth[r0], d10 = depth[r0 + 1], d01 = depth[r1], d11 = depth[r1 + 1];
+    const float w00 = (1.0f - fx) * (1.0f - fy), w10 = fx * (1.0f - fy), w01 = (1.0f - fx) * fy, w11 = fx * fy;
+    float acc = 0.0f, wsum = 0.0f;
+    if (d00 > 0.0f) { acc += d00 * w00; wsum += w00; }
+    if (d10 > 0.0f) { acc += d10 * w10; wsum += w10; }
+    if (d01 > 0.0f) { acc += d01 * w01; wsum += w01; }
+    if (d11 > 0.0f) { acc += d11 * w11; wsum += w11; }
+    return wsum > 1e-6f ? (double)(acc / wsum) : 0.0;
+}
+
+// fuse.rs:98-163 for one pixel.
+__global__ __launch_bounds__(256) void cand_kernel(const FuseView* __restrict__ V, int nv,
+                                                   const int32_t* __restrict__ order, int64_t n_pix, int min_cons,
+                                                   double depth_diff, uint8_t* __restrict__ flag,
+                                                   double* __restrict__ xyz, uchar4* __restrict__ rgb) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_pix) return;
+    int lo = 0, hi = nv - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (V[mid].pix_off <= g) lo = mid; else hi = mid - 1;
+    }
+    const FuseView& v = V[lo];
+    const int64_t li = g - v.pix_off;
+    const int px = (int)(li % v.w), py = (int)(li / v.w);
+    uint8_t ok = 0;
+    const double zr = (double)v.depth[li];
+    if (zr > 0.0) {
+        double nx, ny;
+        cam_normalize(v.f, v.cx, v.cy, v.k, (double)px, (double)py, nx, ny);
+        const double dr[3] = {nx * zr - v.t[0], ny * zr - v.t[1], 1.0 * zr - v.t[2]};
+        double xw[3];
+        mat_t_vec(v.R, dr, xw);
+        double acc[3] = {xw[0], xw[1], xw[2]};
+        int ncons = 0;
+        for (int q = 0; q < v.ord_n; ++q) {
+            const FuseView& s = V[order[v.ord_off + q]];
+            if (!s.depth) continue;
+            double xs[3];
+            mat_vec(s.R, xw, xs);
+            xs[0] = xs[0] + s.t[0];
+            xs[1] = xs[1] + s.t[1];
+            xs[2] = xs[2] + s.t[2];
+            if (xs[2] <= 1e-6) continue;
+            // Pose::project (geom.rs:63-76): the same x_cam, so z > 1e-9 holds
+            const double xn = xs[0] / xs[2], yn = xs[1] / xs[2];
+            const double r2 = xn * xn + yn * yn;
+            const double dd = 1.0 + s.k * r2;
+            const double us = s.f * xn * dd + s.cx, vs = s.f * yn * dd + s.cy;
+            const double zs = sample_depth(s.depth, s.w, s.h, us, vs);
+            if (zs <= 0.0) continue;
+            const double rel = fabs((xs[2] - zs) / xs[2]);
+            if (rel < depth_diff) {
+                double sx, sy;
+                cam_normalize(s.f, s.cx, s.cy, s.k, us, vs, sx, sy);
+                const double d2[3] = {sx * zs - s.t[0], sy * zs - s.t[1], 1.0 * zs - s.t[2]};
+                double xw2[3];
+                mat_t_vec(s.R, d2, xw2);
+                acc[0] = acc[0] + xw2[0];
+                acc[1] = acc[1] + xw2[1];
+                acc[2] = acc[2] + xw2[2];
+                ++ncons;
+            }
+        }
+        if (ncons >= min_cons) {
+            const double m = (double)(ncons + 1);
+            xyz[3 * g] = acc[0] / m;
+            xyz[3 * g + 1] = acc[1] / m;
+            xyz[3 * g + 2] = acc[2] / m;
+            uchar4 c;
+            if (v.rgb) {
+                c.x = v.rgb[3 * li];
+                c.y = v.rgb[3 * li + 1];
+                c.z = v.rgb[3 * li + 2];
+            } else {
+                c.x = c.y = c.z = v.gray[li];
+            }
+            c.w = 0;
+            rgb[g] = c;
+            ok = 1;
+        }
+    }
+    flag[g] = ok;
+}
+
+__global__ void key_init_kernel(long long* kr) {
+    if (threadIdx.x < 3) kr[threadIdx.x] = 9223372036854775807LL;
+    else if (threadIdx.x < 6) kr[threadIdx.x] = (-9223372036854775807LL - 1);
+}
+
+// fuse.rs:180-184: floor(p / cell) as i64
+__global__ __launch_bounds__(256) void key_kernel(const int32_t* __restrict__ cid, int n, const double* __restrict__ xyz,
+                                                  double cell, long long* __restrict__ key3, long long* __restrict__ kr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    long long k[3] = {0, 0, 0};
+    const bool act = i < n;
+    if (act) {
+        const int64_t g = cid[i];
+        for (int a = 0; a < 3; ++a) {
+            k[a] = f64_to_i64_sat(floor(xyz[3 * g + a] / cell));
+            key3[3 * (size_t)i + a] = k[a];
+        }
+    }
+    // wave-level min/max, then one atomic per wave and axis
+    for (int a = 0; a < 3; ++a) {
+        long long mn = act ? k[a] : 9223372036854775807LL;
+        long long mx = act ? k[a] : (-9223372036854775807LL - 1);
+        for (int off = 32; off > 0; off >>= 1) {
+            const long long omn = __shfl_xor(mn, off, 64);
+            const long long omx = __shfl_xor(mx, off, 64);
+            mn = omn < mn ? omn : mn;
+            mx = omx > mx ? omx : mx;
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&kr[a], mn);
+            atomicMax(&kr[3 + a], mx);
+        }
+    }
+}
+
+__device__ __forceinline__ u64 mix64(u64 k) {
+    k ^= k >> 33;
+    k *= 0xFF51AFD7ED558CCDULL;
+    k ^= k >> 33;
+    k *= 0xC4CEB9FE1A85EC53ULL;
+    k ^= k >> 33;
+    return k;
+}
+
+__global__ __launch_bounds__(256) void insert_kernel(const long long* __restrict__ key3, int n,
+                                                     const long long* __restrict__ kr, u64* __restrict__ table,
+                                                     u64 cap_mask, uint32_t* __restrict__ slot_of,
+                                                     int* __restrict__ err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u64 key = ((u64)(key3[3 * (size_t)i] - kr[0]) << 42) | ((u64)(key3[3 * (size_t)i + 1] - kr[1]) << 21) |
+                    (u64)(key3[3 * (size_t)i + 2] - kr[2]);
+    u64 h = mix64(key) & cap_mask;
+    for (u64 probe = 0; probe <= cap_mask; ++probe) {
+        const u64 prev = atomicCAS((unsigned long long*)&table[h], (unsigned long long)EMPTY_KEY,
+                                   (unsigned long long)key);
+        if (prev == EMPTY_KEY || prev == key) {
+            slot_of[i] = (uint32_t)h;
+            return;
+        }
+        h = (h + 1) & cap_mask;
+    }
+    atomicExch(err, 1);   // table full: impossible with cap >= 2n, reported anyway
+    slot_of[i] = 0;
+}
+
+__global__ void iota_kernel(uint32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (uint32_t)i;
+}
+
+__global__ void fill_u64_k(u64* p, size_t n, u64 v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void fill_u8_k(uint8_t* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// One thread per voxel run of the sorted pairs (fuse.rs:185-212).
+__global__ __launch_bounds__(256) void reduce_kernel(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval,
+                                                     int n, const int32_t* __restrict__ cid,
+                                                     const double* __restrict__ xyz, const uchar4* __restrict__ rgb,
+                                                     DevPoint* __restrict__ by_first, uint8_t* __restrict__ first_flag) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t key = skey[p];
+    if (p > 0 && skey[p - 1] == key) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    uint16_t r = 0, g = 0, b = 0;
+    uint32_t cnt = 0;
+    const uint32_t first = sval[p];
+    for (int q = p; q < n && skey[q] == key; ++q) {
+        const int64_t gi = cid[sval[q]];
+        sx = sx + xyz[3 * gi];
+        sy = sy + xyz[3 * gi + 1];
+        sz = sz + xyz[3 * gi + 2];
+        const uchar4 c = rgb[gi];
+        r = (uint16_t)(r + c.x);
+        g = (uint16_t)(g + c.y);
+        b = (uint16_t)(b + c.z);
+        ++cnt;
+    }
+    const double m = (double)cnt;
+    DevPoint o;
+    o.x = sx / m;
+    o.y = sy / m;
+    o.z = sz / m;
+    o.r = f64_to_u8_sat(round((double)r / m));
+    o.g = f64_to_u8_sat(round((double)g / m));
+    o.b = f64_to_u8_sat(round((double)b / m));
+    for (int k = 0; k < 5; ++k) o.pad[k] = 0;
+    by_first[first] = o;
+    first_flag[first] = 1;
+}
+
+// --------------------------------------------------------------------------
+// host driver
+// --------------------------------------------------------------------------
+
+template <class T>
+struct Buf {
+    T* p = nullptr;
+    size_t cap = 0;
+    hipError_t need(size_t n) {
+        if (p && n <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t c = n < 1024 ? 1024 : n + n / 8;
+        hipError_t e = hipMalloc((void**)&p, c * sizeof(T));
+        if (e == hipSuccess) cap = c;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct FuseScratch {
+    Buf<uint8_t> flag, first_flag, tmp;
+    Buf<double> xyz;
+    Buf<uchar4> rgb;
+    Buf<int32_t> cid, counts;   // counts: [0] n_sel, [1] n_pts, [2] err
+    Buf<long long> key3, kr;    // kr: per-axis key min (3) and max (3)
+    Buf<uint32_t> slot, iota, skey, sval;
+    Buf<DevPoint> by_first, out;
+    Buf<u64> table;
+};
+
+#define FCHK(x)                                 \
+    do {                                        \
+        hipError_t e_ = (x);                    \
+        if (e_ != hipSuccess) return e_;        \
+    } while (0)
+
+void free_fuse_scratch(FuseScratch* s) {
+    if (!s) return;
+    s->flag.release(); s->first_flag.release(); s->tmp.release(); s->xyz.release(); s->rgb.release();
+    s->cid.release(); s->counts.release(); s->key3.release(); s->kr.release(); s->slot.release();
+    s->iota.release(); s->skey.release(); s->sval.release(); s->by_first.release(); s->out.release();
+    s->table.release();
+    delete s;
+}
+
+static inline unsigned nblk(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t run_fusion(FuseScratch*& S, const FuseArgs& a, const FuseView* h_views, hipStream_t st, int64_t* n_cand,
+                      int64_t* n_pts, const void** d_points, int* limit_err) {
+    (void)h_views;
+    *limit_err = 0;
+    *n_cand = 0;
+    *n_pts = 0;
+    if (!S) S = new FuseScratch();
+    FCHK(S->kr.need(6));
+    FCHK(S->counts.need(4));
+    const size_t np = (size_t)a.n_pix;
+    if (np == 0) return hipSuccess;
+    if (np > 0x7fffffffULL) {
+        *limit_err = 1;
+        return hipSuccess;
+    }
+    FCHK(S->flag.need(np));
+    FCHK(S->xyz.need(3 * np));
+    FCHK(S->rgb.need(np));
+    FCHK(S->cid.need(np));
+    hipLaunchKernelGGL(cand_kernel, dim3(nblk(np)), dim3(256), 0, st, a.d_views, a.nviews, a.d_order, (int64_t)np,
+                       a.min_consistent, a.depth_diff, S->flag.p, S->xyz.p, S->rgb.p);
+    FCHK(hipGetLastError());
+    // compaction of candidates (order preserving)
+    hipcub::CountingInputIterator<int32_t> it0(0);
+    size_t t1 = 0;
+    FCHK(hipcub::DeviceSelect::Flagged(nullptr, t1, it0, S->flag.p, S->cid.p, S->counts.p, (int)np, st));
+    FCHK(S->tmp.need(t1));
+    t1 = S->tmp.cap;
+    FCHK(hipcub::DeviceSelect::Flagged(S->tmp.p, t1, it0, S->flag.p, S->cid.p, S->counts.p, (int)np, st));
+    int nsel = 0;
+    FCHK(hipMemcpyAsync(&nsel, S->counts.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    FCHK(hipStreamSynchronize(st));
+    *n_cand = nsel;
+    if (nsel == 0) return hipSuccess;
+    const size_t n = (size_t)nsel;
+    FCHK(S->key3.need(3 * n));
+    FCHK(S->slot.need(n));
+    FCHK(S->iota.need(n));
+    FCHK(S->skey.need(n));
+    FCHK(S->sval.need(n));
+    FCHK(S->by_first.need(n));
+    FCHK(S->first_flag.need(n));
+    FCHK(S->out.need(n));
+    hipLaunchKernelGGL(key_init_kernel, dim3(1), dim3(64), 0, st, S->kr.p);
+    hipLaunchKernelGGL(key_kernel, dim3(nblk(n)), dim3(256), 0, st, S->cid.p, (int)n, S->xyz.p, a.cell, S->key3.p,
+                       S->kr.p);
+    FCHK(hipGetLastError());
+    long long kr[6];
+    FCHK(hipMemcpyAsync(kr, S->kr.p, sizeof(kr), hipMemcpyDeviceToHost, st));
+    FCHK(hipStreamSynchronize(st));
+    for (int ax = 0; ax < 3; ++ax) {
+        // the packed key holds 21 bits per axis (2,097,151 cells of extent)
+        const unsigned long long ext = (unsigned long long)kr[3 + ax] - (unsigned long long)kr[ax];
+        if (ext >= (1ULL << 21)) {
+            *limit_err = 2;
+            return hipSuccess;
+        }
+    }
+    size_t cap = 1024;
+    int bits = 10;
+    while (cap < 2 * n) {
+        cap <<= 1;
+        ++bits;
+    }
+    FCHK(S->table.need(cap));
+    hipLaunchKernelGGL(fill_u64_k, dim3(nblk(cap)), dim3(256), 0, st, S->table.p, cap, EMPTY_KEY);
+    hipLaunchKernelGGL(fill_u64_k, dim3(1), dim3(64), 0, st, (u64*)S->counts.p, (size_t)2, 0ULL);
+    hipLaunchKernelGGL(insert_kernel, dim3(nblk(n)), dim3(256), 0, st, S->key3.p, (int)n, S->kr.p, S->table.p,
+                       (u64)(cap - 1), S->slot.p, S->counts.p + 2);
+    hipLaunchKernelGGL(iota_kernel, dim3(nblk(n)), dim3(256), 0, st, S->iota.p, (int)n);
+    FCHK(hipGetLastError());
+    // stable sort by slot: each voxel's candidates become contiguous, in reference order
+    size_t t2 = 0;
+    FCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, t2, S->slot.p, S->skey.p, S->iota.p, S->sval.p, (int)n, 0, bits,
+                                            st));
+    FCHK(S->tmp.need(t2));
+    t2 = S->tmp.cap;
+    FCHK(hipcub::DeviceRadixSort::SortPairs(S->tmp.p, t2, S->slot.p, S->skey.p, S->iota.p, S->sval.p, (int)n, 0, bits,
+                                            st));
+    hipLaunchKernelGGL(fill_u8_k, dim3(nblk(n)), dim3(256), 0, st, S->first_flag.p, n);
+    hipLaunchKernelGGL(reduce_kernel, dim3(nblk(n)), dim3(256), 0, st, S->skey.p, S->sval.p, (int)n, S->cid.p,
+                       S->xyz.p, S->rgb.p, S->by_first.p, S->first_flag.p);
+    FCHK(hipGetLastError());
+    size_t t3 = 0;
+    FCHK(hipcub::DeviceSelect::Flagged(nullptr, t3, S->by_first.p, S->first_flag.p, S->out.p, S->counts.p + 1, (int)n,
+                                       st));
+    FCHK(S->tmp.need(t3));
+    t3 = S->tmp.cap;
+    FCHK(hipcub::DeviceSelect::Flagged(S->tmp.p, t3, S->by_first.p, S->first_flag.p, S->out.p, S->counts.p + 1, (int)n,
+                                       st));
+    int cnt[3] = {0, 0, 0};
+    FCHK(hipMemcpyAsync(cnt, S->counts.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    FCHK(hipStreamSynchronize(st));
+    if (cnt[2]) {
+        *limit_err = 3;
+        return hipSuccess;
+    }
+    *n_pts = cnt[1];
+    *d_points = S->out.p;
+    return hipSuccess;
+}
+
+}  // namespace mvs

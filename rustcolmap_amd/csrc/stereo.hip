@@ -1,0 +1,533 @@
+// stereo.hip -- census, ray setup, plane-sweep init, propagation passes and
+// validity gate of the dense stereo path, written for gfx950 (wave64).
+//
+// Reference semantics: mvs/src/lib.rs:119-142 (census_map) and
+// mvs/src/stereo.rs:114-394 (RefWorkspace::new, plane_cost, estimate_depth).
+//
+// Work decomposition: every stage is one launch over a tile space that
+// concatenates the tiles of all jobs (reference views) of a batch, so views
+// of uneven size and support-set size K share a launch.  A workgroup owns
+// one TW x TH pixel tile of one job; the reference census and rays of the
+// tile plus a 2-pixel apron (the 5x5 cost window) are staged in LDS with
+// clamp-to-edge addressing, so every window tap of every candidate plane is
+// an LDS read; source-census taps are plane-dependent gathers served by
+// L1/L2.  A pass reads plane buffer `par` and writes `par ^ 1` for every
+// pixel (Jacobi, exactly the reference's double-buffered pass,
+// stereo.rs:271-371).
+#include "mvs_device.h"
+#include "stereo_kernels.h"
+
+namespace mvs {
+
+// --------------------------------------------------------------------------
+// census + rays for all views of an upload, one launch
+// --------------------------------------------------------------------------
+
+constexpr int CT_W = 32, CT_H = 8;
+
+__global__ __launch_bounds__(CT_W* CT_H) void prepare_kernel(const PrepView* __restrict__ views, int nviews) {
+    // view of this block: last view whose tile_off <= blockIdx.x (uniform)
+    int lo = 0, hi = nviews - 1;
+    const int b = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (views[mid].tile_off <= b) lo = mid; else hi = mid - 1;
+    }
+    const PrepView& V = views[lo];
+    const int w = V.w, h = V.h;
+    const int t = b - V.tile_off;
+    const int x0 = (t % V.tiles_x) * CT_W, y0 = (t / V.tiles_x) * CT_H;
+    __shared__ uint8_t s[CT_H + 6][CT_W + 6];
+    for (int i = threadIdx.x; i < (CT_H + 6) * (CT_W + 6); i += CT_W * CT_H) {
+        const int ay = i / (CT_W + 6), ax = i % (CT_W + 6);
+        const int gx = min(max(x0 - 3 + ax, 0), w - 1);
+        const int gy = min(max(y0 - 3 + ay, 0), h - 1);
+        s[ay][ax] = V.gray[(size_t)gy * w + gx];
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % CT_W, ly = threadIdx.x / CT_W;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= w || y >= h) return;
+    const uint8_t c = s[ly + 3][lx + 3];
+    u64 bits = 0;
+#pragma unroll
+    for (int dy = -3; dy <= 3; dy++)
+#pragma unroll
+        for (int dx = -3; dx <= 3; dx++) {
+            if (dx == 0 && dy == 0) continue;
+            if (s[ly + 3 + dy][lx + 3 + dx] < c) bits |= 1ULL << ((dy + 3) * 7 + (dx + 3));
+        }
+    const size_t i = (size_t)y * w + x;
+    if (V.census) V.census[i] = bits;
+    if (V.rays) {
+        double xo, yo;
+        cam_normalize(V.f, V.cx, V.cy, V.k, (double)x, (double)y, xo, yo);
+        V.rays[i] = make_float2((float)xo, (float)yo);
+    }
+}
+
+// --------------------------------------------------------------------------
+// plane_cost (stereo.rs:158-228)
+// --------------------------------------------------------------------------
+
+// Reference-window access: LDS tile with apron (production) ...
+struct LdsRef {
+    const u64* cen;
+    const float2* ray;
+    int c, pitch;
+    __device__ __forceinline__ int idx(int dy, int dx) const { return c + dy * pitch + dx; }
+};
+// ... or clamped global reads (instrumentation kernel).
+struct GlobRef {
+    const u64* cen;
+    const float2* ray;
+    int px, py, w, h;
+    __device__ __forceinline__ int idx(int dy, int dx) const {
+        const int yy = min(max(py + dy, 0), h - 1), xx = min(max(px + dx, 0), w - 1);
+        return yy * w + xx;
+    }
+};
+
+struct EvalInfo {
+    int ham, ninv;
+    float peak;
+};
+
+// Cost of plane (n, d) at the window's centre pixel.  `thr` is
+// (hint + 0.12) with hint = best + 0.02.  Abort iff some (source, row)
+// prefix average exceeds thr; with INFO the full window is evaluated and
+// the abort is decided from the peak prefix average, which is the same
+// predicate.
+template <bool INFO, class Ref>
+__device__ __forceinline__ float eval_plane(const Ref& R, const SrcDesc* __restrict__ S, int nsrc, float nx,
+                                            float ny, float nz, float d, float thr, EvalInfo* info) {
+    float sum = 0.0f;
+    int ham = 0, ninv = 0;
+    float peak = -__builtin_inff();
+    bool aborted = false;
+    for (int si = 0; si < nsrc; ++si) {
+        const SrcDesc& s = S[si];
+        const u64* __restrict__ cen = s.census;
+        const int sw = s.w, sh = s.h;
+        const float r0 = s.rs[0], r1 = s.rs[1], r2 = s.rs[2], r3 = s.rs[3], r4 = s.rs[4], r5 = s.rs[5];
+        const float r6 = s.rs[6], r7 = s.rs[7], r8 = s.rs[8];
+        const float t0 = s.ts[0], t1 = s.ts[1], t2 = s.ts[2];
+        const float f = s.f, cx = s.cx, cy = s.cy, kk = s.k;
+#pragma unroll
+        for (int dy = -2; dy <= 2; ++dy) {
+#pragma unroll
+            for (int dx = -2; dx <= 2; ++dx) {
+                const int j = R.idx(dy, dx);
+                const float2 r = R.ray[j];
+                const float denom = (nx * r.x + ny * r.y) + nz;
+                if (denom < 0.05f) { sum += 1.0f; ++ninv; continue; }
+                const float z = d / denom;
+                const float xr = z * r.x, yr = z * r.y;
+                const float xs = ((r0 * xr + r1 * yr) + r2 * z) + t0;
+                const float ys = ((r3 * xr + r4 * yr) + r5 * z) + t1;
+                const float zs = ((r6 * xr + r7 * yr) + r8 * z) + t2;
+                if (zs < 1e-3f) { sum += 1.0f; ++ninv; continue; }
+                const float xn = xs / zs, yn = ys / zs;
+                const float dist = 1.0f + kk * (xn * xn + yn * yn);
+                int u, v;
+                const bool oku = cast_in_range(f * xn * dist + cx, sw, u);
+                const bool okv = cast_in_range(f * yn * dist + cy, sh, v);
+                if (!(oku && okv)) { sum += 1.0f; ++ninv; continue; }
+                const int pc = __popcll(R.cen[j] ^ cen[(size_t)v * sw + u]);
+                ham += pc;
+                sum += pc_over_48(pc);
+            }
+            const float part = sum / ((float)(dy + 3) * 5.0f * (float)(si + 1));
+            if (INFO) {
+                peak = fmaxf(peak, part);
+                aborted = aborted || (part > thr);
+            } else if (part > thr) {
+                return 1e9f;
+            }
+        }
+    }
+    if (INFO) {
+        info->ham = ham;
+        info->ninv = ninv;
+        info->peak = peak;
+        if (aborted) return 1e9f;
+    }
+    return sum / (25.0f * (float)nsrc);
+}
+
+// --------------------------------------------------------------------------
+// tile staging
+// --------------------------------------------------------------------------
+
+template <int TW, int TH>
+__device__ __forceinline__ void stage_tile(const JobDesc& J, int x0, int y0, u64* s_cen, float2* s_ray) {
+    constexpr int PW = TW + 4, PH = TH + 4;
+    const int w = J.w, h = J.h;
+    for (int i = threadIdx.x; i < PW * PH; i += TW * TH) {
+        const int ay = i / PW, ax = i % PW;
+        const int gx = min(max(x0 - 2 + ax, 0), w - 1);
+        const int gy = min(max(y0 - 2 + ay, 0), h - 1);
+        const size_t g = (size_t)gy * w + gx;
+        s_cen[i] = J.census[g];
+        s_ray[i] = J.rays[g];
+    }
+}
+
+// XCD-aware block order: hardware hands consecutive workgroups to the 8
+// XCDs round-robin; remapping gives each XCD a contiguous run of tiles so
+// neighbouring tiles (which gather overlapping source-census lines) share
+// that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    const int x = b & 7, i = b >> 3;
+    const int per = n >> 3, rem = n & 7;
+    return x * per + min(x, rem) + i;
+}
+
+// --------------------------------------------------------------------------
+// init sweep (stereo.rs:247-269)
+// --------------------------------------------------------------------------
+
+template <int TW, int TH>
+__device__ __forceinline__ void init_tile(const JobDesc& J, const SrcDesc* __restrict__ srcs, int x0, int y0, int nh,
+                                          u64* s_cen, float2* s_ray, unsigned& ne) {
+    constexpr int PW = TW + 4;
+    const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= J.w || y >= J.h) return;
+    const LdsRef R{s_cen, s_ray, (ly + 2) * PW + (lx + 2), PW};
+    const SrcDesc* S = srcs + J.src_off;
+    const float inv_min = J.inv_min, inv_max = J.inv_max;
+    float best_c = __builtin_inff(), best_d = 0.0f;
+    for (int hi = 0; hi < nh; ++hi) {
+        const float inv = inv_min + (inv_max - inv_min) * (float)hi / (float)(nh - 1);
+        const float d = 1.0f / inv;
+        const float c = eval_plane<false>(R, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, (best_c + 0.02f) + 0.12f, nullptr);
+        ++ne;
+        if (c < best_c) { best_c = c; best_d = d; }
+    }
+    const size_t i = (size_t)y * J.w + x;
+    J.plane[0][i] = make_float4(0.0f, 0.0f, 1.0f, best_d);
+    J.cost[0][i] = fminf(best_c, 1.0f);
+}
+
+// --------------------------------------------------------------------------
+// one pass (stereo.rs:279-369)
+// --------------------------------------------------------------------------
+
+template <int TW, int TH>
+__device__ __forceinline__ void pass_tile(const JobDesc& J, const SrcDesc* __restrict__ srcs, int x0, int y0, int par,
+                                          u64 salt, float sig_inv, float sig_n, u64* s_cen, float2* s_ray,
+                                          unsigned& ne) {
+    constexpr int PW = TW + 4;
+    const int w = J.w, h = J.h;
+    const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= w || y >= h) return;
+    const int c = (ly + 2) * PW + (lx + 2);
+    const LdsRef R{s_cen, s_ray, c, PW};
+    const SrcDesc* S = srcs + J.src_off;
+    const int nsrc = J.nsrc;
+    const size_t i = (size_t)y * w + x;
+    const float4* __restrict__ prev = J.plane[par];
+    const float2 ri = s_ray[c];
+    u64 seed = pixel_rng(x, y, salt);
+    const float4 cur = prev[i];
+    const float cur_cost = J.cost[par][i];
+    float best_c = cur_cost;
+    float b0 = cur.x, b1 = cur.y, b2 = cur.z;
+    float bd = cur.w * ((b0 * ri.x + b1 * ri.y) + b2);
+    const float zlo = J.zmin * 0.5f, zhi = J.zmax * 2.0f;
+
+    // Seven candidates in the reference's order: the 4 neighbour planes
+    // L, R, U, D (stereo.rs:293-321), then 3 perturbations of the current
+    // plane (stereo.rs:324-353).  Candidate geometry never depends on
+    // costs, so one evaluation site serves all seven; each candidate's
+    // hint is the best cost found before it.
+    const float range = J.inv_max - J.inv_min;
+    for (int q = 0; q < 7; ++q) {
+        float cnx, cny, cnz, d_plane;
+        if (q < 4) {
+            const int dx = q == 0 ? -1 : (q == 1 ? 1 : 0);
+            const int dy = q == 2 ? -1 : (q == 3 ? 1 : 0);
+            const int xx = x + dx, yy = y + dy;
+            if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
+            const float4 nb = prev[(size_t)yy * w + xx];
+            const float2 rj = s_ray[c + dy * PW + dx];
+            const float a_nb = (nb.x * rj.x + nb.y * rj.y) + nb.z;
+            if (a_nb < 0.1f) continue;
+            d_plane = nb.w * a_nb;
+            const float a_here = (nb.x * ri.x + nb.y * ri.y) + nb.z;
+            if (a_here < 0.1f) continue;
+            const float zc = d_plane / a_here;
+            if (!(zlo <= zc && zc <= zhi)) continue;
+            cnx = nb.x;
+            cny = nb.y;
+            cnz = nb.z;
+        } else {
+            const float inv = 1.0f / fmaxf(cur.w, 1e-6f);
+            const float g0 = gauss3(seed);
+            const float inv_new = inv + sig_inv * range * g0;
+            if (inv_new <= 0.0f) continue;
+            const float zc = 1.0f / inv_new;
+            if (!(zlo <= zc && zc <= zhi)) continue;
+            const float g1 = gauss3(seed);
+            float nnx = cur.x + sig_n * g1;
+            const float g2 = gauss3(seed);
+            float nny = cur.y + sig_n * g2;
+            const float g3 = gauss3(seed);
+            float nnz = cur.z + sig_n * g3;
+            const float len = sqrtf((nnx * nnx + nny * nny) + nnz * nnz);
+            if (len < 1e-6f) continue;
+            nnx = nnx / len;
+            nny = nny / len;
+            nnz = nnz / len;
+            const float a_here = (nnx * ri.x + nny * ri.y) + nnz;
+            if (a_here < 0.1f) continue;
+            d_plane = zc * a_here;
+            cnx = nnx;
+            cny = nny;
+            cnz = nnz;
+        }
+        const float cst = eval_plane<false>(R, S, nsrc, cnx, cny, cnz, d_plane, (best_c + 0.02f) + 0.12f, nullptr);
+        ++ne;
+        if (cst < best_c) { best_c = cst; b0 = cnx; b1 = cny; b2 = cnz; bd = d_plane; }
+    }
+
+    // accept on strict improvement (stereo.rs:355-367); otherwise carry over
+    float4 outp = cur;
+    float outc = cur_cost;
+    if (best_c < cur_cost) {
+        const float a_here = (b0 * ri.x + b1 * ri.y) + b2;
+        if (a_here > 0.05f) {
+            outp = make_float4(b0, b1, b2, bd / a_here);
+            outc = fminf(best_c, 1.0f);
+        }
+    }
+    J.plane[par ^ 1][i] = outp;
+    J.cost[par ^ 1][i] = outc;
+}
+
+// --------------------------------------------------------------------------
+// kernels: tile grid (one block per tile) and persistent tile queue
+// --------------------------------------------------------------------------
+
+template <int TW, int TH>
+__device__ __forceinline__ void block_count(unsigned ne, unsigned* s_cnt, u64* counter) {
+    atomicAdd(s_cnt, ne);
+    __syncthreads();
+    if (threadIdx.x == 0 && counter) atomicAdd(counter, (u64)*s_cnt);
+}
+
+template <int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void init_kernel(const JobDesc* __restrict__ jobs, const SrcDesc* __restrict__ srcs,
+                                                      const int4* __restrict__ tiles, int ntiles, int nh,
+                                                      u64* __restrict__ counter) {
+    __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
+    __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
+    __shared__ unsigned s_cnt;
+    const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
+    const JobDesc& J = jobs[tl.x];
+    if (threadIdx.x == 0) s_cnt = 0;
+    stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
+    __syncthreads();
+    unsigned ne = 0;
+    init_tile<TW, TH>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne);
+    block_count<TW, TH>(ne, &s_cnt, counter);
+}
+
+template <int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void pass_kernel(const JobDesc* __restrict__ jobs, const SrcDesc* __restrict__ srcs,
+                                                      const int4* __restrict__ tiles, int ntiles, int par, u64 salt,
+                                                      float sig_inv, float sig_n, u64* __restrict__ counter) {
+    __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
+    __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
+    __shared__ unsigned s_cnt;
+    const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
+    const JobDesc& J = jobs[tl.x];
+    if (threadIdx.x == 0) s_cnt = 0;
+    stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
+    __syncthreads();
+    unsigned ne = 0;
+    pass_tile<TW, TH>(J, srcs, tl.y, tl.z, par, salt, sig_inv, sig_n, s_cen, s_ray, ne);
+    block_count<TW, TH>(ne, &s_cnt, counter);
+}
+
+// Persistent form: a grid sized to the machine pulls tiles from an atomic
+// queue head (queue[0]); every block leaves once the head passes ntiles.
+template <int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void pass_persistent_kernel(const JobDesc* __restrict__ jobs,
+                                                                 const SrcDesc* __restrict__ srcs,
+                                                                 const int4* __restrict__ tiles, int ntiles, int par,
+                                                                 u64 salt, float sig_inv, float sig_n,
+                                                                 u64* __restrict__ counter, int* __restrict__ queue) {
+    __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
+    __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
+    __shared__ unsigned s_cnt;
+    __shared__ int s_tile;
+    if (threadIdx.x == 0) s_cnt = 0;
+    unsigned ne = 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(queue, 1);
+        __syncthreads();
+        const int t = s_tile;
+        if (t >= ntiles) break;
+        const int4 tl = tiles[t];
+        const JobDesc& J = jobs[tl.x];
+        stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
+        __syncthreads();
+        pass_tile<TW, TH>(J, srcs, tl.y, tl.z, par, salt, sig_inv, sig_n, s_cen, s_ray, ne);
+        __syncthreads();   // LDS tile and s_tile are reused by the next iteration
+    }
+    block_count<TW, TH>(ne, &s_cnt, counter);
+}
+
+// --------------------------------------------------------------------------
+// gate (stereo.rs:373-393) + outputs
+// --------------------------------------------------------------------------
+
+template <int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void gate_kernel(const JobDesc* __restrict__ jobs, const int4* __restrict__ tiles,
+                                                      int par, float cost_thr) {
+    const int4 tl = tiles[blockIdx.x];
+    const JobDesc& J = jobs[tl.x];
+    const int x = tl.y + threadIdx.x % TW, y = tl.z + threadIdx.x / TW;
+    if (x >= J.w || y >= J.h) return;
+    const size_t i = (size_t)y * J.w + x;
+    const float4 p = J.plane[par][i];
+    const float cst = J.cost[par][i];
+    const int tex = __popcll(J.census[i]);
+    const bool textured = tex >= 4 && tex <= 44;
+    const bool ok = textured && cst < cost_thr && p.w > J.zmin * 0.5f && p.w < J.zmax * 2.0f;
+    J.depth_out[i] = ok ? p.w : 0.0f;
+    J.cost_out[i] = cst;
+    if (J.normal_out) {
+        J.normal_out[3 * i] = p.x;
+        J.normal_out[3 * i + 1] = p.y;
+        J.normal_out[3 * i + 2] = p.z;
+    }
+}
+
+__global__ void fill_u64_kernel(u64* p, int n, u64 v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// --------------------------------------------------------------------------
+// instrumentation kernels
+// --------------------------------------------------------------------------
+
+__global__ void debug_cost_kernel(JobDesc J, const SrcDesc* __restrict__ S, const float* __restrict__ q, int n,
+                                  float* cost, int* ham, int* ninv, float* peak) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float* e = q + 7 * t;
+    const int px = (int)e[0], py = (int)e[1];
+    const GlobRef R{J.census, J.rays, px, py, J.w, J.h};
+    EvalInfo info;
+    cost[t] = eval_plane<true>(R, S, J.nsrc, e[2], e[3], e[4], e[5], e[6] + 0.12f, &info);
+    ham[t] = info.ham;
+    ninv[t] = info.ninv;
+    peak[t] = info.peak;
+}
+
+__global__ void debug_rng_kernel(const int* xy, const u64* salt, int n, int ndraw, u64* seed_out, float* uni) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    u64 s = pixel_rng(xy[2 * t], xy[2 * t + 1], salt[t]);
+    seed_out[t] = s;
+    for (int k = 0; k < ndraw; k++) uni[(size_t)t * ndraw + k] = uniform01(s);
+}
+
+// --------------------------------------------------------------------------
+// launchers
+// --------------------------------------------------------------------------
+
+hipError_t launch_prepare(const PrepView* d_views, int nviews, int nblocks, hipStream_t st) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(prepare_kernel, dim3(nblocks), dim3(CT_W * CT_H), 0, st, d_views, nviews);
+    return hipGetLastError();
+}
+
+int prepare_tiles(int w, int h, int& tx) {
+    tx = (w + CT_W - 1) / CT_W;
+    return tx * ((h + CT_H - 1) / CT_H);
+}
+
+hipError_t launch_fill_u64(u64* p, int n, u64 v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+#define TILE_CASES(X) X(16, 16) X(32, 8) X(16, 8) X(8, 8)
+
+hipError_t launch_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles,
+                       int nh, u64* counter, hipStream_t st) {
+#define X(W, H)                                                                                       \
+    if (cfg.tw == W && cfg.th == H) {                                                                 \
+        hipLaunchKernelGGL((init_kernel<W, H>), dim3(ntiles), dim3(W * H), 0, st, jobs, srcs, tiles, ntiles, nh, \
+                           counter);                                                                  \
+        return hipGetLastError();                                                                     \
+    }
+    TILE_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_pass(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles,
+                       int par, u64 salt, float sig_inv, float sig_n, u64* counter, int* queue, int grid_persistent,
+                       hipStream_t st) {
+#define X(W, H)                                                                                                   \
+    if (cfg.tw == W && cfg.th == H) {                                                                             \
+        if (cfg.persistent) {                                                                                     \
+            hipLaunchKernelGGL((pass_persistent_kernel<W, H>), dim3(grid_persistent), dim3(W * H), 0, st, jobs, srcs, \
+                               tiles, ntiles, par, salt, sig_inv, sig_n, counter, queue);                         \
+        } else {                                                                                                  \
+            hipLaunchKernelGGL((pass_kernel<W, H>), dim3(ntiles), dim3(W * H), 0, st, jobs, srcs, tiles, ntiles, par, \
+                               salt, sig_inv, sig_n, counter);                                                    \
+        }                                                                                                         \
+        return hipGetLastError();                                                                                 \
+    }
+    TILE_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_gate(const TileCfg& cfg, const JobDesc* jobs, const int4* tiles, int ntiles, int par,
+                       float cost_thr, hipStream_t st) {
+#define X(W, H)                                                                                                  \
+    if (cfg.tw == W && cfg.th == H) {                                                                            \
+        hipLaunchKernelGGL((gate_kernel<W, H>), dim3(ntiles), dim3(W * H), 0, st, jobs, tiles, par, cost_thr);    \
+        return hipGetLastError();                                                                                \
+    }
+    TILE_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+int persistent_blocks_per_cu(const TileCfg& cfg) {
+    int nb = 0;
+#define X(W, H)                                                                                          \
+    if (cfg.tw == W && cfg.th == H) {                                                                    \
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pass_persistent_kernel<W, H>, W * H, 0); \
+    }
+    TILE_CASES(X)
+#undef X
+    return nb > 0 ? nb : 1;
+}
+
+hipError_t launch_debug_cost(const JobDesc& J, const SrcDesc* S, const float* q, int n, float* cost, int* ham,
+                             int* ninv, float* peak, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(debug_cost_kernel, dim3((n + 63) / 64), dim3(64), 0, st, J, S, q, n, cost, ham, ninv, peak);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_rng(const int* xy, const u64* salt, int n, int ndraw, u64* seed_out, float* uni,
+                            hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(debug_rng_kernel, dim3((n + 63) / 64), dim3(64), 0, st, xy, salt, n, ndraw, seed_out, uni);
+    return hipGetLastError();
+}
+
+}  // namespace mvs

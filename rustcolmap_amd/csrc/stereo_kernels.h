@@ -1,0 +1,56 @@
+// stereo_kernels.h -- host-visible launchers of the stereo and fusion kernels.
+#pragma once
+#include "mvs_device.h"
+
+namespace mvs {
+
+// One view of an upload: census + rays are produced for it.
+struct PrepView {
+    const uint8_t* gray;
+    u64* census;
+    float2* rays;
+    int32_t w, h;
+    double f, cx, cy, k;
+    int32_t tiles_x, tile_off;
+};
+
+// Launch shape of the stereo stages (picked per shape class by the tuner).
+struct TileCfg {
+    int tw = 16, th = 16;
+    int persistent = 0;   // 1: persistent grid over an atomic tile queue
+};
+
+int prepare_tiles(int w, int h, int& tiles_x);
+hipError_t launch_prepare(const PrepView* d_views, int nviews, int nblocks, hipStream_t st);
+hipError_t launch_fill_u64(u64* p, int n, u64 v, hipStream_t st);
+hipError_t launch_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles,
+                       int nh, u64* counter, hipStream_t st);
+hipError_t launch_pass(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles,
+                       int par, u64 salt, float sig_inv, float sig_n, u64* counter, int* queue, int grid_persistent,
+                       hipStream_t st);
+hipError_t launch_gate(const TileCfg& cfg, const JobDesc* jobs, const int4* tiles, int ntiles, int par,
+                       float cost_thr, hipStream_t st);
+int persistent_blocks_per_cu(const TileCfg& cfg);
+hipError_t launch_debug_cost(const JobDesc& J, const SrcDesc* S, const float* q, int n, float* cost, int* ham,
+                             int* ninv, float* peak, hipStream_t st);
+hipError_t launch_debug_rng(const int* xy, const u64* salt, int n, int ndraw, u64* seed_out, float* uni,
+                            hipStream_t st);
+
+// fusion (fuse.hip)
+struct FuseScratch;
+struct FuseArgs {
+    const FuseView* d_views;   // nviews
+    int nviews;
+    const int32_t* d_order;    // concatenated fusion orders
+    int64_t n_pix;             // candidate index space (pixels of views with depth)
+    int min_consistent;
+    double depth_diff;
+    double cell;
+};
+// Runs fusion; points (first-insertion order) are written to a device buffer
+// owned by the scratch; returns the counts through n_cand / n_pts.
+hipError_t run_fusion(FuseScratch*& scratch, const FuseArgs& a, const FuseView* h_views, hipStream_t st,
+                      int64_t* n_cand, int64_t* n_pts, const void** d_points, int* limit_err);
+void free_fuse_scratch(FuseScratch* s);
+
+}  // namespace mvs

@@ -1,0 +1,54 @@
+// tuner.h -- launch-shape table for the stereo stages.
+//
+// A shape class is (longest image side bucket, K bucket).  The table starts
+// from built-in defaults and is overwritten by mvs_autotune, which times the
+// candidate tile shapes / schedules on the live device for the batch it is
+// given.  MVS_TILE="WxH[p]" in the environment pins a shape (p = persistent).
+#pragma once
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <map>
+#include <utility>
+
+#include "stereo_kernels.h"
+
+namespace mvs {
+
+struct Tuner {
+    std::map<std::pair<int, int>, TileCfg> table;
+    bool forced = false;
+    TileCfg force_cfg;
+
+    static std::pair<int, int> shape_class(int maxw, int maxh, int maxk) {
+        const int side = maxw > maxh ? maxw : maxh;
+        const int sb = side <= 512 ? 0 : (side <= 1024 ? 1 : (side <= 2048 ? 2 : 3));
+        const int kb = maxk <= 4 ? 0 : (maxk <= 8 ? 1 : 2);
+        return {sb, kb};
+    }
+
+    static bool parse(const char* s, TileCfg& c) {
+        int w = 0, h = 0;
+        char p = 0;
+        const int n = sscanf(s, "%dx%d%c", &w, &h, &p);
+        if (n < 2) return false;
+        c.tw = w;
+        c.th = h;
+        c.persistent = (n == 3 && p == 'p') ? 1 : 0;
+        return true;
+    }
+
+    TileCfg pick(int maxw, int maxh, int maxk, int njobs) const {
+        (void)njobs;
+        if (forced) return force_cfg;
+        const char* env = getenv("MVS_TILE");
+        TileCfg c;
+        if (env && parse(env, c)) return c;
+        auto it = table.find(shape_class(maxw, maxh, maxk));
+        if (it != table.end()) return it->second;
+        return TileCfg();   // 16x16, tile grid
+    }
+};
+
+}  // namespace mvs

@@ -1,0 +1,246 @@
+"""GPU parity of the HIP path against the CPU oracle (bit-exact).
+
+Integer, index and float work of the stereo path is compared bit for bit:
+census codes, RNG streams, per-evaluation Hamming sums, invalid-tap counts,
+returned costs, plane buffers after every pass, gated depth / cost /
+normals and evaluation counts.  Fusion output is compared point by point
+(same first-insertion order on both sides).
+"""
+import numpy as np
+import pytest
+
+from conftest import bits, scene, view_data
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_census_bitexact(ctx):
+    import oracle
+    rng = np.random.default_rng(5)
+    for (h, w) in [(1, 1), (2, 3), (7, 7), (8, 33), (31, 64), (240, 320), (97, 131)]:
+        _say("census %dx%d" % (w, h))
+        g = rng.integers(0, 256, size=(h, w), dtype=np.uint8)
+        if h * w > 4:
+            g[: h // 2, : w // 2] = 77      # flat region
+        got = ctx.census(g)
+        exp = oracle.census(g)
+        assert np.array_equal(got, exp), (h, w)
+
+
+def test_view_census_and_upload(ctx):
+    import oracle
+    sc = scene("tiny")
+    views = view_data(sc)
+    ctx.upload_views(views)
+    for i, v in enumerate(views):
+        assert np.array_equal(ctx.view_census(i), oracle.census(v.gray))
+
+
+def test_rng_streams(ctx):
+    import oracle
+    rng = np.random.default_rng(1)
+    xy = np.concatenate([rng.integers(-5, 3000, size=(200, 2)), [[0, 0], [-1, -1], [2**31 - 1, -2**31]]])
+    salt = np.concatenate([rng.integers(0, 40, size=200), [7, 0, 2**63 + 5]]).astype(np.uint64)
+    seed, uni = ctx.debug_rng(xy, salt, 12)
+    for t in range(len(xy)):
+        s = oracle.pixel_rng(int(xy[t, 0]), int(xy[t, 1]), int(salt[t]))
+        assert int(seed[t]) == s
+        assert np.array_equal(bits(uni[t]), bits(np.asarray(oracle.uniform_stream(s, 12), np.float32)))
+
+
+def _queries(rng, w, h, n, zmin=1.0, zmax=10.0):
+    q = np.zeros((n, 7), np.float32)
+    q[:, 0] = rng.integers(0, w, n)
+    q[:, 1] = rng.integers(0, h, n)
+    nrm = rng.normal(size=(n, 3))
+    nrm[:, 2] = np.abs(nrm[:, 2]) + 0.3
+    nrm[: n // 3] = [0.0, 0.0, 1.0]
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    q[:, 2:5] = nrm
+    q[:, 5] = rng.uniform(zmin * 0.4, zmax * 2.5, n)
+    hint = rng.choice([np.inf, 0.1, 0.25, 0.4, 1.02], n)
+    q[:, 6] = hint
+    # a few planes behind the camera / grazing
+    q[-5:, 4] = -0.2
+    q[-10:-5, 5] = -1.0
+    return q
+
+
+def test_plane_cost_parity(ctx):
+    import oracle
+    for name, k in (("tiny", 2), ("tiny", 4), ("corner320", 4)):
+        sc = scene(name)
+        views = view_data(sc)
+        ctx.upload_views(views)
+        ov = oracle.OracleViews(views)
+        rng = np.random.default_rng(k)
+        from rustcolmap_amd.scene import select_sources
+        srcs_all = select_sources(sc.views, k)
+        for ref in (0, len(views) // 2):
+            _say("plane_cost %s ref %d K %d" % (name, ref, k))
+            srcs = srcs_all[ref]
+            v = views[ref]
+            q = _queries(rng, v.w, v.h, 400)
+            cost, ham, ninv, peak = ctx.debug_plane_cost(ref, srcs, q)
+            ws = ov.workspace(ref, srcs, 1.0, 10.0)
+            for t in range(len(q)):
+                c, info = ws.plane_cost(*q[t, :2].astype(int), *q[t, 2:7], info=True)
+                c0 = ws.plane_cost(*q[t, :2].astype(int), *q[t, 2:7], info=False)
+                assert np.float32(c).view(np.uint32) == np.float32(c0).view(np.uint32)
+                assert np.float32(cost[t]).view(np.uint32) == np.float32(c).view(np.uint32), (t, cost[t], c)
+                assert ham[t] == info.ham_sum and ninv[t] == info.n_invalid, t
+                assert np.float32(peak[t]).view(np.uint32) == np.float32(info.peak).view(np.uint32), t
+
+
+def _run_both(ctx, ov, ref, srcs, opts, max_passes, zmin=1.0, zmax=10.0):
+    dm, planes, ne = ctx.estimate_depth(ref, srcs, zmin, zmax, opts, max_passes=max_passes, want_planes=True)
+    d, c, n, pl, ne_o = ov.estimate_depth(ref, srcs, zmin, zmax, opts, max_passes=max_passes, want_planes=True)
+    return (dm, planes, ne), (d, c, n, pl, ne_o)
+
+
+def test_estimate_depth_every_pass_tiny(ctx):
+    import oracle
+    from rustcolmap_amd.mvs import MvsOptions
+    from rustcolmap_amd.scene import select_sources
+    sc = scene("tiny")
+    views = view_data(sc)
+    ctx.upload_views(views)
+    ov = oracle.OracleViews(views)
+    srcs = select_sources(sc.views, 3)
+    opts = MvsOptions(num_sources=3, sweeps=3)
+    for ref in range(len(views)):
+        for mp in range(0, 7):
+            _say("tiny ref %d passes %d" % (ref, mp))
+            (dm, planes, ne), (d, c, n, pl, ne_o) = _run_both(ctx, ov, ref, srcs[ref], opts, mp)
+            assert np.array_equal(bits(planes), bits(pl)), (ref, mp, np.argwhere(bits(planes) != bits(pl))[:5])
+            assert np.array_equal(bits(dm.depth), bits(d))
+            assert np.array_equal(bits(dm.cost), bits(c))
+            assert np.array_equal(bits(dm.normals), bits(n))
+            assert np.array_equal(ne[: 1 + mp], ne_o[: 1 + mp])
+
+
+def test_estimate_depth_corner320_all_views(ctx):
+    """10 views x 320x240, K=4, 6 sweeps, inverse depth [0.1, 1]."""
+    import oracle
+    from rustcolmap_amd.mvs import MvsOptions
+    from rustcolmap_amd.scene import select_sources
+    sc = scene("corner320")
+    views = view_data(sc)
+    ctx.upload_views(views)
+    ov = oracle.OracleViews(views)
+    srcs = select_sources(sc.views, 4)
+    opts = MvsOptions(num_sources=4, sweeps=6)
+    for ref in range(len(views)):
+        _say("corner320 ref %d" % ref)
+        (dm, planes, ne), (d, c, n, pl, ne_o) = _run_both(ctx, ov, ref, srcs[ref], opts, -1)
+        assert np.array_equal(bits(planes), bits(pl)), ref
+        assert np.array_equal(bits(dm.depth), bits(d)), ref
+        assert np.array_equal(bits(dm.cost), bits(c)), ref
+        assert np.array_equal(ne, ne_o), ref
+
+
+def test_batch_modes_match_single(ctx):
+    """All batch schedules (per-job launches, tile grid, persistent queue,
+    graph replay and direct launches) give the single-view results."""
+    import oracle
+    from rustcolmap_amd.mvs import MvsOptions
+    from rustcolmap_amd.scene import select_sources
+    sc = scene("corner320")
+    views = view_data(sc)
+    ctx.upload_views(views)
+    ov = oracle.OracleViews(views)
+    opts = MvsOptions(num_sources=4, sweeps=2)
+    # uneven K per job
+    ks = [2, 3, 4, 5, 2, 4, 3, 5, 4, 2]
+    jobs = []
+    for ref in range(len(views)):
+        jobs.append((ref, select_sources(sc.views, ks[ref])[ref], 1.0, 10.0))
+    exp = {}
+    for ref, s, zmin, zmax in jobs:
+        d, c, n, _, _ = ov.estimate_depth(ref, s, zmin, zmax, opts)
+        exp[ref] = (d, c, n)
+    for mode in (1, 2, 3, 6, 7, 2, 3):
+        _say("batch mode %d" % mode)
+        out = ctx.estimate_depth_batch(jobs, opts, mode=mode)
+        for ref in exp:
+            assert np.array_equal(bits(out[ref].depth), bits(exp[ref][0])), (mode, ref)
+            assert np.array_equal(bits(out[ref].cost), bits(exp[ref][1])), (mode, ref)
+            assert np.array_equal(bits(out[ref].normals), bits(exp[ref][2])), (mode, ref)
+        # resident copy equals the downloaded one
+        dm = ctx.get_depth(3)
+        assert np.array_equal(bits(dm.depth), bits(exp[3][0]))
+
+
+def test_graph_replays_with_eager_work_between(ctx):
+    import oracle
+    import torch
+    from rustcolmap_amd.mvs import MvsOptions
+    from rustcolmap_amd.scene import select_sources
+    sc = scene("tiny")
+    views = view_data(sc)
+    ctx.upload_views(views)
+    ov = oracle.OracleViews(views)
+    srcs = select_sources(sc.views, 2)
+    opts = MvsOptions(num_sources=2, sweeps=2)
+    jobs = [(r, srcs[r], 1.0, 10.0) for r in range(len(views))]
+    exp = {r: ov.estimate_depth(r, srcs[r], 1.0, 10.0, opts)[0] for r in range(len(views))}
+    for rep in range(4):
+        _say("graph replay %d" % rep)
+        x = torch.randn(1 << 20, device="cuda")
+        _ = (x * 2).sum().item()
+        torch.cuda.synchronize()
+        out = ctx.estimate_depth_batch(jobs, opts, mode=2)
+        st = ctx.stats()
+        assert st.graph_used == 1
+        for r in exp:
+            assert np.array_equal(bits(out[r].depth), bits(exp[r])), (rep, r)
+
+
+def test_fusion_parity(ctx):
+    import oracle
+    from rustcolmap_amd.mvs import MvsOptions
+    from rustcolmap_amd.scene import select_sources
+    sc = scene("corner320")
+    views = view_data(sc)
+    ctx.upload_views(views)
+    ov = oracle.OracleViews(views)
+    opts = MvsOptions(num_sources=4, sweeps=2)
+    srcs = select_sources(sc.views, 4)
+    dms = []
+    for ref in range(len(views)):
+        d, c, n, _, _ = ov.estimate_depth(ref, srcs[ref], 1.0, 10.0, opts)
+        dms.append(d)
+    dms[7] = None     # a view without a depth map
+    order = [[j for j in select_sources(sc.views, 9)[i] if dms[j] is not None] for i in range(len(views))]
+    from rustcolmap_amd.mvs import DepthMap
+    for cell, mc in ((0.01, 2), (0.004, 1), (0.05, 3)):
+        _say("fusion cell %g min_consistent %d" % (cell, mc))
+        o2 = MvsOptions(min_consistent=mc)
+        got = ctx.fuse_depth_maps([None if d is None else DepthMap(d, d) for d in dms], order, cell, o2)
+        exp, ncand, _ = ov.fuse(dms, order, mc, o2.depth_diff, cell)
+        assert ctx.stats().n_candidates == ncand
+        assert len(got) == len(exp)
+        for f in ("x", "y", "z"):
+            assert np.array_equal(bits(got[f]), bits(exp[f])), f
+        assert np.array_equal(got["rgb"], exp["rgb"])
+
+
+def test_fusion_resident_matches_host_maps(ctx):
+    from rustcolmap_amd.mvs import MvsOptions
+    from rustcolmap_amd.scene import select_sources
+    sc = scene("corner320")
+    views = view_data(sc)
+    ctx.upload_views(views)
+    opts = MvsOptions(num_sources=4, sweeps=2)
+    srcs = select_sources(sc.views, 4)
+    jobs = [(r, srcs[r], 1.0, 10.0) for r in range(len(views))]
+    out = ctx.estimate_depth_batch(jobs, opts, mode=0)
+    order = select_sources(sc.views, 9)
+    a = ctx.fuse_resident([True] * len(views), order, 0.01, opts)
+    b = ctx.fuse_depth_maps([out[r] for r in range(len(views))], order, 0.01, opts)
+    assert len(a) == len(b) and len(a) > 1000
+    assert np.array_equal(a, b)
