@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--mode", type=int, default=0, help="batch schedule (see mvs_estimate_depth_batch)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--profile", action="store_true",
+                    help="one directly launched batch plus a 1 GiB device copy (for counter calibration), no JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -78,6 +81,25 @@ def main():
     ctx = MvsContext(0)
     ctx.upload_views(views)
     ms_prepare = ctx.stats().ms_prepare
+
+    if args.profile:
+        ctx.estimate_depth_batch(jobs, opts, mode=(args.mode & 3) | 4, download=False)
+        st = ctx.stats()
+        log("profile run: passes %.1f ms, evals pass %d" % (st.ms_pass, st.n_eval_pass))
+        x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB
+        y = torch.empty_like(x)
+        x.fill_(1.0)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log("copy of 1073741824 bytes done")
+        return
+
+    tuned = None
+    if not args.no_tune:
+        t2 = time.time()
+        tuned = ctx.autotune(jobs, opts, passes=1)
+        log("autotune picked tile %dx%d persistent=%d evaluator=%d in %.1fs" % (tuned + (time.time() - t2,)))
 
     for _ in range(args.warmup):
         ctx.estimate_depth_batch(jobs, opts, mode=args.mode, download=False)
@@ -132,7 +154,9 @@ def main():
         "config": {"workload": "%s: %d views %dx%d f=%g K=%d sweeps=%d num_hyp=48 voxel=%g"
                    % (args.workload, len(views), views[0].w, views[0].h, views[0].cam.f, K, args.sweeps,
                       args.cell),
-                   "global_batch": len(views), "parallelism": "single-gpu"},
+                   "global_batch": len(views), "parallelism": "single-gpu",
+                   "launch": None if tuned is None else
+                   {"tile": "%dx%d" % tuned[:2], "persistent": tuned[2], "evaluator": tuned[3]}},
         "roofline": {"bound": "hbm", "kernel": "pass_kernel", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_launch": int(pass_bytes / n_passes),

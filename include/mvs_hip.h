@@ -136,11 +136,22 @@ int mvs_estimate_depth_ex(mvs_ctx *ctx, int32_t ref, const int32_t *srcs, int32_
  * jobs at once: one launch per stage covers every job (uneven sizes and K),
  * and the stage schedule is replayed from a captured HIP graph.  Results
  * stay resident for mvs_fuse and are copied to the jobs' host pointers
- * where those are non-NULL.  `mode`: 0 = auto, 1 = one launch per job and
- * stage (unbatched reference schedule), 2 = batched tile grid,
- * 3 = batched persistent tile queue. */
+ * where those are non-NULL.  `mode` bits 0-1: 0 = tuned schedule,
+ * 1 = one launch per job and stage (unbatched), 2 = batched tile grid,
+ * 3 = batched persistent tile queue; bit 2 (+4): launch directly instead
+ * of replaying the graph, with per-stage device timings in mvs_stats. */
 int mvs_estimate_depth_batch(mvs_ctx *ctx, const mvs_job_t *jobs, int32_t njobs, const mvs_options_t *opts,
                              int32_t mode);
+
+/* Launch-shape autotuner (no counterpart in the reference): runs the batch
+ * with every candidate (tile 16x16 / 32x8 / 16x8 / 8x8, reference-order or
+ * hoisted-geometry evaluator, tile grid or persistent queue) for `passes`
+ * passes, and records the fastest for the batch's shape class (longest
+ * side bucket x K bucket); later batches of that class use it.  Results
+ * are identical for every candidate.  best (optional, 4 ints) receives
+ * tile width, tile height, persistent flag, evaluator. */
+int mvs_autotune(mvs_ctx *ctx, const mvs_job_t *jobs, int32_t njobs, const mvs_options_t *opts, int32_t passes,
+                 int32_t *best);
 
 /* Download the resident depth / cost / normals of a view computed by the
  * last batch (NULL pointers skip). */

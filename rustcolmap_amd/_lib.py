@@ -70,6 +70,7 @@ _SIGS = {
                                         C.POINTER(MvsOptionsC), C.c_int32, _P, _P, _P, _P, _P]),
     "mvs_estimate_depth_batch": (C.c_int, [_P, C.POINTER(MvsJobC), C.c_int32, C.POINTER(MvsOptionsC), C.c_int32]),
     "mvs_get_depth": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
+    "mvs_autotune": (C.c_int, [_P, C.POINTER(MvsJobC), C.c_int32, C.POINTER(MvsOptionsC), C.c_int32, _P]),
     "mvs_fuse": (C.c_int, [_P, _P, _P, _P, C.c_double, C.POINTER(MvsOptionsC),
                            C.POINTER(C.POINTER(MvsPointC)), C.POINTER(C.c_int64)]),
     "mvs_fuse_depth_maps": (C.c_int, [_P, _P, _P, _P, C.c_double, C.POINTER(MvsOptionsC),

@@ -415,7 +415,7 @@ static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_op
     uint32_t thr_bits;
     memcpy(&thr_bits, &o->cost_thr, 4);
     P.sig.push_back(thr_bits);
-    P.sig.push_back((uint64_t)(P.cfg.tw * 1000 + P.cfg.th * 10 + P.cfg.persistent));
+    P.sig.push_back((uint64_t)(P.cfg.tw * 10000 + P.cfg.th * 100 + P.cfg.persistent * 10 + P.cfg.eval));
     P.grid_p = std::min((int)P.tiles.size(), c->n_cu * persistent_blocks_per_cu(P.cfg));
     P.sig.push_back((uint64_t)P.grid_p);
     P.sig.push_back((uint64_t)(uintptr_t)c->jobs.p);
@@ -629,6 +629,65 @@ int mvs_estimate_depth_batch(mvs_ctx* c, const mvs_job_t* jobs, int32_t njobs, c
         return run_batch(c, jobs, njobs, opts, mode, -1, nullptr, nullptr);
     } catch (const std::exception& e) {
         return fail(MVS_ERR_NOMEM, std::string("mvs_estimate_depth_batch: ") + e.what());
+    }
+}
+
+int mvs_autotune(mvs_ctx* c, const mvs_job_t* jobs, int32_t njobs, const mvs_options_t* opts, int32_t passes,
+                 int32_t* best) {
+    if (!c || !opts || !jobs || njobs < 1) return fail(MVS_ERR_ARG, "mvs_autotune: bad arguments");
+    if (passes < 1) passes = 2;
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        static const int cand[][4] = {{16, 16, 0, 1}, {16, 16, 1, 1}, {32, 8, 0, 1}, {32, 8, 1, 1}, {16, 8, 0, 1},
+                                      {16, 8, 1, 1}, {8, 8, 0, 1},   {8, 8, 1, 1},  {16, 16, 0, 0}, {16, 16, 1, 0},
+                                      {32, 8, 0, 0}, {16, 8, 0, 0}};
+        const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
+        const int n_passes = (int)std::max<uint32_t>(opts->sweeps, 1) * 2;
+        double best_t = 1e300;
+        TileCfg best_cfg;
+        int maxw = 0, maxh = 0, maxk = 0;
+        for (int j = 0; j < njobs; j++) {
+            if (jobs[j].ref < 0 || jobs[j].ref >= (int)c->views.size()) return fail(MVS_ERR_ARG, "mvs_autotune: bad ref");
+            maxw = std::max(maxw, c->views[jobs[j].ref].w);
+            maxh = std::max(maxh, c->views[jobs[j].ref].h);
+            maxk = std::max(maxk, jobs[j].k);
+        }
+        for (int i = 0; i < ncand; i++) {
+            TileCfg cf;
+            cf.tw = cand[i][0];
+            cf.th = cand[i][1];
+            cf.persistent = cand[i][2];
+            cf.eval = cand[i][3];
+            c->tuner.forced = true;
+            c->tuner.force_cfg = cf;
+            double t = 0;
+            for (int rep = 0; rep < 2; rep++) {
+                const int r = run_batch(c, jobs, njobs, opts, 4, passes, nullptr, nullptr);
+                if (r) {
+                    c->tuner.forced = false;
+                    return r;
+                }
+                t = c->stats.ms_init + c->stats.ms_pass * (double)n_passes / (double)passes;
+            }
+            if (t < best_t) {
+                best_t = t;
+                best_cfg = cf;
+            }
+        }
+        c->tuner.forced = false;
+        c->tuner.table[Tuner::shape_class(maxw, maxh, maxk)] = best_cfg;
+        c->graph.release();
+        if (best) {
+            best[0] = best_cfg.tw;
+            best[1] = best_cfg.th;
+            best[2] = best_cfg.persistent;
+            best[3] = best_cfg.eval;
+        }
+        return MVS_OK;
+    } catch (const std::exception& e) {
+        c->tuner.forced = false;
+        return fail(MVS_ERR_NOMEM, std::string("mvs_autotune: ") + e.what());
     }
 }
 

@@ -155,6 +155,121 @@ __device__ __forceinline__ float eval_plane(const Ref& R, const SrcDesc* __restr
     return sum / (25.0f * (float)nsrc);
 }
 
+// Production evaluator.  Same value as eval_plane<false>, organised for the
+// GPU:
+//   * the window geometry (denom test, z = d / denom, z*xo, z*yo) does not
+//     depend on the source, so it is computed once, while source 0 is
+//     evaluated, and kept in registers for the other sources -- the same
+//     operations on the same operands, hence the same values;
+//   * taps are branch-free: an invalid tap still issues a (clamped) gather
+//     and adds 1.0 by select, so the five gathers of a window row are in
+//     flight together;
+//   * source census reads go through the global address space.
+typedef __attribute__((address_space(1))) const u64 gu64;
+
+struct SrcP {
+    gu64* cen;
+    int sw, sh;
+    float r0, r1, r2, r3, r4, r5, r6, r7, r8, t0, t1, t2, f, cx, cy, k;
+};
+
+__device__ __forceinline__ SrcP load_src(const SrcDesc& s) {
+    SrcP p;
+    p.cen = (gu64*)s.census;
+    p.sw = s.w;
+    p.sh = s.h;
+    p.r0 = s.rs[0]; p.r1 = s.rs[1]; p.r2 = s.rs[2]; p.r3 = s.rs[3]; p.r4 = s.rs[4];
+    p.r5 = s.rs[5]; p.r6 = s.rs[6]; p.r7 = s.rs[7]; p.r8 = s.rs[8];
+    p.t0 = s.ts[0]; p.t1 = s.ts[1]; p.t2 = s.ts[2];
+    p.f = s.f; p.cx = s.cx; p.cy = s.cy; p.k = s.k;
+    return p;
+}
+
+// `(x) as i32` lands in [0, n): NaN -> 0 (valid for n > 0), else -1 < x < n.
+__device__ __forceinline__ bool cast_ok(float x, int n, int& out) {
+    const bool in = (x > -1.0f) && (x < (float)n);
+    out = (int)(in ? x : 0.0f);
+    return in || (x != x);
+}
+
+// One (window tap, source) of stereo.rs:187-213 given the window geometry.
+__device__ __forceinline__ bool tap_index(float z, float xr, float yr, bool vd, const SrcP& P, int& idx) {
+    const float xs = ((P.r0 * xr + P.r1 * yr) + P.r2 * z) + P.t0;
+    const float ys = ((P.r3 * xr + P.r4 * yr) + P.r5 * z) + P.t1;
+    const float zs = ((P.r6 * xr + P.r7 * yr) + P.r8 * z) + P.t2;
+    const float xn = xs / zs, yn = ys / zs;
+    const float dist = 1.0f + P.k * (xn * xn + yn * yn);
+    int u, v;
+    const bool oku = cast_ok(P.f * xn * dist + P.cx, P.sw, u);
+    const bool okv = cast_ok(P.f * yn * dist + P.cy, P.sh, v);
+    const bool ok = vd && !(zs < 1e-3f) && oku && okv;
+    idx = ok ? v * P.sw + u : 0;
+    return ok;
+}
+
+template <int PW>
+__device__ __forceinline__ float eval_plane_fast(const u64* __restrict__ s_cen, const float2* __restrict__ s_ray, int c,
+                                                 const SrcDesc* __restrict__ S, int nsrc, float nx, float ny, float nz,
+                                                 float d, float thr) {
+    float gz[25], gx[25], gy[25];
+    unsigned okd = 0;
+    float sum = 0.0f;
+    {
+        const SrcP P = load_src(S[0]);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            int idx[5];
+            bool ok[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const int t = r * 5 + q;
+                const float2 ray = s_ray[c + (r - 2) * PW + (q - 2)];
+                const float denom = (nx * ray.x + ny * ray.y) + nz;
+                const bool vd = !(denom < 0.05f);
+                const float z = d / denom;
+                gz[t] = z;
+                gx[t] = z * ray.x;
+                gy[t] = z * ray.y;
+                okd |= (vd ? 1u : 0u) << t;
+                ok[q] = tap_index(gz[t], gx[t], gy[t], vd, P, idx[q]);
+            }
+            u64 cs[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) cs[q] = P.cen[idx[q]];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const u64 rc = s_cen[c + (r - 2) * PW + (q - 2)];
+                sum += ok[q] ? pc_over_48(__popcll(rc ^ cs[q])) : 1.0f;
+            }
+            if (sum / ((float)(r + 1) * 5.0f) > thr) return 1e9f;
+        }
+    }
+    for (int si = 1; si < nsrc; ++si) {
+        const SrcP P = load_src(S[si]);
+        const float fs = (float)(si + 1);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            int idx[5];
+            bool ok[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const int t = r * 5 + q;
+                ok[q] = tap_index(gz[t], gx[t], gy[t], (okd >> t) & 1u, P, idx[q]);
+            }
+            u64 cs[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) cs[q] = P.cen[idx[q]];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const u64 rc = s_cen[c + (r - 2) * PW + (q - 2)];
+                sum += ok[q] ? pc_over_48(__popcll(rc ^ cs[q])) : 1.0f;
+            }
+            if (sum / ((float)(r + 1) * 5.0f * fs) > thr) return 1e9f;
+        }
+    }
+    return sum / (25.0f * (float)nsrc);
+}
+
 // --------------------------------------------------------------------------
 // tile staging
 // --------------------------------------------------------------------------
@@ -187,7 +302,7 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
 // init sweep (stereo.rs:247-269)
 // --------------------------------------------------------------------------
 
-template <int TW, int TH>
+template <int TW, int TH, int EV>
 __device__ __forceinline__ void init_tile(const JobDesc& J, const SrcDesc* __restrict__ srcs, int x0, int y0, int nh,
                                           u64* s_cen, float2* s_ray, unsigned& ne) {
     constexpr int PW = TW + 4;
@@ -201,7 +316,9 @@ __device__ __forceinline__ void init_tile(const JobDesc& J, const SrcDesc* __res
     for (int hi = 0; hi < nh; ++hi) {
         const float inv = inv_min + (inv_max - inv_min) * (float)hi / (float)(nh - 1);
         const float d = 1.0f / inv;
-        const float c = eval_plane<false>(R, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, (best_c + 0.02f) + 0.12f, nullptr);
+        const float thr = (best_c + 0.02f) + 0.12f;
+        const float c = EV ? eval_plane_fast<PW>(s_cen, s_ray, R.c, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, thr)
+                           : eval_plane<false>(R, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, thr, nullptr);
         ++ne;
         if (c < best_c) { best_c = c; best_d = d; }
     }
@@ -214,7 +331,7 @@ __device__ __forceinline__ void init_tile(const JobDesc& J, const SrcDesc* __res
 // one pass (stereo.rs:279-369)
 // --------------------------------------------------------------------------
 
-template <int TW, int TH>
+template <int TW, int TH, int EV>
 __device__ __forceinline__ void pass_tile(const JobDesc& J, const SrcDesc* __restrict__ srcs, int x0, int y0, int par,
                                           u64 salt, float sig_inv, float sig_n, u64* s_cen, float2* s_ray,
                                           unsigned& ne) {
@@ -288,7 +405,9 @@ __device__ __forceinline__ void pass_tile(const JobDesc& J, const SrcDesc* __res
             cny = nny;
             cnz = nnz;
         }
-        const float cst = eval_plane<false>(R, S, nsrc, cnx, cny, cnz, d_plane, (best_c + 0.02f) + 0.12f, nullptr);
+        const float thr = (best_c + 0.02f) + 0.12f;
+        const float cst = EV ? eval_plane_fast<PW>(s_cen, s_ray, c, S, nsrc, cnx, cny, cnz, d_plane, thr)
+                             : eval_plane<false>(R, S, nsrc, cnx, cny, cnz, d_plane, thr, nullptr);
         ++ne;
         if (cst < best_c) { best_c = cst; b0 = cnx; b1 = cny; b2 = cnz; bd = d_plane; }
     }
@@ -318,7 +437,7 @@ __device__ __forceinline__ void block_count(unsigned ne, unsigned* s_cnt, u64* c
     if (threadIdx.x == 0 && counter) atomicAdd(counter, (u64)*s_cnt);
 }
 
-template <int TW, int TH>
+template <int TW, int TH, int EV>
 __global__ __launch_bounds__(TW* TH) void init_kernel(const JobDesc* __restrict__ jobs, const SrcDesc* __restrict__ srcs,
                                                       const int4* __restrict__ tiles, int ntiles, int nh,
                                                       u64* __restrict__ counter) {
@@ -331,11 +450,11 @@ __global__ __launch_bounds__(TW* TH) void init_kernel(const JobDesc* __restrict_
     stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
     __syncthreads();
     unsigned ne = 0;
-    init_tile<TW, TH>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne);
+    init_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne);
     block_count<TW, TH>(ne, &s_cnt, counter);
 }
 
-template <int TW, int TH>
+template <int TW, int TH, int EV>
 __global__ __launch_bounds__(TW* TH) void pass_kernel(const JobDesc* __restrict__ jobs, const SrcDesc* __restrict__ srcs,
                                                       const int4* __restrict__ tiles, int ntiles, int par, u64 salt,
                                                       float sig_inv, float sig_n, u64* __restrict__ counter) {
@@ -348,13 +467,13 @@ __global__ __launch_bounds__(TW* TH) void pass_kernel(const JobDesc* __restrict_
     stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
     __syncthreads();
     unsigned ne = 0;
-    pass_tile<TW, TH>(J, srcs, tl.y, tl.z, par, salt, sig_inv, sig_n, s_cen, s_ray, ne);
+    pass_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, par, salt, sig_inv, sig_n, s_cen, s_ray, ne);
     block_count<TW, TH>(ne, &s_cnt, counter);
 }
 
 // Persistent form: a grid sized to the machine pulls tiles from an atomic
 // queue head (queue[0]); every block leaves once the head passes ntiles.
-template <int TW, int TH>
+template <int TW, int TH, int EV>
 __global__ __launch_bounds__(TW* TH) void pass_persistent_kernel(const JobDesc* __restrict__ jobs,
                                                                  const SrcDesc* __restrict__ srcs,
                                                                  const int4* __restrict__ tiles, int ntiles, int par,
@@ -375,7 +494,7 @@ __global__ __launch_bounds__(TW* TH) void pass_persistent_kernel(const JobDesc* 
         const JobDesc& J = jobs[tl.x];
         stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
         __syncthreads();
-        pass_tile<TW, TH>(J, srcs, tl.y, tl.z, par, salt, sig_inv, sig_n, s_cen, s_ray, ne);
+        pass_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, par, salt, sig_inv, sig_n, s_cen, s_ray, ne);
         __syncthreads();   // LDS tile and s_tile are reused by the next iteration
     }
     block_count<TW, TH>(ne, &s_cnt, counter);
@@ -459,15 +578,15 @@ hipError_t launch_fill_u64(u64* p, int n, u64 v, hipStream_t st) {
     return hipGetLastError();
 }
 
-#define TILE_CASES(X) X(16, 16) X(32, 8) X(16, 8) X(8, 8)
+#define TILE_CASES(X) X(16, 16, 0) X(16, 16, 1) X(32, 8, 0) X(32, 8, 1) X(16, 8, 0) X(16, 8, 1) X(8, 8, 1)
 
 hipError_t launch_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles,
                        int nh, u64* counter, hipStream_t st) {
-#define X(W, H)                                                                                       \
-    if (cfg.tw == W && cfg.th == H) {                                                                 \
-        hipLaunchKernelGGL((init_kernel<W, H>), dim3(ntiles), dim3(W * H), 0, st, jobs, srcs, tiles, ntiles, nh, \
-                           counter);                                                                  \
-        return hipGetLastError();                                                                     \
+#define X(W, H, E)                                                                                     \
+    if (cfg.tw == W && cfg.th == H && cfg.eval == E) {                                                 \
+        hipLaunchKernelGGL((init_kernel<W, H, E>), dim3(ntiles), dim3(W * H), 0, st, jobs, srcs, tiles, ntiles, \
+                           nh, counter);                                                               \
+        return hipGetLastError();                                                                      \
     }
     TILE_CASES(X)
 #undef X
@@ -477,16 +596,16 @@ hipError_t launch_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* s
 hipError_t launch_pass(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles,
                        int par, u64 salt, float sig_inv, float sig_n, u64* counter, int* queue, int grid_persistent,
                        hipStream_t st) {
-#define X(W, H)                                                                                                   \
-    if (cfg.tw == W && cfg.th == H) {                                                                             \
-        if (cfg.persistent) {                                                                                     \
-            hipLaunchKernelGGL((pass_persistent_kernel<W, H>), dim3(grid_persistent), dim3(W * H), 0, st, jobs, srcs, \
-                               tiles, ntiles, par, salt, sig_inv, sig_n, counter, queue);                         \
-        } else {                                                                                                  \
-            hipLaunchKernelGGL((pass_kernel<W, H>), dim3(ntiles), dim3(W * H), 0, st, jobs, srcs, tiles, ntiles, par, \
-                               salt, sig_inv, sig_n, counter);                                                    \
-        }                                                                                                         \
-        return hipGetLastError();                                                                                 \
+#define X(W, H, E)                                                                                                 \
+    if (cfg.tw == W && cfg.th == H && cfg.eval == E) {                                                             \
+        if (cfg.persistent) {                                                                                      \
+            hipLaunchKernelGGL((pass_persistent_kernel<W, H, E>), dim3(grid_persistent), dim3(W * H), 0, st, jobs, \
+                               srcs, tiles, ntiles, par, salt, sig_inv, sig_n, counter, queue);                    \
+        } else {                                                                                                   \
+            hipLaunchKernelGGL((pass_kernel<W, H, E>), dim3(ntiles), dim3(W * H), 0, st, jobs, srcs, tiles, ntiles, \
+                               par, salt, sig_inv, sig_n, counter);                                                \
+        }                                                                                                          \
+        return hipGetLastError();                                                                                  \
     }
     TILE_CASES(X)
 #undef X
@@ -500,16 +619,16 @@ hipError_t launch_gate(const TileCfg& cfg, const JobDesc* jobs, const int4* tile
         hipLaunchKernelGGL((gate_kernel<W, H>), dim3(ntiles), dim3(W * H), 0, st, jobs, tiles, par, cost_thr);    \
         return hipGetLastError();                                                                                \
     }
-    TILE_CASES(X)
+    X(16, 16) X(32, 8) X(16, 8) X(8, 8)
 #undef X
     return hipErrorInvalidValue;
 }
 
 int persistent_blocks_per_cu(const TileCfg& cfg) {
     int nb = 0;
-#define X(W, H)                                                                                          \
-    if (cfg.tw == W && cfg.th == H) {                                                                    \
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pass_persistent_kernel<W, H>, W * H, 0); \
+#define X(W, H, E)                                                                                           \
+    if (cfg.tw == W && cfg.th == H && cfg.eval == E) {                                                       \
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pass_persistent_kernel<W, H, E>, W * H, 0); \
     }
     TILE_CASES(X)
 #undef X

@@ -18,6 +18,7 @@ struct PrepView {
 struct TileCfg {
     int tw = 16, th = 16;
     int persistent = 0;   // 1: persistent grid over an atomic tile queue
+    int eval = 1;         // plane_cost evaluator: 0 = reference loop order, 1 = hoisted geometry
 };
 
 int prepare_tiles(int w, int h, int& tiles_x);

@@ -28,14 +28,14 @@ struct Tuner {
         return {sb, kb};
     }
 
+    // "WxH[p][r]": tile shape, p = persistent queue, r = reference-order evaluator
     static bool parse(const char* s, TileCfg& c) {
-        int w = 0, h = 0;
-        char p = 0;
-        const int n = sscanf(s, "%dx%d%c", &w, &h, &p);
-        if (n < 2) return false;
+        int w = 0, h = 0, used = 0;
+        if (sscanf(s, "%dx%d%n", &w, &h, &used) < 2) return false;
         c.tw = w;
         c.th = h;
-        c.persistent = (n == 3 && p == 'p') ? 1 : 0;
+        c.persistent = strchr(s + used, 'p') ? 1 : 0;
+        c.eval = strchr(s + used, 'r') ? 0 : 1;
         return true;
     }
 

@@ -239,6 +239,22 @@ class MvsContext:
               "mvs_estimate_depth_batch")
         return out if download else None
 
+    def autotune(self, jobs, opts: MvsOptions, passes: int = 2):
+        """Time the launch-shape candidates on `jobs` and keep the fastest
+        for their shape class.  Returns (tile_w, tile_h, persistent, evaluator)."""
+        arr = (MvsJobC * len(jobs))()
+        keep = []
+        for i, (ref, srcs, zmin, zmax) in enumerate(jobs):
+            s = np.ascontiguousarray(srcs, np.int32)
+            keep.append(s)
+            arr[i].ref, arr[i].k = int(ref), len(s)
+            arr[i].srcs = s.ctypes.data_as(C.POINTER(C.c_int32))
+            arr[i].zmin, arr[i].zmax = float(zmin), float(zmax)
+        best = np.zeros(4, np.int32)
+        oc = opts.to_c()
+        check(self._lib.mvs_autotune(self._h, arr, len(jobs), C.byref(oc), int(passes), ptr(best)), "mvs_autotune")
+        return tuple(int(b) for b in best)
+
     def get_depth(self, view: int) -> DepthMap:
         v = self.views[view]
         d = np.empty((v.h, v.w), np.float32)

@@ -83,6 +83,8 @@ class SceneSpec:
     noise_sigma: float = 0.0
     tex_freq: float = 4.0           # lattice cells per world unit at the coarsest octave
     octaves: int = 4
+    persistence: float = 0.65       # amplitude ratio between successive octaves
+    obj_region: Tuple[float, float] = (0.3, 1.8)   # object centres: x, z range (corner side of the arc)
     background: int = 128
 
 
@@ -122,7 +124,7 @@ def _hash(ix, iy, iz, seed: int):
     return (h & 0xFFFFFF).to(torch.float32) * (1.0 / 16777216.0)
 
 
-def _noise2(s, t, seed: int, freq: float, octaves: int):
+def _noise2(s, t, seed: int, freq: float, octaves: int, persistence: float = 0.5):
     acc = torch.zeros_like(s, dtype=torch.float32)
     norm = 0.0
     amp = 1.0
@@ -143,11 +145,11 @@ def _noise2(s, t, seed: int, freq: float, octaves: int):
         v = (h00 * (1 - ax) + h10 * ax) * (1 - ay) + (h01 * (1 - ax) + h11 * ax) * ay
         acc += amp * v
         norm += amp
-        amp *= 0.5
+        amp *= persistence
     return acc / norm
 
 
-def _noise3(p, seed: int, freq: float, octaves: int):
+def _noise3(p, seed: int, freq: float, octaves: int, persistence: float = 0.5):
     acc = torch.zeros(p.shape[:-1], dtype=torch.float32, device=p.device)
     norm = 0.0
     amp = 1.0
@@ -165,12 +167,14 @@ def _noise3(p, seed: int, freq: float, octaves: int):
                     v += wgt * _hash(q0[..., 0] + cx, q0[..., 1] + cy, q0[..., 2] + cz + 7919 * o, seed)
         acc += amp * v
         norm += amp
-        amp *= 0.5
+        amp *= persistence
     return acc / norm
 
 
-def _tone(n):
-    return torch.clamp(128.0 + 330.0 * (n - 0.5), 0.0, 255.0)
+def _tone(n, persistence=0.5):
+    # value-noise sums concentrate around 0.5; stretch to use the u8 range
+    gain = 330.0 if persistence <= 0.5 else 420.0
+    return torch.clamp(128.0 + gain * (n - 0.5), 0.0, 255.0)
 
 
 # --------------------------------------------------------------------------
@@ -185,11 +189,11 @@ def config_spec(name: str) -> SceneSpec:
     if name == "room1080":       # 20 x 1920x1080, f=1650, 90 deg arc, 12 objects
         return SceneSpec(seed=2, n_views=20, sizes=((1920, 1080),), focal=1650.0, arc_deg=90.0,
                          radius=4.0, arc_center=0.85, n_objects=12, flat_patch=True, noise_sigma=2.0,
-                         tex_freq=12.0)
+                         tex_freq=24.0, persistence=0.65)
     if name == "mixed64":        # 64 views of uneven size, 30 objects
         return SceneSpec(seed=3, n_views=64, sizes=((640, 480), (1280, 720), (1920, 1080), (2560, 1440)),
                          size_seeded=True, focal_rel=0.86, arc_deg=90.0, radius=4.0, arc_center=0.85,
-                         n_objects=30, flat_patch=True, noise_sigma=2.0, tex_freq=12.0)
+                         n_objects=30, flat_patch=True, noise_sigma=2.0, tex_freq=24.0, persistence=0.65)
     if name == "tiny":           # small scene for fast parity tests
         return SceneSpec(seed=11, n_views=5, sizes=((64, 48),), focal=60.0, arc_deg=40.0,
                          radius=4.0, arc_center=-1.0, n_objects=2, tex_freq=3.0)
@@ -201,10 +205,10 @@ def _objects(spec: SceneSpec, rng: np.random.Generator):
     for i in range(spec.n_objects):
         kind = "sphere" if rng.random() < 0.5 else "box"
         size = float(rng.uniform(*spec.obj_size))
-        # placed between the corner and the camera arc, resting on or above the floor
-        x = float(rng.uniform(0.4, 2.6))
-        z = float(rng.uniform(0.4, 2.6))
-        y = size + float(rng.uniform(0.0, 1.2))
+        # placed in the corner region, >= ~2 units from every camera, resting on or above the floor
+        x = float(rng.uniform(*spec.obj_region))
+        z = float(rng.uniform(*spec.obj_region))
+        y = size + float(rng.uniform(0.0, 1.0))
         objs.append({"kind": kind, "c": (x, y, z), "s": size, "seed": int(spec.seed * 1000 + 500 + i)})
     return objs
 
@@ -312,17 +316,18 @@ def _render_view(spec, objs, w, h, f, R, t, device, noise_seed):
     for pid, (a, b1, b2) in enumerate(((1, 0, 2), (0, 2, 1), (2, 0, 1))):
         m = sid == pid
         if m.any():
-            nv = _noise2(p[..., b1][m], p[..., b2][m], spec.seed * 1000 + 100 + pid, spec.tex_freq, spec.octaves)
-            val[m] = _tone(nv)
+            nv = _noise2(p[..., b1][m], p[..., b2][m], spec.seed * 1000 + 100 + pid, spec.tex_freq, spec.octaves,
+                         spec.persistence)
+            val[m] = _tone(nv, spec.persistence)
     for oi, o in enumerate(objs):
         m = sid == 3 + oi
         if m.any():
             q = p[m] - torch.tensor(o["c"], dtype=dt, device=device)
-            nv = _noise3(q + 50.0, o["seed"], spec.tex_freq, spec.octaves)
-            val[m] = _tone(nv)
+            nv = _noise3(q + 50.0, o["seed"], spec.tex_freq, spec.octaves, spec.persistence)
+            val[m] = _tone(nv, spec.persistence)
     if spec.flat_patch:
         # textureless rectangle on the x = 0 wall
-        m = (sid == 1) & (p[..., 2] >= 0.9) & (p[..., 2] <= 2.6) & (p[..., 1] >= 1.0) & (p[..., 1] <= 2.6)
+        m = (sid == 1) & (p[..., 2] >= 1.6) & (p[..., 2] <= 2.4) & (p[..., 1] >= 1.2) & (p[..., 1] <= 2.0)
         val[m] = 150.0
     if spec.noise_sigma > 0:
         g = torch.Generator(device=device)

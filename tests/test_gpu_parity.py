@@ -244,3 +244,46 @@ def test_fusion_resident_matches_host_maps(ctx):
     b = ctx.fuse_depth_maps([out[r] for r in range(len(views))], order, 0.01, opts)
     assert len(a) == len(b) and len(a) > 1000
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("shape", ["16x16", "16x16r", "32x8", "32x8p", "16x8r", "16x8p", "8x8", "16x16pr", "8x8p"])
+def test_every_launch_shape_is_exact(ctx, shape, monkeypatch):
+    """Tile shape, evaluator and schedule change only speed, never values."""
+    import oracle
+    from rustcolmap_amd.mvs import MvsOptions
+    from rustcolmap_amd.scene import select_sources
+    monkeypatch.setenv("MVS_TILE", shape)
+    sc = scene("corner320")
+    views = view_data(sc)
+    ctx.upload_views(views)
+    ov = oracle.OracleViews(views)
+    opts = MvsOptions(num_sources=4, sweeps=2)
+    srcs = select_sources(sc.views, 4)
+    jobs = [(r, srcs[r], 1.0, 10.0) for r in (0, 4, 9)]
+    _say("launch shape %s" % shape)
+    for mode in (0, 4):
+        out = ctx.estimate_depth_batch(jobs, opts, mode=mode)
+        for r, s, zmin, zmax in jobs:
+            d, c, n, _, _ = ov.estimate_depth(r, s, zmin, zmax, opts)
+            assert np.array_equal(bits(out[r].depth), bits(d)), (shape, mode, r)
+            assert np.array_equal(bits(out[r].cost), bits(c)), (shape, mode, r)
+
+
+def test_autotune_then_exact(ctx):
+    import oracle
+    from rustcolmap_amd.mvs import MvsOptions
+    from rustcolmap_amd.scene import select_sources
+    sc = scene("corner320")
+    views = view_data(sc)
+    ctx.upload_views(views)
+    ov = oracle.OracleViews(views)
+    opts = MvsOptions(num_sources=4, sweeps=2)
+    srcs = select_sources(sc.views, 4)
+    jobs = [(r, srcs[r], 1.0, 10.0) for r in range(len(views))]
+    best = ctx.autotune(jobs, opts, passes=1)
+    _say("autotune picked %s" % (best,))
+    assert best[0] * best[1] in (64, 128, 256) and best[2] in (0, 1) and best[3] in (0, 1)
+    out = ctx.estimate_depth_batch(jobs, opts, mode=0)
+    for r, s, zmin, zmax in jobs:
+        d, c, n, _, _ = ov.estimate_depth(r, s, zmin, zmax, opts)
+        assert np.array_equal(bits(out[r].depth), bits(d)), r
