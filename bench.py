@@ -37,7 +37,57 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def rocprof_summary(path, copy_bytes=1 << 30, only="mvs::"):
+    """Summarise rocprofv3 CSV output under `path`: per-kernel dispatch
+    count / total / average duration (kernel_stats or kernel_trace CSV), and
+    per-kernel averages of any collected counters (counter_collection CSV),
+    with FETCH_SIZE / WRITE_SIZE scaled by what the same counter reads for
+    the plain device copy of `copy_bytes` bytes that `--profile` runs."""
+    import csv
+    import glob
+    from collections import defaultdict
+
+    def short(name):
+        return name.split("(")[0].replace("void ", "")[:90]
+
+    lines = []
+    for f in sorted(glob.glob(os.path.join(path, "**", "*kernel_stats.csv"), recursive=True)):
+        lines.append("# kernel stats: %s" % os.path.relpath(f, path))
+        lines.append("%-90s %8s %14s %12s %7s" % ("kernel", "calls", "total_ns", "avg_ns", "pct"))
+        for r in csv.DictReader(open(f)):
+            if only and not short(r["Name"]).startswith(only):
+                continue
+            lines.append("%-90s %8s %14s %12.0f %7.2f" % (short(r["Name"]), r["Calls"], r["TotalDurationNs"],
+                                                          float(r["AverageNs"]), float(r["Percentage"])))
+    for f in sorted(glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True)):
+        acc = defaultdict(lambda: [0.0, 0])
+        for r in csv.DictReader(open(f)):
+            key = (short(r["Kernel_Name"]), r["Counter_Name"])
+            acc[key][0] += float(r["Counter_Value"])
+            acc[key][1] += 1
+        lines.append("# counters: %s" % os.path.relpath(f, path))
+        copy = {}
+        for (k, cn), (tot, cnt) in acc.items():
+            if k.startswith("mvs::copy_kernel"):
+                copy.setdefault(cn, []).append(tot / cnt)
+        for (k, cn), (tot, cnt) in sorted(acc.items()):
+            if only and not k.startswith(only):
+                continue
+            avg = tot / cnt
+            line = "%-90s %-12s dispatches %5d  avg %.4g" % (k, cn, cnt, avg)
+            if cn in ("FETCH_SIZE", "WRITE_SIZE") and copy.get(cn):
+                # a plain copy of copy_bytes reads and writes copy_bytes each
+                ref = max(copy[cn])
+                scale = copy_bytes / ref if ref > 0 else float("nan")
+                line += "  -> %.4g bytes (copy-calibrated x%.4g)" % (avg * scale, scale)
+            lines.append(line)
+    return "\n".join(lines)
+
+
 def main():
+    if len(sys.argv) > 2 and sys.argv[1] == "--rocprof-summary":
+        print(rocprof_summary(sys.argv[2]))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -51,6 +101,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--batched", action="store_true",
+                    help="uneven batch: 64 views of mixed size, K uniform in [4, 12]; batched graph schedule vs "
+                         "per-view launches (prints its own JSON line)")
     ap.add_argument("--profile", action="store_true",
                     help="one directly launched batch plus a 1 GiB device copy (for counter calibration), no JSON")
     args = ap.parse_args()
@@ -65,6 +118,8 @@ def main():
     from rustcolmap_amd.scene import config_spec, render_scene, select_sources
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.batched:
+        return batched_bench(args)
     t0 = time.time()
     spec = config_spec(args.workload)
     if args.views:
@@ -86,12 +141,8 @@ def main():
         ctx.estimate_depth_batch(jobs, opts, mode=(args.mode & 3) | 4, download=False)
         st = ctx.stats()
         log("profile run: passes %.1f ms, evals pass %d" % (st.ms_pass, st.n_eval_pass))
-        x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB
-        y = torch.empty_like(x)
-        x.fill_(1.0)
-        torch.cuda.synchronize()
-        y.copy_(x)
-        torch.cuda.synchronize()
+        from rustcolmap_amd import _lib as L
+        L.check(L.load().mvs_debug_copy(ctx._h, 1 << 30), "mvs_debug_copy")   # mvs::copy_kernel, 1 GiB
         log("copy of 1073741824 bytes done")
         return
 
@@ -172,6 +223,55 @@ def main():
         if st.ms_init > 0 else None,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def batched_bench(args):
+    """64 views of uneven size (seeded draw over 640x480 .. 2560x1440,
+    f = 0.86 W), K uniform in [4, 12] per view, 8 sweeps: one batched
+    schedule (graph replay of one launch per stage) against one launch per
+    view and stage."""
+    import torch
+    from rustcolmap_amd.mvs import MvsContext, MvsOptions, ViewData
+    from rustcolmap_amd.scene import config_spec, render_scene, select_sources
+    spec = config_spec("mixed64")
+    if args.views:
+        spec.n_views = args.views
+    t0 = time.time()
+    sc = render_scene(spec, device="cuda", with_gt=False)
+    log("rendered %d views in %.1fs" % (len(sc.views), time.time() - t0))
+    views = [ViewData.from_scene_view(v) for v in sc.views]
+    rng = np.random.default_rng(spec.seed + 100)
+    ks = rng.integers(4, 13, size=len(views))
+    jobs = [(r, select_sources(sc.views, int(ks[r]))[r], 1.0, 10.0) for r in range(len(views))]
+    opts = MvsOptions(num_sources=12, sweeps=args.sweeps)
+    npix = sum(v.w * v.h for v in views)
+    ctx = MvsContext(0)
+    ctx.upload_views(views)
+    tuned = None if args.no_tune else ctx.autotune(jobs, opts, passes=1)
+    res = {}
+    for name, mode in (("batched_graph", 0), ("per_view_launches", 1)):
+        for _ in range(args.warmup):
+            ctx.estimate_depth_batch(jobs, opts, mode=mode, download=False)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.estimate_depth_batch(jobs, opts, mode=mode, download=False)
+        torch.cuda.synchronize()
+        ms = 1000.0 * (time.perf_counter() - t1) / args.steps
+        res[name] = {"ms_per_step": round(ms, 2), "mpix_iter_per_s": round(npix * args.sweeps / ms / 1000.0, 2)}
+        log("%s: %.1f ms/step" % (name, ms))
+    out = {"metric": "depth Mpix.iter/s, uneven batch (64 views, mixed sizes, K in [4,12])",
+           "value": res["batched_graph"]["mpix_iter_per_s"], "unit": "Mpix*iter/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["batched_graph"]["ms_per_step"],
+           "higher_is_better": True, "data": "synthetic",
+           "config": {"workload": "mixed64: %d views, %.1f Mpx total, K %d..%d, sweeps %d"
+                      % (len(views), npix / 1e6, int(ks.min()), int(ks.max()), args.sweeps),
+                      "launch": None if tuned is None else
+                      {"tile": "%dx%d" % tuned[:2], "persistent": tuned[2], "evaluator": tuned[3]}},
+           "schedules": res,
+           "speedup_batched_vs_per_view": round(res["per_view_launches"]["ms_per_step"]
+                                                / res["batched_graph"]["ms_per_step"], 3)}
     print(json.dumps(out), flush=True)
 
 

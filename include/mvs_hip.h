@@ -93,6 +93,8 @@ typedef struct mvs_stats {
     int64_t n_points;
     int32_t n_pass_launches;
     int32_t graph_used;
+    double ms_learned_volume; /* learned init: sweep volume (part of ms_init) */
+    double ms_learned_net;    /* learned init: network forward (part of ms_init) */
 } mvs_stats_t;
 
 /* MvsOptions::default (mvs/src/lib.rs:56-70). */
@@ -197,11 +199,70 @@ int mvs_get_stats(mvs_ctx *ctx, mvs_stats_t *out);
 int mvs_debug_plane_cost(mvs_ctx *ctx, int32_t ref, const int32_t *srcs, int32_t k, const float *q, int32_t n,
                          float *cost, int32_t *ham, int32_t *ninv, float *peak);
 
+/* Instrumentation: one plain device-to-device copy of `bytes` (a multiple
+ * of 16) by the library's copy kernel; a reference for traffic counters. */
+int mvs_debug_copy(mvs_ctx *ctx, int64_t bytes);
+
 /* Per-pixel RNG of estimate_depth (stereo.rs:47-78) evaluated on the
  * device: for n (x, y, salt) triples, the seed and the first `ndraw`
  * uniform() values of its stream. */
 int mvs_debug_rng(mvs_ctx *ctx, const int32_t *xy, const uint64_t *salt, int32_t n, int32_t ndraw,
                   uint64_t *seed_out, float *uniform_out);
+
+/* ------------------------------------------------------------------------
+ * Learned plane proposals (no counterpart in the reference: an additional
+ * initialisation for estimate_depth's plane sweep, stereo.rs:247-269).
+ * A 5-level U-Net (3x3 convs, channels 32-64-96-128-160, bf16 on MFMA with
+ * fp32 accumulation) maps the gray level plus a 32-plane census sweep volume
+ * over 4 sources to normalised inverse depth, unit normal and confidence.
+ * ------------------------------------------------------------------------ */
+typedef struct mvs_net mvs_net;
+
+/* Network with seeded He-normal weights, bound to the context's device. */
+int mvs_net_create(mvs_ctx *ctx, uint64_t seed, mvs_net **out);
+void mvs_net_destroy(mvs_net *net);
+int64_t mvs_net_num_params(mvs_net *net);
+/* fp32 master weights (mvs_net_num_params floats). */
+int mvs_net_get_params(mvs_net *net, float *out);
+int mvs_net_set_params(mvs_net *net, const float *in);
+
+/* 32-plane fronto-parallel sweep volume of one job (its k sources): for
+ * plane h = 0..31, inv = 1/zmax + (1/zmin - 1/zmax) * h / 31 and the
+ * plane_cost of (0, 0, 1, 1/inv) with no abort; out = w*h*32 floats. */
+int mvs_sweep_volume(mvs_ctx *ctx, const mvs_job_t *job, float *out);
+
+/* Adds training samples: for each job (a view of the context, usually 4
+ * sources), the sweep volume is computed and kept on the device with the
+ * host target (w*h*5 floats: normalised inverse depth (1/z - 1/zmax) /
+ * (1/zmin - 1/zmax), unit normal, validity 0/1).  Returns the sample count
+ * (>= 0) or an error. */
+int mvs_net_add_samples(mvs_net *net, const mvs_job_t *jobs, int32_t njobs, const float *const *targets);
+int mvs_net_clear_samples(mvs_net *net);
+
+/* One Adam step on samples idx[0..n) (same image size): forward, loss
+ * (L1 inverse depth + 0.5 (1 - cos) normal on valid pixels, 0.25 BCE
+ * confidence against "within 2 % of the target"), backward.  loss4
+ * (optional): mean depth L1, mean normal term, mean BCE, valid fraction. */
+int mvs_net_train_step(mvs_net *net, const int32_t *idx, int32_t n, float lr, float *loss4);
+
+/* Prediction for one job: w*h*5 floats (inverse depth normalised to the
+ * job's range, unit normal, confidence). */
+int mvs_net_predict(mvs_net *net, const mvs_job_t *job, float *pred);
+
+/* Gradient probe for tests: forward + loss + backward on a host batch
+ * (input n*Hp*Wp*64 floats, rounded to bf16; target n*h*w*5) without an
+ * optimizer step.  grads: num_params floats; pred: n*h*w*5; loss4 raw sums. */
+int mvs_net_debug_grads(mvs_net *net, const float *input, const float *target, int32_t n, int32_t Hp, int32_t Wp,
+                        int32_t h, int32_t w, float *grads, float *pred, float *loss4);
+
+/* estimate_depth with the learned initialisation: sweep volume over the
+ * first vol_k sources, network, then per pixel the proposed plane where
+ * confidence >= conf_thr (else the plain plane sweep), then 2 * opts->sweeps
+ * passes and the gate as in mvs_estimate_depth_batch.  Results stay
+ * resident and are copied to the jobs' host pointers where non-NULL.
+ * n_accepted (optional): pixels that started from a proposed plane. */
+int mvs_estimate_depth_learned(mvs_ctx *ctx, mvs_net *net, const mvs_job_t *jobs, int32_t njobs,
+                               const mvs_options_t *opts, int32_t vol_k, float conf_thr, int64_t *n_accepted);
 
 #ifdef __cplusplus
 }

@@ -50,7 +50,7 @@ class MvsStatsC(C.Structure):
         ("taps_per_eval_sum", C.c_int64), ("ms_prepare", C.c_double), ("ms_init", C.c_double),
         ("ms_pass", C.c_double), ("ms_gate", C.c_double), ("ms_fuse", C.c_double),
         ("n_candidates", C.c_int64), ("n_points", C.c_int64), ("n_pass_launches", C.c_int32),
-        ("graph_used", C.c_int32),
+        ("graph_used", C.c_int32), ("ms_learned_volume", C.c_double), ("ms_learned_net", C.c_double),
     ]
 
 
@@ -81,6 +81,21 @@ _SIGS = {
     "mvs_get_stats": (C.c_int, [_P, C.POINTER(MvsStatsC)]),
     "mvs_debug_plane_cost": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, C.c_int32, _P, _P, _P, _P]),
     "mvs_debug_rng": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, _P, _P]),
+    "mvs_debug_copy": (C.c_int, [_P, C.c_int64]),
+    "mvs_net_create": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
+    "mvs_net_destroy": (None, [_P]),
+    "mvs_net_num_params": (C.c_int64, [_P]),
+    "mvs_net_get_params": (C.c_int, [_P, _P]),
+    "mvs_net_set_params": (C.c_int, [_P, _P]),
+    "mvs_sweep_volume": (C.c_int, [_P, C.POINTER(MvsJobC), _P]),
+    "mvs_net_add_samples": (C.c_int, [_P, C.POINTER(MvsJobC), C.c_int32, _P]),
+    "mvs_net_clear_samples": (C.c_int, [_P]),
+    "mvs_net_train_step": (C.c_int, [_P, _P, C.c_int32, C.c_float, _P]),
+    "mvs_net_predict": (C.c_int, [_P, C.POINTER(MvsJobC), _P]),
+    "mvs_net_debug_grads": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P,
+                                      _P]),
+    "mvs_estimate_depth_learned": (C.c_int, [_P, _P, C.POINTER(MvsJobC), C.c_int32, C.POINTER(MvsOptionsC),
+                                             C.c_int32, C.c_float, C.POINTER(C.c_int64)]),
 }
 
 _lib = None
@@ -110,7 +125,7 @@ class MvsError(RuntimeError):
 
 
 def check(code: int, where: str):
-    if code != MVS_OK:
+    if code < MVS_OK:
         raise MvsError(code, where, load().mvs_last_error().decode("utf-8", "replace"))
 
 

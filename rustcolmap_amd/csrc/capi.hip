@@ -18,6 +18,8 @@
 #include <vector>
 
 #include "../../include/mvs_hip.h"
+#include "net_internal.h"
+#include "net_kernels.h"
 #include "stereo_kernels.h"
 #include "tuner.h"
 
@@ -937,6 +939,25 @@ int mvs_debug_plane_cost(mvs_ctx* c, int32_t ref, const int32_t* srcs, int32_t k
     return MVS_OK;
 }
 
+int mvs_debug_copy(mvs_ctx* c, int64_t bytes) {
+    if (!c || bytes <= 0 || (bytes & 15)) return fail(MVS_ERR_ARG, "mvs_debug_copy: bytes must be a positive multiple of 16");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int r = set_device(c)) return r;
+    DBuf<uint8_t> a, b;
+    if (a.need((size_t)bytes) != hipSuccess || b.need((size_t)bytes) != hipSuccess) {
+        a.release();
+        b.release();
+        return fail(MVS_ERR_NOMEM, "mvs_debug_copy: allocation failed");
+    }
+    hipError_t e = hipMemsetAsync(a.p, 1, (size_t)bytes, c->stream);
+    if (e == hipSuccess) e = launch_copy(a.p, b.p, (size_t)bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    a.release();
+    b.release();
+    if (e != hipSuccess) return fail(MVS_ERR_HIP, std::string("mvs_debug_copy: ") + hipGetErrorString(e));
+    return MVS_OK;
+}
+
 int mvs_debug_rng(mvs_ctx* c, const int32_t* xy, const uint64_t* salt, int32_t n, int32_t ndraw, uint64_t* seed_out,
                   float* uniform_out) {
     if (!c || !xy || !salt || n < 0 || ndraw < 0 || !seed_out || (ndraw > 0 && !uniform_out))
@@ -957,6 +978,482 @@ int mvs_debug_rng(mvs_ctx* c, const int32_t* xy, const uint64_t* salt, int32_t n
     if (ndraw > 0) HCHK(hipMemcpyAsync(uniform_out, du, (size_t)n * ndraw * 4, hipMemcpyDeviceToHost, c->stream));
     HCHK(hipStreamSynchronize(c->stream));
     return MVS_OK;
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------------------------
+// learned plane proposals
+// --------------------------------------------------------------------------
+
+struct NetSample {
+    int view = 0, h = 0, w = 0;
+    DBuf<uint16_t> vol;     // [h*w][32] bf16 sweep features
+    DBuf<uint8_t> gray;     // [h*w]
+    DBuf<float> target;     // [h*w][5]
+};
+
+struct mvs_net {
+    mvs_ctx* ctx = nullptr;
+    net::Net net;
+    std::vector<NetSample*> samples;
+    DBuf<uint16_t> vol_tmp;
+    DBuf<float> volf_tmp;
+    DBuf<VolOut> vol_outs;
+    DBuf<uintptr_t> ptrs;     // device pointer tables (vol, gray, pred)
+    DBuf<float> preds;
+    DBuf<u64> cnt;
+};
+
+static inline int round16(int v) { return (v + 15) & ~15; }
+
+static hipError_t launch_zero_grads(mvs_net* nt) { return net::launch_zero(nt->net.g, nt->net.n_params, nt->ctx->stream); }
+
+// Plans `jobs` with 16x16 tiles (the sweep-volume and proposal kernels' shape).
+static int plan_16(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_options_t* o, BatchPlan& P, int eval) {
+    c->tuner.forced = true;
+    c->tuner.force_cfg = TileCfg();
+    c->tuner.force_cfg.eval = eval;
+    const int r = plan_batch(c, jobs, njobs, o, 4, -1, P);
+    c->tuner.forced = false;
+    return r;
+}
+
+static int upload_plan(mvs_ctx* c, const BatchPlan& P) {
+    c->graph.release();
+    HCHK(c->jobs.need(P.jobs.size()));
+    HCHK(c->srcs.need(P.srcs.size()));
+    HCHK(c->tiles.need(P.tiles.size()));
+    HCHK(c->counters.need(1 + P.n_passes + (P.n_passes + 1) / 2 + 1 + 4));
+    HCHK(hipMemcpyAsync(c->jobs.p, P.jobs.data(), P.jobs.size() * sizeof(JobDesc), hipMemcpyHostToDevice, c->stream));
+    HCHK(hipMemcpyAsync(c->srcs.p, P.srcs.data(), P.srcs.size() * sizeof(SrcDesc), hipMemcpyHostToDevice, c->stream));
+    HCHK(hipMemcpyAsync(c->tiles.p, P.tiles.data(), P.tiles.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    return MVS_OK;
+}
+
+// Sweep volume of each job into per-job outputs (device pointers).
+static int run_volume(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, int njobs, const std::vector<VolOut>& outs) {
+    mvs_options_t o;
+    mvs_options_default(&o);
+    BatchPlan P;
+    if (int r = plan_16(c, jobs, njobs, &o, P, 1)) return r;
+    if (int r = upload_plan(c, P)) return r;
+    HCHK(nt->vol_outs.need(outs.size()));
+    HCHK(hipMemcpyAsync(nt->vol_outs.p, outs.data(), outs.size() * sizeof(VolOut), hipMemcpyHostToDevice, c->stream));
+    HCHK(launch_volume(c->jobs.p, c->srcs.p, c->tiles.p, (int)P.tiles.size(), 32, nt->vol_outs.p, c->stream));
+    return MVS_OK;
+}
+
+// Forward of the network over n same-size samples given device vol/gray
+// pointers; predictions written to dpred ([n][h][w][5]).
+static int net_predict_batch(mvs_net* nt, const std::vector<const uint16_t*>& vols,
+                             const std::vector<const uint8_t*>& grays, int h, int w, float* dpred, bool train,
+                             const float* dtarget, float* loss4) {
+    mvs_ctx* c = nt->ctx;
+    const int n = (int)vols.size();
+    const int Hp = round16(h), Wp = round16(w);
+    HCHK(nt->net.ensure(n, Hp, Wp, train));
+    HCHK(nt->ptrs.need(2 * n));
+    std::vector<uintptr_t> tab(2 * n);
+    for (int i = 0; i < n; i++) {
+        tab[i] = (uintptr_t)vols[i];
+        tab[n + i] = (uintptr_t)grays[i];
+    }
+    HCHK(hipMemcpyAsync(nt->ptrs.p, tab.data(), tab.size() * sizeof(uintptr_t), hipMemcpyHostToDevice, c->stream));
+    HCHK(launch_assemble((const uint16_t* const*)nt->ptrs.p, (const uint8_t* const*)(nt->ptrs.p + n), n, h, w, Hp, Wp,
+                         nt->net.input(), c->stream));
+    HCHK(nt->net.forward());
+    HCHK(nt->net.head(train, h, w, dpred, dtarget, 0.02f));
+    if (train) {
+        HCHK(nt->net.backward());
+        if (loss4) {
+            HCHK(hipMemcpyAsync(loss4, nt->net.loss, 4 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        }
+    }
+    return MVS_OK;
+}
+
+extern "C" {
+
+int mvs_net_create(mvs_ctx* c, uint64_t seed, mvs_net** out) {
+    if (!c || !out) return fail(MVS_ERR_ARG, "mvs_net_create: NULL argument");
+    *out = nullptr;
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        mvs_net* nt = new mvs_net();
+        nt->ctx = c;
+        hipError_t e = nt->net.init(seed, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            nt->net.release();
+            delete nt;
+            return fail(MVS_ERR_HIP, std::string("mvs_net_create: ") + hipGetErrorString(e));
+        }
+        *out = nt;
+        return MVS_OK;
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_net_create: ") + e.what());
+    }
+}
+
+void mvs_net_destroy(mvs_net* nt) {
+    if (!nt) return;
+    std::lock_guard<std::mutex> lk(nt->ctx->mu);
+    (void)hipSetDevice(nt->ctx->device);
+    (void)hipStreamSynchronize(nt->ctx->stream);
+    for (NetSample* s : nt->samples) {
+        s->vol.release();
+        s->gray.release();
+        s->target.release();
+        delete s;
+    }
+    nt->net.release();
+    nt->vol_tmp.release();
+    nt->volf_tmp.release();
+    nt->vol_outs.release();
+    nt->ptrs.release();
+    nt->preds.release();
+    nt->cnt.release();
+    delete nt;
+}
+
+int64_t mvs_net_num_params(mvs_net* nt) { return nt ? nt->net.n_params : -1; }
+
+int mvs_net_get_params(mvs_net* nt, float* out) {
+    if (!nt || !out) return fail(MVS_ERR_ARG, "mvs_net_get_params: NULL argument");
+    std::lock_guard<std::mutex> lk(nt->ctx->mu);
+    if (int r = set_device(nt->ctx)) return r;
+    HCHK(hipMemcpyAsync(out, nt->net.p, nt->net.n_params * 4, hipMemcpyDeviceToHost, nt->ctx->stream));
+    HCHK(hipStreamSynchronize(nt->ctx->stream));
+    return MVS_OK;
+}
+
+int mvs_net_set_params(mvs_net* nt, const float* in) {
+    if (!nt || !in) return fail(MVS_ERR_ARG, "mvs_net_set_params: NULL argument");
+    std::lock_guard<std::mutex> lk(nt->ctx->mu);
+    if (int r = set_device(nt->ctx)) return r;
+    HCHK(hipMemcpyAsync(nt->net.p, in, nt->net.n_params * 4, hipMemcpyHostToDevice, nt->ctx->stream));
+    HCHK(nt->net.refresh_weights());
+    HCHK(hipStreamSynchronize(nt->ctx->stream));
+    return MVS_OK;
+}
+
+int mvs_sweep_volume(mvs_ctx* c, const mvs_job_t* job, float* out) {
+    if (!c || !job || !out) return fail(MVS_ERR_ARG, "mvs_sweep_volume: NULL argument");
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        if (job->ref < 0 || job->ref >= (int)c->views.size()) return fail(MVS_ERR_ARG, "mvs_sweep_volume: bad ref");
+        const ViewState& v = c->views[job->ref];
+        const size_t np = (size_t)v.w * v.h;
+        mvs_net tmp;
+        tmp.ctx = c;
+        DBuf<float> vf;
+        HCHK(vf.need(np * 32));
+        std::vector<VolOut> outs(1);
+        outs[0].vol = nullptr;
+        outs[0].volf = vf.p;
+        int r = run_volume(c, &tmp, job, 1, outs);
+        if (r == MVS_OK) {
+            hipError_t e = hipMemcpyAsync(out, vf.p, np * 32 * 4, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) r = fail(MVS_ERR_HIP, std::string("mvs_sweep_volume: ") + hipGetErrorString(e));
+        }
+        vf.release();
+        tmp.vol_outs.release();
+        return r;
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_sweep_volume: ") + e.what());
+    }
+}
+
+int mvs_net_add_samples(mvs_net* nt, const mvs_job_t* jobs, int32_t njobs, const float* const* targets) {
+    if (!nt || !jobs || njobs < 1 || !targets) return fail(MVS_ERR_ARG, "mvs_net_add_samples: bad arguments");
+    mvs_ctx* c = nt->ctx;
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        std::vector<VolOut> outs(njobs);
+        std::vector<NetSample*> fresh;
+        for (int j = 0; j < njobs; j++) {
+            if (jobs[j].ref < 0 || jobs[j].ref >= (int)c->views.size())
+                return fail(MVS_ERR_ARG, "mvs_net_add_samples: bad ref");
+            const ViewState& v = c->views[jobs[j].ref];
+            NetSample* s = new NetSample();
+            s->view = jobs[j].ref;
+            s->h = v.h;
+            s->w = v.w;
+            const size_t np = (size_t)v.w * v.h;
+            HCHK(s->vol.need(np * 32));
+            HCHK(s->gray.need(np));
+            HCHK(s->target.need(np * 5));
+            HCHK(hipMemcpyAsync(s->gray.p, v.gray.p, np, hipMemcpyDeviceToDevice, c->stream));
+            HCHK(hipMemcpyAsync(s->target.p, targets[j], np * 5 * 4, hipMemcpyHostToDevice, c->stream));
+            outs[j].vol = s->vol.p;
+            outs[j].volf = nullptr;
+            fresh.push_back(s);
+        }
+        if (int r = run_volume(c, nt, jobs, njobs, outs)) return r;
+        HCHK(hipStreamSynchronize(c->stream));
+        for (NetSample* s : fresh) nt->samples.push_back(s);
+        return (int)nt->samples.size();
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_net_add_samples: ") + e.what());
+    }
+}
+
+int mvs_net_clear_samples(mvs_net* nt) {
+    if (!nt) return fail(MVS_ERR_ARG, "mvs_net_clear_samples: NULL");
+    std::lock_guard<std::mutex> lk(nt->ctx->mu);
+    (void)hipSetDevice(nt->ctx->device);
+    (void)hipStreamSynchronize(nt->ctx->stream);
+    for (NetSample* s : nt->samples) {
+        s->vol.release();
+        s->gray.release();
+        s->target.release();
+        delete s;
+    }
+    nt->samples.clear();
+    return MVS_OK;
+}
+
+int mvs_net_train_step(mvs_net* nt, const int32_t* idx, int32_t n, float lr, float* loss4) {
+    if (!nt || !idx || n < 1) return fail(MVS_ERR_ARG, "mvs_net_train_step: bad arguments");
+    mvs_ctx* c = nt->ctx;
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        std::vector<const uint16_t*> vols;
+        std::vector<const uint8_t*> grays;
+        int h = -1, w = -1;
+        for (int i = 0; i < n; i++) {
+            if (idx[i] < 0 || idx[i] >= (int)nt->samples.size()) return fail(MVS_ERR_ARG, "train_step: bad sample");
+            NetSample* s = nt->samples[idx[i]];
+            if (h < 0) {
+                h = s->h;
+                w = s->w;
+            } else if (s->h != h || s->w != w) {
+                return fail(MVS_ERR_ARG, "train_step: samples of one batch must share their size");
+            }
+            vols.push_back(s->vol.p);
+            grays.push_back(s->gray.p);
+        }
+        const size_t np = (size_t)h * w;
+        HCHK(nt->net.ensure(n, round16(h), round16(w), true));
+        for (int i = 0; i < n; i++)
+            HCHK(hipMemcpyAsync(nt->net.target + (size_t)i * np * 5, nt->samples[idx[i]]->target.p, np * 5 * 4,
+                                hipMemcpyDeviceToDevice, c->stream));
+        float l4[4] = {0, 0, 0, 0};
+        if (int r = net_predict_batch(nt, vols, grays, h, w, nullptr, true, nt->net.target, l4)) return r;
+        HCHK(nt->net.adam(lr));
+        HCHK(hipStreamSynchronize(c->stream));
+        if (loss4) {
+            const float npx = (float)((double)n * np);
+            loss4[0] = l4[0] / std::max(l4[3], 1.0f);   // mean L1 (normalised inverse depth) over valid px
+            loss4[1] = l4[1] / std::max(l4[3], 1.0f);   // mean (1 - cos) over valid px
+            loss4[2] = l4[2] / npx;                     // mean confidence BCE
+            loss4[3] = l4[3] / npx;                     // valid fraction
+        }
+        return MVS_OK;
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_net_train_step: ") + e.what());
+    }
+}
+
+int mvs_net_predict(mvs_net* nt, const mvs_job_t* job, float* pred) {
+    if (!nt || !job || !pred) return fail(MVS_ERR_ARG, "mvs_net_predict: NULL argument");
+    mvs_ctx* c = nt->ctx;
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        if (job->ref < 0 || job->ref >= (int)c->views.size()) return fail(MVS_ERR_ARG, "mvs_net_predict: bad ref");
+        const ViewState& v = c->views[job->ref];
+        const size_t np = (size_t)v.w * v.h;
+        HCHK(nt->vol_tmp.need(np * 32));
+        HCHK(nt->preds.need(np * 5));
+        std::vector<VolOut> outs(1);
+        outs[0].vol = nt->vol_tmp.p;
+        outs[0].volf = nullptr;
+        if (int r = run_volume(c, nt, job, 1, outs)) return r;
+        std::vector<const uint16_t*> vols(1, nt->vol_tmp.p);
+        std::vector<const uint8_t*> grays(1, v.gray.p);
+        if (int r = net_predict_batch(nt, vols, grays, v.h, v.w, nt->preds.p, false, nullptr, nullptr)) return r;
+        HCHK(hipMemcpyAsync(pred, nt->preds.p, np * 5 * 4, hipMemcpyDeviceToHost, c->stream));
+        HCHK(hipStreamSynchronize(c->stream));
+        return MVS_OK;
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_net_predict: ") + e.what());
+    }
+}
+
+int mvs_net_debug_grads(mvs_net* nt, const float* input, const float* target, int32_t n, int32_t Hp, int32_t Wp,
+                        int32_t h, int32_t w, float* grads, float* pred, float* loss4) {
+    if (!nt || !input || !target || n < 1 || Hp % 16 || Wp % 16 || h > Hp || w > Wp)
+        return fail(MVS_ERR_ARG, "mvs_net_debug_grads: bad arguments");
+    mvs_ctx* c = nt->ctx;
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        HCHK(nt->net.ensure(n, Hp, Wp, true));
+        const size_t ni = (size_t)n * Hp * Wp * 64;
+        std::vector<uint16_t> hb(ni);
+        for (size_t i = 0; i < ni; i++) {
+            uint32_t u;
+            memcpy(&u, &input[i], 4);
+            u += 0x7fffu + ((u >> 16) & 1u);
+            hb[i] = (uint16_t)(u >> 16);
+        }
+        HCHK(hipMemcpyAsync(nt->net.input(), hb.data(), ni * 2, hipMemcpyHostToDevice, c->stream));
+        HCHK(hipMemcpyAsync(nt->net.target, target, (size_t)n * h * w * 5 * 4, hipMemcpyHostToDevice, c->stream));
+        HCHK(launch_zero_grads(nt));
+        HCHK(nt->net.forward());
+        HCHK(nt->net.head(true, h, w, nt->net.pred, nt->net.target, 0.02f));
+        HCHK(nt->net.backward());
+        if (grads) HCHK(hipMemcpyAsync(grads, nt->net.g, nt->net.n_params * 4, hipMemcpyDeviceToHost, c->stream));
+        if (pred) HCHK(hipMemcpyAsync(pred, nt->net.pred, (size_t)n * h * w * 5 * 4, hipMemcpyDeviceToHost, c->stream));
+        if (loss4) HCHK(hipMemcpyAsync(loss4, nt->net.loss, 16, hipMemcpyDeviceToHost, c->stream));
+        HCHK(launch_zero_grads(nt));
+        HCHK(hipStreamSynchronize(c->stream));
+        return MVS_OK;
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_net_debug_grads: ") + e.what());
+    }
+}
+
+int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, int32_t njobs,
+                               const mvs_options_t* opts, int32_t vol_k, float conf_thr, int64_t* n_accepted) {
+    if (!c || !nt || !jobs || njobs < 1 || !opts || vol_k < 1)
+        return fail(MVS_ERR_ARG, "mvs_estimate_depth_learned: bad arguments");
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        // 1. sweep volumes over the first vol_k sources of every job
+        std::vector<mvs_job_t> vj(jobs, jobs + njobs);
+        size_t total = 0;
+        std::vector<size_t> off(njobs);
+        for (int j = 0; j < njobs; j++) {
+            if (jobs[j].ref < 0 || jobs[j].ref >= (int)c->views.size())
+                return fail(MVS_ERR_ARG, "mvs_estimate_depth_learned: bad ref");
+            vj[j].k = std::min(jobs[j].k, vol_k);
+            off[j] = total;
+            total += (size_t)c->views[jobs[j].ref].w * c->views[jobs[j].ref].h;
+        }
+        HCHK(nt->vol_tmp.need(total * 32));
+        HCHK(nt->preds.need(total * 5));
+        HCHK(hipEventRecord(c->ev[0], c->stream));
+        std::vector<VolOut> outs(njobs);
+        for (int j = 0; j < njobs; j++) {
+            outs[j].vol = nt->vol_tmp.p + off[j] * 32;
+            outs[j].volf = nullptr;
+        }
+        if (int r = run_volume(c, nt, vj.data(), njobs, outs)) return r;
+        HCHK(hipEventRecord(c->ev[1], c->stream));
+        // 2. network over same-size groups, in chunks that bound activation memory
+        std::vector<int> order(njobs);
+        for (int j = 0; j < njobs; j++) order[j] = j;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            const ViewState &va = c->views[jobs[a].ref], &vb = c->views[jobs[b].ref];
+            return va.w != vb.w ? va.w < vb.w : va.h < vb.h;
+        });
+        for (size_t i = 0; i < order.size();) {
+            const ViewState& v0 = c->views[jobs[order[i]].ref];
+            const int64_t px = (int64_t)round16(v0.w) * round16(v0.h);
+            const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(16, ((int64_t)24 << 20) / px));
+            std::vector<const uint16_t*> vols;
+            std::vector<const uint8_t*> grays;
+            size_t k = i;
+            while (k < order.size() && (int)vols.size() < chunk) {
+                const ViewState& vk = c->views[jobs[order[k]].ref];
+                if (vk.w != v0.w || vk.h != v0.h) break;
+                vols.push_back(nt->vol_tmp.p + off[order[k]] * 32);
+                grays.push_back(vk.gray.p);
+                ++k;
+            }
+            // predictions of a chunk land contiguously; jobs of one chunk are consecutive in `order`
+            DBuf<float> chunk_pred;
+            float* dst = nt->preds.p + off[order[i]] * 5;
+            bool contiguous = true;
+            for (size_t q = i; q < k; q++)
+                if (off[order[q]] != off[order[i]] + (q - i) * (size_t)v0.w * v0.h) contiguous = false;
+            if (!contiguous) {
+                HCHK(chunk_pred.need((k - i) * (size_t)v0.w * v0.h * 5));
+                dst = chunk_pred.p;
+            }
+            if (int r = net_predict_batch(nt, vols, grays, v0.h, v0.w, dst, false, nullptr, nullptr)) return r;
+            if (!contiguous) {
+                for (size_t q = i; q < k; q++)
+                    HCHK(hipMemcpyAsync(nt->preds.p + off[order[q]] * 5, dst + (q - i) * (size_t)v0.w * v0.h * 5,
+                                        (size_t)v0.w * v0.h * 5 * 4, hipMemcpyDeviceToDevice, c->stream));
+                HCHK(hipStreamSynchronize(c->stream));
+                chunk_pred.release();
+            }
+            i = k;
+        }
+        HCHK(hipEventRecord(c->ev[2], c->stream));
+        // 3. proposal init with the full source sets, then the passes and the gate
+        BatchPlan P;
+        if (int r = plan_16(c, jobs, njobs, opts, P, 1)) return r;
+        if (int r = upload_plan(c, P)) return r;
+        const int ncnt = 1 + P.n_passes;
+        HCHK(nt->ptrs.need(njobs));
+        std::vector<uintptr_t> tab(njobs);
+        for (int j = 0; j < njobs; j++) tab[j] = (uintptr_t)(nt->preds.p + off[j] * 5);
+        HCHK(hipMemcpyAsync(nt->ptrs.p, tab.data(), njobs * sizeof(uintptr_t), hipMemcpyHostToDevice, c->stream));
+        HCHK(launch_fill_u64(c->counters.p, ncnt + (P.n_passes + 1) / 2 + 1 + 4, 0ULL, c->stream));
+        u64* acc_counter = c->counters.p + ncnt + (P.n_passes + 1) / 2 + 1;
+        HCHK(launch_proposal_init(P.cfg, c->jobs.p, c->srcs.p, c->tiles.p, (int)P.tiles.size(), P.nh,
+                                  (const float* const*)nt->ptrs.p, conf_thr, c->counters.p, acc_counter, c->stream));
+        HCHK(hipEventRecord(c->ev[3], c->stream));
+        for (int p = 0; p < P.run_passes; p++) {
+            const float t = (float)p / (float)P.n_passes;
+            const float sig_inv = 0.02f * powf(0.28f, t * 4.0f);
+            const float sig_n = 0.30f * powf(0.42f, t * 4.0f);
+            HCHK(launch_pass(P.cfg, c->jobs.p, c->srcs.p, c->tiles.p, (int)P.tiles.size(), p & 1, (u64)p + 7, sig_inv,
+                             sig_n, c->counters.p + 1 + p, ((int*)(c->counters.p + ncnt)) + p, P.grid_p, c->stream));
+        }
+        HCHK(hipEventRecord(c->ev[4], c->stream));
+        HCHK(launch_gate(P.cfg, c->jobs.p, c->tiles.p, (int)P.tiles.size(), P.run_passes & 1, opts->cost_thr,
+                         c->stream));
+        HCHK(hipEventRecord(c->ev[5], c->stream));
+        HCHK(hipStreamSynchronize(c->stream));
+        float t_vol = 0, t_net = 0, t_init = 0, t_pass = 0, t_gate = 0;
+        (void)hipEventElapsedTime(&t_vol, c->ev[0], c->ev[1]);
+        (void)hipEventElapsedTime(&t_net, c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&t_init, c->ev[2], c->ev[3]);
+        (void)hipEventElapsedTime(&t_pass, c->ev[3], c->ev[4]);
+        (void)hipEventElapsedTime(&t_gate, c->ev[4], c->ev[5]);
+        std::vector<u64> cnt(ncnt);
+        HCHK(hipMemcpy(cnt.data(), c->counters.p, ncnt * sizeof(u64), hipMemcpyDeviceToHost));
+        u64 acc = 0;
+        HCHK(hipMemcpy(&acc, acc_counter, sizeof(u64), hipMemcpyDeviceToHost));
+        c->stats.n_eval_init = (int64_t)cnt[0];
+        c->stats.n_eval_pass = 0;
+        for (int p = 0; p < P.run_passes; p++) c->stats.n_eval_pass += (int64_t)cnt[1 + p];
+        c->stats.n_pixels = P.n_pixels;
+        c->stats.taps_per_eval_sum = P.taps;
+        c->stats.ms_init = t_vol + t_net + t_init;
+        c->stats.ms_pass = t_pass;
+        c->stats.ms_gate = t_gate;
+        c->stats.ms_learned_volume = t_vol;
+        c->stats.ms_learned_net = t_net;
+        c->stats.n_pass_launches = P.run_passes;
+        c->stats.graph_used = 0;
+        if (n_accepted) *n_accepted = (int64_t)acc;
+        for (int j = 0; j < njobs; j++) {
+            ViewState& rv = c->views[jobs[j].ref];
+            const size_t np = (size_t)rv.w * rv.h;
+            rv.has_depth = true;
+            if (jobs[j].depth) HCHK(hipMemcpyAsync(jobs[j].depth, rv.depth.p, np * 4, hipMemcpyDeviceToHost, c->stream));
+            if (jobs[j].cost) HCHK(hipMemcpyAsync(jobs[j].cost, rv.cost_out.p, np * 4, hipMemcpyDeviceToHost, c->stream));
+            if (jobs[j].normals)
+                HCHK(hipMemcpyAsync(jobs[j].normals, rv.normal.p, 3 * np * 4, hipMemcpyDeviceToHost, c->stream));
+        }
+        HCHK(hipStreamSynchronize(c->stream));
+        return MVS_OK;
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_estimate_depth_learned: ") + e.what());
+    }
 }
 
 }  // extern "C"

@@ -526,6 +526,137 @@ __global__ __launch_bounds__(TW* TH) void gate_kernel(const JobDesc* __restrict_
     }
 }
 
+// --------------------------------------------------------------------------
+// learned initialisation support
+// --------------------------------------------------------------------------
+
+__device__ __forceinline__ uint16_t f2bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Fronto-parallel census sweep volume: nplanes planes uniform in inverse
+// depth over [1/zmax, 1/zmin] (the init sweep's spacing with nplanes
+// hypotheses), each costed by plane_cost over the job's sources with no
+// abort (hint = +inf).  fp32 costs (optional) and bf16 network features
+// (cost - 0.5) * 2 are written per pixel.
+template <int TW, int TH>
+__global__ __launch_bounds__(TW* TH) void volume_kernel(const JobDesc* __restrict__ jobs,
+                                                        const SrcDesc* __restrict__ srcs,
+                                                        const int4* __restrict__ tiles, int ntiles, int nplanes,
+                                                        const VolOut* __restrict__ outs) {
+    constexpr int PW = TW + 4;
+    __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
+    __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
+    const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
+    const JobDesc& J = jobs[tl.x];
+    stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
+    __syncthreads();
+    const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
+    const int x = tl.y + lx, y = tl.z + ly;
+    if (x >= J.w || y >= J.h) return;
+    const int c = (ly + 2) * PW + (lx + 2);
+    const SrcDesc* S = srcs + J.src_off;
+    const VolOut& O = outs[tl.x];
+    const size_t i = (size_t)y * J.w + x;
+    for (int hi = 0; hi < nplanes; ++hi) {
+        const float inv = J.inv_min + (J.inv_max - J.inv_min) * (float)hi / (float)(nplanes - 1);
+        const float d = 1.0f / inv;
+        const float cst = eval_plane_fast<PW>(s_cen, s_ray, c, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, __builtin_inff());
+        if (O.volf) O.volf[i * nplanes + hi] = cst;
+        if (O.vol) O.vol[i * 32 + hi] = f2bf16((cst - 0.5f) * 2.0f);
+    }
+}
+
+// Initialisation from the proposal network: where the confidence reaches
+// conf_thr and the proposed normal faces the ray (n . r >= 0.1), the pixel
+// starts from the proposed plane with its plane_cost over all sources;
+// elsewhere it runs the plain plane-sweep initialisation.
+template <int TW, int TH, int EV>
+__global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __restrict__ jobs,
+                                                               const SrcDesc* __restrict__ srcs,
+                                                               const int4* __restrict__ tiles, int ntiles, int nh,
+                                                               const float* const* __restrict__ preds, float conf_thr,
+                                                               u64* __restrict__ counter, u64* __restrict__ accepted) {
+    constexpr int PW = TW + 4;
+    __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
+    __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
+    __shared__ unsigned s_cnt, s_acc;
+    const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
+    const JobDesc& J = jobs[tl.x];
+    if (threadIdx.x == 0) {
+        s_cnt = 0;
+        s_acc = 0;
+    }
+    stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
+    __syncthreads();
+    unsigned ne = 0, na = 0;
+    const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
+    const int x = tl.y + lx, y = tl.z + ly;
+    if (x < J.w && y < J.h) {
+        const int c = (ly + 2) * PW + (lx + 2);
+        const size_t i = (size_t)y * J.w + x;
+        const float* pr = preds[tl.x] + 5 * i;
+        const float inv_n = pr[0], nx = pr[1], ny = pr[2], nz = pr[3], conf = pr[4];
+        const float2 ri = s_ray[c];
+        const float a = (nx * ri.x + ny * ri.y) + nz;
+        if (conf >= conf_thr && a >= 0.1f && inv_n >= 0.0f && inv_n <= 1.0f) {
+            const float inv = J.inv_min + (J.inv_max - J.inv_min) * inv_n;
+            const float z = 1.0f / inv;
+            const float cst = eval_plane_fast<PW>(s_cen, s_ray, c, srcs + J.src_off, J.nsrc, nx, ny, nz, z * a,
+                                                  __builtin_inff());
+            ++ne;
+            ++na;
+            J.plane[0][i] = make_float4(nx, ny, nz, z);
+            J.cost[0][i] = fminf(cst, 1.0f);
+        } else {
+            unsigned ne2 = 0;
+            init_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne2);
+            ne += ne2;
+        }
+    }
+    atomicAdd(&s_cnt, ne);
+    atomicAdd(&s_acc, na);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (counter) atomicAdd(counter, (u64)s_cnt);
+        if (accepted) atomicAdd(accepted, (u64)s_acc);
+    }
+}
+
+// Network input [n][Hp][Wp][64] bf16: channels 0-31 sweep features, 32 the
+// normalised gray level, 33-63 zero; zero outside the h x w image.
+__global__ void assemble_input_kernel(const uint16_t* const* __restrict__ vols, const uint8_t* const* __restrict__ grays,
+                                      int n, int h, int w, int Hp, int Wp, uint16_t* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // one thread per (pixel, 8 channels)
+    const int64_t total = (int64_t)n * Hp * Wp * 8;
+    if (t >= total) return;
+    const int g = (int)(t & 7);
+    const int64_t p = t >> 3;
+    const int x = (int)(p % Wp);
+    const int y = (int)((p / Wp) % Hp);
+    const int b = (int)(p / ((int64_t)Wp * Hp));
+    typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+    us8 v = us8{0, 0, 0, 0, 0, 0, 0, 0};
+    if (x < w && y < h) {
+        const size_t i = (size_t)y * w + x;
+        if (g < 4) {
+            v = *(const us8*)(vols[b] + i * 32 + 8 * g);
+        } else if (g == 4) {
+            v[0] = f2bf16((float)grays[b][i] * (1.0f / 255.0f) - 0.5f);
+        }
+    }
+    *(us8*)(out + (size_t)p * 64 + 8 * g) = v;
+}
+
+// Plain device copy (counter calibration: reads and writes n16 * 16 bytes).
+__global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void fill_u64_kernel(u64* p, int n, u64 v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -570,6 +701,12 @@ hipError_t launch_prepare(const PrepView* d_views, int nviews, int nblocks, hipS
 int prepare_tiles(int w, int h, int& tx) {
     tx = (w + CT_W - 1) / CT_W;
     return tx * ((h + CT_H - 1) / CT_H);
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st) {
+    const size_t n16 = bytes / 16;
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, st, (const uint4*)src, (uint4*)dst, n16);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_u64(u64* p, int n, u64 v, hipStream_t st) {
@@ -633,6 +770,35 @@ int persistent_blocks_per_cu(const TileCfg& cfg) {
     TILE_CASES(X)
 #undef X
     return nb > 0 ? nb : 1;
+}
+
+hipError_t launch_volume(const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles, int nplanes,
+                         const VolOut* outs, hipStream_t st) {
+    hipLaunchKernelGGL((volume_kernel<16, 16>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles, nplanes, outs);
+    return hipGetLastError();
+}
+
+hipError_t launch_proposal_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles,
+                                int ntiles, int nh, const float* const* preds, float conf_thr, u64* counter,
+                                u64* accepted, hipStream_t st) {
+    if (cfg.tw == 16 && cfg.th == 16 && cfg.eval == 0) {
+        hipLaunchKernelGGL((proposal_init_kernel<16, 16, 0>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles,
+                           nh, preds, conf_thr, counter, accepted);
+    } else if (cfg.tw == 16 && cfg.th == 16) {
+        hipLaunchKernelGGL((proposal_init_kernel<16, 16, 1>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles,
+                           nh, preds, conf_thr, counter, accepted);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble(const uint16_t* const* vols, const uint8_t* const* grays, int n, int h, int w, int Hp,
+                           int Wp, uint16_t* out, hipStream_t st) {
+    const int64_t total = (int64_t)n * Hp * Wp * 8;
+    hipLaunchKernelGGL(assemble_input_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, vols, grays, n,
+                       h, w, Hp, Wp, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_debug_cost(const JobDesc& J, const SrcDesc* S, const float* q, int n, float* cost, int* ham,

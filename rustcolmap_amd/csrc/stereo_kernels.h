@@ -14,6 +14,12 @@ struct PrepView {
     int32_t tiles_x, tile_off;
 };
 
+// Outputs of the sweep-volume kernel for one job (either may be NULL).
+struct VolOut {
+    uint16_t* vol;   // [pix][32] bf16 network features
+    float* volf;     // [pix][nplanes] fp32 plane costs
+};
+
 // Launch shape of the stereo stages (picked per shape class by the tuner).
 struct TileCfg {
     int tw = 16, th = 16;
@@ -24,6 +30,7 @@ struct TileCfg {
 int prepare_tiles(int w, int h, int& tiles_x);
 hipError_t launch_prepare(const PrepView* d_views, int nviews, int nblocks, hipStream_t st);
 hipError_t launch_fill_u64(u64* p, int n, u64 v, hipStream_t st);
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
 hipError_t launch_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles,
                        int nh, u64* counter, hipStream_t st);
 hipError_t launch_pass(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles,
@@ -32,6 +39,13 @@ hipError_t launch_pass(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* s
 hipError_t launch_gate(const TileCfg& cfg, const JobDesc* jobs, const int4* tiles, int ntiles, int par,
                        float cost_thr, hipStream_t st);
 int persistent_blocks_per_cu(const TileCfg& cfg);
+hipError_t launch_volume(const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles, int nplanes,
+                         const VolOut* outs, hipStream_t st);
+hipError_t launch_proposal_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles,
+                                int ntiles, int nh, const float* const* preds, float conf_thr, u64* counter,
+                                u64* accepted, hipStream_t st);
+hipError_t launch_assemble(const uint16_t* const* vols, const uint8_t* const* grays, int n, int h, int w, int Hp,
+                           int Wp, uint16_t* out, hipStream_t st);
 hipError_t launch_debug_cost(const JobDesc& J, const SrcDesc* S, const float* q, int n, float* cost, int* ham,
                              int* ninv, float* peak, hipStream_t st);
 hipError_t launch_debug_rng(const int* xy, const u64* salt, int n, int ndraw, u64* seed_out, float* uni,

@@ -117,6 +117,8 @@ class MvsStats:
     n_points: int = 0
     n_pass_launches: int = 0
     graph_used: int = 0
+    ms_learned_volume: float = 0.0
+    ms_learned_net: float = 0.0
 
 
 class MvsContext:
@@ -412,3 +414,122 @@ def write_ply_binary(path: str, pts: np.ndarray):
     lib = _lib.load()
     p = np.ascontiguousarray(pts, POINT_DTYPE)
     check(lib.mvs_write_ply(str(path).encode(), ptr(p), len(p)), "mvs_write_ply")
+
+
+def _jobs_c(jobs):
+    arr = (MvsJobC * len(jobs))()
+    keep = []
+    for i, jb in enumerate(jobs):
+        ref, srcs, zmin, zmax = jb[:4]
+        s = np.ascontiguousarray(srcs, np.int32)
+        keep.append(s)
+        arr[i].ref, arr[i].k = int(ref), len(s)
+        arr[i].srcs = s.ctypes.data_as(C.POINTER(C.c_int32))
+        arr[i].zmin, arr[i].zmax = float(zmin), float(zmax)
+    return arr, keep
+
+
+class ProposalNet:
+    """The learned plane-proposal network (BF16 U-Net on MFMA) of a context."""
+
+    def __init__(self, ctx: MvsContext, seed: int = 0):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        h = C.c_void_p()
+        check(self._lib.mvs_net_create(ctx._h, C.c_uint64(seed), C.byref(h)), "mvs_net_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.mvs_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def num_params(self) -> int:
+        return int(self._lib.mvs_net_num_params(self._h))
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.num_params, np.float32)
+        check(self._lib.mvs_net_get_params(self._h, ptr(out)), "mvs_net_get_params")
+        return out
+
+    def set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != self.num_params:
+            raise ValueError("expected %d parameters, got %d" % (self.num_params, p.size))
+        check(self._lib.mvs_net_set_params(self._h, ptr(p)), "mvs_net_set_params")
+
+    def add_samples(self, jobs, targets) -> int:
+        arr, keep = _jobs_c(jobs)
+        tg = [np.ascontiguousarray(t, np.float32) for t in targets]
+        tp = (C.c_void_p * len(tg))(*[t.ctypes.data for t in tg])
+        r = self._lib.mvs_net_add_samples(self._h, arr, len(jobs), tp)
+        check(r, "mvs_net_add_samples")
+        return int(r)
+
+    def clear_samples(self):
+        check(self._lib.mvs_net_clear_samples(self._h), "mvs_net_clear_samples")
+
+    def train_step(self, idx, lr: float):
+        i = np.ascontiguousarray(idx, np.int32)
+        loss = np.zeros(4, np.float32)
+        check(self._lib.mvs_net_train_step(self._h, ptr(i), len(i), C.c_float(lr), ptr(loss)), "mvs_net_train_step")
+        return loss
+
+    def predict(self, job) -> np.ndarray:
+        arr, keep = _jobs_c([job])
+        v = self.ctx.views[job[0]]
+        out = np.empty((v.h, v.w, 5), np.float32)
+        check(self._lib.mvs_net_predict(self._h, arr, ptr(out)), "mvs_net_predict")
+        return out
+
+    def debug_grads(self, inp: np.ndarray, target: np.ndarray, h: int, w: int):
+        x = np.ascontiguousarray(inp, np.float32)
+        n, Hp, Wp, c = x.shape
+        assert c == 64
+        t = np.ascontiguousarray(target, np.float32)
+        grads = np.empty(self.num_params, np.float32)
+        pred = np.empty((n, h, w, 5), np.float32)
+        loss = np.empty(4, np.float32)
+        check(self._lib.mvs_net_debug_grads(self._h, ptr(x), ptr(t), n, Hp, Wp, h, w, ptr(grads), ptr(pred),
+                                            ptr(loss)), "mvs_net_debug_grads")
+        return grads, pred, loss
+
+
+def sweep_volume(ctx: MvsContext, job) -> np.ndarray:
+    """32-plane census sweep volume of a job: (h, w, 32) plane costs."""
+    arr, keep = _jobs_c([job])
+    v = ctx.views[job[0]]
+    out = np.empty((v.h, v.w, 32), np.float32)
+    check(ctx._lib.mvs_sweep_volume(ctx._h, arr, ptr(out)), "mvs_sweep_volume")
+    return out
+
+
+def estimate_depth_learned(ctx: MvsContext, net: ProposalNet, jobs, opts: MvsOptions, vol_k: int = 4,
+                           conf_thr: float = 0.5, download: bool = True):
+    """Learned-init stereo over `jobs` ((ref, srcs, zmin, zmax) tuples).
+    Returns ({ref: DepthMap} or None, number of pixels started from a proposal)."""
+    arr, keep = _jobs_c(jobs)
+    out = {}
+    if download:
+        for i, jb in enumerate(jobs):
+            v = ctx.views[jb[0]]
+            d = np.empty((v.h, v.w), np.float32)
+            c = np.empty((v.h, v.w), np.float32)
+            n = np.empty((v.h, v.w, 3), np.float32)
+            keep += [d, c, n]
+            arr[i].depth = d.ctypes.data_as(C.POINTER(C.c_float))
+            arr[i].cost = c.ctypes.data_as(C.POINTER(C.c_float))
+            arr[i].normals = n.ctypes.data_as(C.POINTER(C.c_float))
+            out[int(jb[0])] = DepthMap(d, c, n)
+    acc = C.c_int64()
+    oc = opts.to_c()
+    check(ctx._lib.mvs_estimate_depth_learned(ctx._h, net._h, arr, len(jobs), C.byref(oc), int(vol_k),
+                                              C.c_float(conf_thr), C.byref(acc)), "mvs_estimate_depth_learned")
+    return (out if download else None), int(acc.value)

@@ -86,6 +86,9 @@ class SceneSpec:
     persistence: float = 0.65       # amplitude ratio between successive octaves
     obj_region: Tuple[float, float] = (0.3, 1.8)   # object centres: x, z range (corner side of the arc)
     background: int = 128
+    room_kind: str = "corner"       # "corner": floor + walls x=0, z=0; "box": closed room |x|,|z| <= room/2, y in [0, wall_h]
+    orbit: bool = False             # cameras on a full circle around the vertical axis through the origin
+    flat_frac: float = 0.0          # fraction of objects rendered flat-shaded (textureless)
 
 
 @dataclass
@@ -194,6 +197,10 @@ def config_spec(name: str) -> SceneSpec:
         return SceneSpec(seed=3, n_views=64, sizes=((640, 480), (1280, 720), (1920, 1080), (2560, 1440)),
                          size_seeded=True, focal_rel=0.86, arc_deg=90.0, radius=4.0, arc_center=0.85,
                          n_objects=30, flat_patch=True, noise_sigma=2.0, tex_freq=24.0, persistence=0.65)
+    if name == "orbit100":       # 100 x 1920x1080 on a 360 deg orbit, radius 5, closed 6-plane room, 50 objects
+        return SceneSpec(seed=5, n_views=100, sizes=((1920, 1080),), focal=1650.0, radius=5.0, orbit=True,
+                         cam_h=1.6, target=(0.0, 0.9, 0.0), room_kind="box", room=16.0, wall_h=5.0, n_objects=50,
+                         obj_region=(-2.5, 2.5), flat_frac=0.1, noise_sigma=2.0, tex_freq=24.0, persistence=0.65)
     if name == "tiny":           # small scene for fast parity tests
         return SceneSpec(seed=11, n_views=5, sizes=((64, 48),), focal=60.0, arc_deg=40.0,
                          radius=4.0, arc_center=-1.0, n_objects=2, tex_freq=3.0)
@@ -209,8 +216,23 @@ def _objects(spec: SceneSpec, rng: np.random.Generator):
         x = float(rng.uniform(*spec.obj_region))
         z = float(rng.uniform(*spec.obj_region))
         y = size + float(rng.uniform(0.0, 1.0))
-        objs.append({"kind": kind, "c": (x, y, z), "s": size, "seed": int(spec.seed * 1000 + 500 + i)})
+        flat = bool(rng.random() < spec.flat_frac)
+        objs.append({"kind": kind, "c": (x, y, z), "s": size, "seed": int(spec.seed * 1000 + 500 + i), "flat": flat})
     return objs
+
+
+def _room_planes(spec: SceneSpec):
+    """(axis, coordinate, normal sign) of the room's planes."""
+    if spec.room_kind == "box":
+        L = spec.room / 2
+        return [(1, 0.0, 1.0), (0, -L, 1.0), (2, -L, 1.0), (0, L, -1.0), (2, L, -1.0), (1, spec.wall_h, -1.0)]
+    return [(1, 0.0, 1.0), (0, 0.0, 1.0), (2, 0.0, 1.0)]
+
+
+def _room_extent(spec: SceneSpec, axis: int):
+    if spec.room_kind == "box":
+        return (0.0, spec.wall_h) if axis == 1 else (-spec.room / 2, spec.room / 2)
+    return (0.0, spec.wall_h) if axis == 1 else (0.0, spec.room)
 
 
 def _view_sizes(spec: SceneSpec, rng: np.random.Generator):
@@ -232,11 +254,17 @@ def render_scene(spec: SceneSpec, device: Optional[str] = None, with_gt: bool = 
     for vi in range(spec.n_views):
         w, h = sizes[vi]
         f = spec.focal_rel * w if spec.focal_rel else spec.focal
-        frac = 0.5 if spec.n_views == 1 else vi / (spec.n_views - 1)
-        th = math.radians(45.0 - spec.arc_deg / 2 + spec.arc_deg * frac)
-        c = axis + spec.radius * np.array([math.cos(th), 0.0, math.sin(th)])
-        c[1] = spec.cam_h
-        R, t = look_at(c, np.array(spec.target))
+        if spec.orbit:
+            th = 2.0 * math.pi * vi / spec.n_views
+            c = spec.radius * np.array([math.cos(th), 0.0, math.sin(th)])
+            c[1] = spec.cam_h
+            R, t = look_at(c, np.array(spec.target))
+        else:
+            frac = 0.5 if spec.n_views == 1 else vi / (spec.n_views - 1)
+            th = math.radians(45.0 - spec.arc_deg / 2 + spec.arc_deg * frac)
+            c = axis + spec.radius * np.array([math.cos(th), 0.0, math.sin(th)])
+            c[1] = spec.cam_h
+            R, t = look_at(c, np.array(spec.target))
         gray, depth, normal = _render_view(spec, objs, w, h, f, R, t, device,
                                            noise_seed=spec.seed * 1000 + vi)
         v = View(w=w, h=h, f=float(f), cx=w * 0.5, cy=h * 0.5, k=0.0, R=R, t=t, gray=gray)
@@ -268,21 +296,21 @@ def _render_view(spec, objs, w, h, f, R, t, device, noise_seed):
         nrm = torch.where(m[..., None], n, nrm)
         sid = torch.where(m, torch.full_like(sid, idv), sid)
 
-    # room planes: axis a at coordinate 0, bounded by room/wall_h
-    for pid, a in enumerate((1, 0, 2)):   # floor (y), wall x=0, wall z=0
+    # room planes (axis, coordinate, normal sign); ids 0-2 are the corner planes
+    for pid, (a, coord, sgn) in enumerate(_room_planes(spec)):
         da = dw[..., a]
         with torch.no_grad():
-            tt = torch.where(da.abs() > 1e-12, -C[a] / da, torch.full_like(da, INF))
+            tt = torch.where(da.abs() > 1e-12, (coord - C[a]) / da, torch.full_like(da, INF))
         p = C + dw * tt[..., None]
         inside = torch.ones_like(tt, dtype=torch.bool)
         for b in range(3):
             if b == a:
                 continue
-            hi = spec.wall_h if b == 1 else spec.room
-            inside &= (p[..., b] >= -1e-9) & (p[..., b] <= hi)
+            lo, hi = _room_extent(spec, b)
+            inside &= (p[..., b] >= lo - 1e-9) & (p[..., b] <= hi + 1e-9)
         tt = torch.where(inside, tt, torch.full_like(tt, INF))
         n = torch.zeros(3, dtype=dt, device=device)
-        n[a] = 1.0
+        n[a] = sgn
         take(tt, n.expand(h, w, 3), pid)
     for oi, o in enumerate(objs):
         c = torch.tensor(o["c"], dtype=dt, device=device)
@@ -308,19 +336,24 @@ def _render_view(spec, objs, w, h, f, R, t, device, noise_seed):
             tf = tmax.min(-1).values
             tt = torch.where((tn <= tf) & (tn > 0), tn, torch.full_like(tn, INF))
             n = torch.nn.functional.one_hot(ax, 3).to(dt) * -torch.sign(torch.gather(dw, -1, ax[..., None]))
-        take(tt, n, 3 + oi)
+        take(tt, n, len(_room_planes(spec)) + oi)
     hit = torch.isfinite(best)
     p = C + dw * torch.where(hit, best, torch.zeros_like(best))[..., None]
     val = torch.full((h, w), float(spec.background), dtype=torch.float32, device=device)
     # planes: 2D value noise in the plane's own coordinates
-    for pid, (a, b1, b2) in enumerate(((1, 0, 2), (0, 2, 1), (2, 0, 1))):
+    for pid, (a, _, _) in enumerate(_room_planes(spec)):
+        b1, b2 = [b for b in (0, 1, 2) if b != a][:2] if a != 1 else (0, 2)
         m = sid == pid
         if m.any():
             nv = _noise2(p[..., b1][m], p[..., b2][m], spec.seed * 1000 + 100 + pid, spec.tex_freq, spec.octaves,
                          spec.persistence)
             val[m] = _tone(nv, spec.persistence)
+    nplanes = len(_room_planes(spec))
     for oi, o in enumerate(objs):
-        m = sid == 3 + oi
+        m = sid == nplanes + oi
+        if m.any() and o.get("flat"):
+            val[m] = 90.0 + 60.0 * (oi % 3)
+            continue
         if m.any():
             q = p[m] - torch.tensor(o["c"], dtype=dt, device=device)
             nv = _noise3(q + 50.0, o["seed"], spec.tex_freq, spec.octaves, spec.persistence)
