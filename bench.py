@@ -151,6 +151,8 @@ def main():
         t2 = time.time()
         tuned = ctx.autotune(jobs, opts, passes=1)
         log("autotune picked tile %dx%d persistent=%d evaluator=%d in %.1fs" % (tuned + (time.time() - t2,)))
+        log("autotune candidates (est. ms per full stereo run): " +
+            ", ".join("%s %.0f" % (n, t) for n, t in ctx.last_tune))
 
     for _ in range(args.warmup):
         ctx.estimate_depth_batch(jobs, opts, mode=args.mode, download=False)
@@ -207,7 +209,8 @@ def main():
                       args.cell),
                    "global_batch": len(views), "parallelism": "single-gpu",
                    "launch": None if tuned is None else
-                   {"tile": "%dx%d" % tuned[:2], "persistent": tuned[2], "evaluator": tuned[3]}},
+                   {"tile": "%dx%d" % tuned[:2], "persistent": tuned[2], "evaluator": tuned[3],
+                    "candidates_ms": {n: round(t, 1) for n, t in ctx.last_tune}}},
         "roofline": {"bound": "hbm", "kernel": "pass_kernel", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_launch": int(pass_bytes / n_passes),

@@ -148,12 +148,15 @@ int mvs_estimate_depth_batch(mvs_ctx *ctx, const mvs_job_t *jobs, int32_t njobs,
 /* Launch-shape autotuner (no counterpart in the reference): runs the batch
  * with every candidate (tile 16x16 / 32x8 / 16x8 / 8x8, reference-order or
  * hoisted-geometry evaluator, tile grid or persistent queue) for `passes`
- * passes, and records the fastest for the batch's shape class (longest
+ * passes (plus the divide-free-abort evaluator), and records the fastest
+ * for the batch's shape class (longest
  * side bucket x K bucket); later batches of that class use it.  Results
  * are identical for every candidate.  best (optional, 4 ints) receives
- * tile width, tile height, persistent flag, evaluator. */
+ * tile width, tile height, persistent flag, evaluator; cand_ms / cand_cfg
+ * (optional, room for 32 entries) the estimated full-run time of every
+ * candidate and its 4 parameters, n_cand their number. */
 int mvs_autotune(mvs_ctx *ctx, const mvs_job_t *jobs, int32_t njobs, const mvs_options_t *opts, int32_t passes,
-                 int32_t *best);
+                 int32_t *best, float *cand_ms, int32_t *cand_cfg, int32_t *n_cand);
 
 /* Download the resident depth / cost / normals of a view computed by the
  * last batch (NULL pointers skip). */
@@ -199,6 +202,10 @@ int mvs_get_stats(mvs_ctx *ctx, mvs_stats_t *out);
 int mvs_debug_plane_cost(mvs_ctx *ctx, int32_t ref, const int32_t *srcs, int32_t k, const float *q, int32_t n,
                          float *cost, int32_t *ham, int32_t *ninv, float *peak);
 
+/* Instrumentation: `(x) as i32` as the stereo kernels evaluate it on the
+ * device (truncate toward zero, saturate, NaN -> 0), for n floats. */
+int mvs_debug_cvt(mvs_ctx *ctx, const float *x, int32_t n, int32_t *out);
+
 /* Instrumentation: one plain device-to-device copy of `bytes` (a multiple
  * of 16) by the library's copy kernel; a reference for traffic counters. */
 int mvs_debug_copy(mvs_ctx *ctx, int64_t bytes);
@@ -208,6 +215,40 @@ int mvs_debug_copy(mvs_ctx *ctx, int64_t bytes);
  * uniform() values of its stream. */
 int mvs_debug_rng(mvs_ctx *ctx, const int32_t *xy, const uint64_t *salt, int32_t n, int32_t ndraw,
                   uint64_t *seed_out, float *uniform_out);
+
+/* ------------------------------------------------------------------------
+ * run_mvs orchestration (mvs/src/lib.rs:294-517)
+ * ------------------------------------------------------------------------ */
+
+/* run_mvs's host scheduling from the sparse reconstruction: co-visibility
+ * (lib.rs:224-243), stereo sources = top num_sources views with >= 10
+ * shared points (lib.rs:326-342; equal counts keep ascending view index),
+ * depth range per view from the 2 / 98 % percentiles of its sparse depths
+ * with the global fallback (lib.rs:245-282), fusion order (lib.rs:398-418).
+ * Sparse point p has xyz[3p..3p+3) and track views
+ * track_view[track_off[p] .. track_off[p+1]) (view indices; < 0 ignored).
+ * Outputs: src_off[n+1] / srcs[n * num_sources], zrange[2n],
+ * fusion_off[n+1] / fusion[n * n], covis[n * n] (optional).
+ * MVS_ERR_STATE when npoints < 8 (lib.rs:310-315). */
+int mvs_plan(const mvs_view_t *views, int32_t nviews, const double *xyz, const int32_t *track_off,
+             const int32_t *track_view, int64_t npoints, const mvs_options_t *opts, int32_t *src_off, int32_t *srcs,
+             double *zrange, int32_t *fusion_off, int32_t *fusion, uint32_t *covis);
+
+typedef struct mvs_run_stats {   /* MvsStats (lib.rs:72-79) + timings */
+    int32_t views, views_with_depth;
+    int64_t dense_points;
+    int64_t n_candidates;
+    double cell;
+    double stereo_ms, fuse_ms;   /* device time */
+    int64_t per_view_valid[4096];
+} mvs_run_stats_t;
+
+/* run_mvs for views at working resolution: plan, batched stereo of every
+ * view with sources, fusion with the robust cell, and (out_dir != NULL)
+ * <out_dir>/dense/dense.ply and mvs_stats.json. */
+int mvs_run_mvs(mvs_ctx *ctx, const mvs_view_t *views, int32_t nviews, const double *xyz, const int32_t *track_off,
+                const int32_t *track_view, int64_t npoints, const mvs_options_t *opts, const char *out_dir,
+                mvs_run_stats_t *stats);
 
 /* ------------------------------------------------------------------------
  * Learned plane proposals (no counterpart in the reference: an additional

@@ -54,6 +54,12 @@ class MvsStatsC(C.Structure):
     ]
 
 
+class MvsRunStatsC(C.Structure):
+    _fields_ = [("views", C.c_int32), ("views_with_depth", C.c_int32), ("dense_points", C.c_int64),
+                ("n_candidates", C.c_int64), ("cell", C.c_double), ("stereo_ms", C.c_double),
+                ("fuse_ms", C.c_double), ("per_view_valid", C.c_int64 * 4096)]
+
+
 # every symbol the header declares, with its ctypes signature
 _P = C.c_void_p
 _SIGS = {
@@ -70,7 +76,8 @@ _SIGS = {
                                         C.POINTER(MvsOptionsC), C.c_int32, _P, _P, _P, _P, _P]),
     "mvs_estimate_depth_batch": (C.c_int, [_P, C.POINTER(MvsJobC), C.c_int32, C.POINTER(MvsOptionsC), C.c_int32]),
     "mvs_get_depth": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
-    "mvs_autotune": (C.c_int, [_P, C.POINTER(MvsJobC), C.c_int32, C.POINTER(MvsOptionsC), C.c_int32, _P]),
+    "mvs_autotune": (C.c_int, [_P, C.POINTER(MvsJobC), C.c_int32, C.POINTER(MvsOptionsC), C.c_int32, _P, _P, _P,
+                               _P]),
     "mvs_fuse": (C.c_int, [_P, _P, _P, _P, C.c_double, C.POINTER(MvsOptionsC),
                            C.POINTER(C.POINTER(MvsPointC)), C.POINTER(C.c_int64)]),
     "mvs_fuse_depth_maps": (C.c_int, [_P, _P, _P, _P, C.c_double, C.POINTER(MvsOptionsC),
@@ -82,6 +89,11 @@ _SIGS = {
     "mvs_debug_plane_cost": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, C.c_int32, _P, _P, _P, _P]),
     "mvs_debug_rng": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, _P, _P]),
     "mvs_debug_copy": (C.c_int, [_P, C.c_int64]),
+    "mvs_debug_cvt": (C.c_int, [_P, _P, C.c_int32, _P]),
+    "mvs_plan": (C.c_int, [C.POINTER(MvsViewC), C.c_int32, _P, _P, _P, C.c_int64, C.POINTER(MvsOptionsC), _P, _P,
+                          _P, _P, _P, _P]),
+    "mvs_run_mvs": (C.c_int, [_P, C.POINTER(MvsViewC), C.c_int32, _P, _P, _P, C.c_int64, C.POINTER(MvsOptionsC),
+                             C.c_char_p, C.POINTER(MvsRunStatsC)]),
     "mvs_net_create": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
     "mvs_net_destroy": (None, [_P]),
     "mvs_net_num_params": (C.c_int64, [_P]),

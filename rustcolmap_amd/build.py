@@ -20,7 +20,7 @@ PKG = os.path.join(ROOT, "rustcolmap_amd")
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libmvs_hip.so")
 OBJDIR = os.path.join(PKG, "_build")
-HIP_SOURCES = ["stereo.hip", "fuse.hip", "net_kernels.hip", "net.hip", "capi.hip"]
+HIP_SOURCES = ["stereo.hip", "fuse.hip", "net_kernels.hip", "net.hip", "capi.hip", "plan.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -44,7 +44,7 @@ def build_library(verbose: bool = False) -> str:
     objs = []
     cmds = []
     for src in HIP_SOURCES:
-        obj = os.path.join(OBJDIR, src.replace(".hip", ".o"))
+        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         cmds.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=len(cmds)) as ex:

@@ -11,6 +11,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
+#include <sys/stat.h>
 
 #include <algorithm>
 #include <mutex>
@@ -79,6 +80,7 @@ struct ViewState {
     DBuf<float> cost[2];
     DBuf<float> depth, cost_out, normal;
     bool has_depth = false;
+    int64_t n_valid = 0;
     void release() {
         gray.release(); rgb.release(); census.release(); rays.release();
         plane[0].release(); plane[1].release(); cost[0].release(); cost[1].release();
@@ -321,6 +323,9 @@ static SrcDesc make_src(const ViewState& rv, const ViewState& sv) {
     return s;
 }
 
+struct BatchPlan;
+static int counter_words(const BatchPlan& P);
+
 struct BatchPlan {
     std::vector<JobDesc> jobs;
     std::vector<SrcDesc> srcs;
@@ -331,6 +336,12 @@ struct BatchPlan {
     int grid_p = 0;   // persistent grid size
     int64_t n_pixels = 0, taps = 0;
 };
+
+// counters buffer: [0] init evals, [1 + p] pass p evals, then the persistent
+// queue heads (one int per pass), then 4 spare words, then one valid-pixel
+// count per job
+static int valid_base(const BatchPlan& P) { return 1 + P.n_passes + (P.n_passes + 1) / 2 + 1 + 4; }
+static int counter_words(const BatchPlan& P) { return valid_base(P) + (int)P.jobs.size(); }
 
 static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_options_t* o, int mode, int max_passes,
                       BatchPlan& P) {
@@ -390,6 +401,23 @@ static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_op
         volatile float one = 1.0f;
         J.inv_min = one / J.zmax;
         J.inv_max = one / J.zmin;
+        // pinhole fast path of the third evaluator: dist = 1 + 0 * r2 is 1
+        // exactly while r2 stays finite; with |rs| <= 1, denom >= 0.05,
+        // zs >= 1e-3 and |d| <= 3 zmax (|ray| + 1), r2 is finite whenever
+        // 120 zmax (|ray| + 1)^2 + |ts| < 1e16, ensured by these bounds
+        {
+            double ray_max = 0.0;
+            if (rv.k == 0.0 && rv.f != 0.0) {
+                const double cand[4] = {fabs(rv.cx), fabs(rv.w - 1 - rv.cx), fabs(rv.cy), fabs(rv.h - 1 - rv.cy)};
+                for (double cv : cand) ray_max = std::max(ray_max, cv / fabs(rv.f));
+            }
+            bool ok = rv.k == 0.0 && rv.f != 0.0 && J.zmax < 1e6f && ray_max < 1e3;
+            for (int s2 = 0; s2 < jb.k && ok; s2++) {
+                const SrcDesc sd = make_src(rv, c->views[jb.srcs[s2]]);
+                for (int i = 0; i < 3; i++) ok = ok && fabsf(sd.ts[i]) < 1e12f;
+            }
+            J.pinhole_ok = ok ? 1 : 0;
+        }
         J.tiles_x = (rv.w + TW - 1) / TW;
         J.tiles_y = (rv.h + TH - 1) / TH;
         J.tile_off = tile_off;
@@ -431,7 +459,7 @@ static hipError_t enqueue_batch(mvs_ctx* c, const BatchPlan& P, float cost_thr, 
     hipStream_t st = c->stream;
     const int ntiles = (int)P.tiles.size();
     const int ncnt = 1 + P.n_passes;
-    hipError_t e = launch_fill_u64(c->counters.p, ncnt + (P.n_passes + 1) / 2 + 1, 0ULL, st);
+    hipError_t e = launch_fill_u64(c->counters.p, counter_words(P), 0ULL, st);
     if (e != hipSuccess) return e;
     if (timed) (void)hipEventRecord(c->ev[0], st);
     e = launch_init(P.cfg, c->jobs.p, c->srcs.p, c->tiles.p, ntiles, P.nh, c->counters.p, st);
@@ -464,7 +492,8 @@ static int run_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_opt
         HCHK(c->jobs.need(P.jobs.size()));
         HCHK(c->srcs.need(P.srcs.size()));
         HCHK(c->tiles.need(P.tiles.size()));
-        HCHK(c->counters.need(ncnt + (P.n_passes + 1) / 2 + 1));
+        HCHK(c->counters.need(counter_words(P)));
+        for (size_t j = 0; j < P.jobs.size(); j++) P.jobs[j].valid_count = c->counters.p + valid_base(P) + j;
         // the job array pointer is part of the signature
         P.sig.back() = (uint64_t)(uintptr_t)c->jobs.p;
         HCHK(hipMemcpyAsync(c->jobs.p, P.jobs.data(), P.jobs.size() * sizeof(JobDesc), hipMemcpyHostToDevice, c->stream));
@@ -475,7 +504,7 @@ static int run_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_opt
     if (sched == 1) {
         // unbatched schedule: every stage launched per job (same kernels)
         float ms_init = 0, ms_pass = 0, ms_gate = 0;
-        HCHK(launch_fill_u64(c->counters.p, ncnt + (P.n_passes + 1) / 2 + 1, 0ULL, c->stream));
+        HCHK(launch_fill_u64(c->counters.p, counter_words(P), 0ULL, c->stream));
         int toff = 0;
         for (int j = 0; j < njobs; j++) {
             const int nt = P.jobs[j].tiles_x * P.jobs[j].tiles_y;
@@ -551,8 +580,9 @@ static int run_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_opt
         c->stats.ms_gate = 0;
         c->stats.graph_used = 1;
     }
-    std::vector<u64> cnt(ncnt);
-    HCHK(hipMemcpy(cnt.data(), c->counters.p, ncnt * sizeof(u64), hipMemcpyDeviceToHost));
+    std::vector<u64> cnt(counter_words(P));
+    HCHK(hipMemcpy(cnt.data(), c->counters.p, cnt.size() * sizeof(u64), hipMemcpyDeviceToHost));
+    for (int j = 0; j < njobs; j++) c->views[jobs[j].ref].n_valid = (int64_t)cnt[valid_base(P) + j];
     c->stats.n_eval_init = (int64_t)cnt[0];
     c->stats.n_eval_pass = 0;
     for (int p = 0; p < P.run_passes; p++) c->stats.n_eval_pass += (int64_t)cnt[1 + p];
@@ -635,15 +665,15 @@ int mvs_estimate_depth_batch(mvs_ctx* c, const mvs_job_t* jobs, int32_t njobs, c
 }
 
 int mvs_autotune(mvs_ctx* c, const mvs_job_t* jobs, int32_t njobs, const mvs_options_t* opts, int32_t passes,
-                 int32_t* best) {
+                 int32_t* best, float* cand_ms, int32_t* cand_cfg, int32_t* n_cand) {
     if (!c || !opts || !jobs || njobs < 1) return fail(MVS_ERR_ARG, "mvs_autotune: bad arguments");
     if (passes < 1) passes = 2;
     try {
         std::lock_guard<std::mutex> lk(c->mu);
         if (int r = set_device(c)) return r;
-        static const int cand[][4] = {{16, 16, 0, 1}, {16, 16, 1, 1}, {32, 8, 0, 1}, {32, 8, 1, 1}, {16, 8, 0, 1},
-                                      {16, 8, 1, 1}, {8, 8, 0, 1},   {8, 8, 1, 1},  {16, 16, 0, 0}, {16, 16, 1, 0},
-                                      {32, 8, 0, 0}, {16, 8, 0, 0}};
+        static const int cand[][4] = {{16, 16, 0, 1}, {16, 16, 1, 1}, {32, 8, 0, 1}, {16, 8, 0, 1}, {8, 8, 0, 1},
+                                      {8, 8, 1, 1},   {16, 16, 0, 2}, {16, 16, 1, 2}, {32, 8, 0, 2}, {16, 8, 0, 2},
+                                      {8, 8, 0, 2},   {8, 8, 1, 2},   {16, 16, 0, 0}, {16, 8, 0, 0}};
         const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
         const int n_passes = (int)std::max<uint32_t>(opts->sweeps, 1) * 2;
         double best_t = 1e300;
@@ -672,11 +702,19 @@ int mvs_autotune(mvs_ctx* c, const mvs_job_t* jobs, int32_t njobs, const mvs_opt
                 }
                 t = c->stats.ms_init + c->stats.ms_pass * (double)n_passes / (double)passes;
             }
+            if (cand_ms) cand_ms[i] = (float)t;
+            if (cand_cfg) {
+                cand_cfg[4 * i] = cf.tw;
+                cand_cfg[4 * i + 1] = cf.th;
+                cand_cfg[4 * i + 2] = cf.persistent;
+                cand_cfg[4 * i + 3] = cf.eval;
+            }
             if (t < best_t) {
                 best_t = t;
                 best_cfg = cf;
             }
         }
+        if (n_cand) *n_cand = ncand;
         c->tuner.forced = false;
         c->tuner.table[Tuner::shape_class(maxw, maxh, maxk)] = best_cfg;
         c->graph.release();
@@ -958,6 +996,20 @@ int mvs_debug_copy(mvs_ctx* c, int64_t bytes) {
     return MVS_OK;
 }
 
+int mvs_debug_cvt(mvs_ctx* c, const float* x, int32_t n, int32_t* out) {
+    if (!c || !x || !out || n < 0) return fail(MVS_ERR_ARG, "mvs_debug_cvt: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int r = set_device(c)) return r;
+    HCHK(c->dbg.need((size_t)n * 8 + 64));
+    float* dx = (float*)c->dbg.p;
+    int* dout = (int*)(c->dbg.p + (((size_t)n * 4 + 15) & ~(size_t)15));
+    HCHK(hipMemcpyAsync(dx, x, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    HCHK(launch_debug_cvt(dx, n, dout, c->stream));
+    HCHK(hipMemcpyAsync(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+    return MVS_OK;
+}
+
 int mvs_debug_rng(mvs_ctx* c, const int32_t* xy, const uint64_t* salt, int32_t n, int32_t ndraw, uint64_t* seed_out,
                   float* uniform_out) {
     if (!c || !xy || !salt || n < 0 || ndraw < 0 || !seed_out || (ndraw > 0 && !uniform_out))
@@ -1019,12 +1071,13 @@ static int plan_16(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_optio
     return r;
 }
 
-static int upload_plan(mvs_ctx* c, const BatchPlan& P) {
+static int upload_plan(mvs_ctx* c, BatchPlan& P) {
+    for (size_t j = 0; j < P.jobs.size(); j++) P.jobs[j].valid_count = nullptr;
     c->graph.release();
     HCHK(c->jobs.need(P.jobs.size()));
     HCHK(c->srcs.need(P.srcs.size()));
     HCHK(c->tiles.need(P.tiles.size()));
-    HCHK(c->counters.need(1 + P.n_passes + (P.n_passes + 1) / 2 + 1 + 4));
+    HCHK(c->counters.need(counter_words(P)));
     HCHK(hipMemcpyAsync(c->jobs.p, P.jobs.data(), P.jobs.size() * sizeof(JobDesc), hipMemcpyHostToDevice, c->stream));
     HCHK(hipMemcpyAsync(c->srcs.p, P.srcs.data(), P.srcs.size() * sizeof(SrcDesc), hipMemcpyHostToDevice, c->stream));
     HCHK(hipMemcpyAsync(c->tiles.p, P.tiles.data(), P.tiles.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
@@ -1400,8 +1453,8 @@ int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, i
         std::vector<uintptr_t> tab(njobs);
         for (int j = 0; j < njobs; j++) tab[j] = (uintptr_t)(nt->preds.p + off[j] * 5);
         HCHK(hipMemcpyAsync(nt->ptrs.p, tab.data(), njobs * sizeof(uintptr_t), hipMemcpyHostToDevice, c->stream));
-        HCHK(launch_fill_u64(c->counters.p, ncnt + (P.n_passes + 1) / 2 + 1 + 4, 0ULL, c->stream));
-        u64* acc_counter = c->counters.p + ncnt + (P.n_passes + 1) / 2 + 1;
+        HCHK(launch_fill_u64(c->counters.p, counter_words(P), 0ULL, c->stream));
+        u64* acc_counter = c->counters.p + valid_base(P) - 4;
         HCHK(launch_proposal_init(P.cfg, c->jobs.p, c->srcs.p, c->tiles.p, (int)P.tiles.size(), P.nh,
                                   (const float* const*)nt->ptrs.p, conf_thr, c->counters.p, acc_counter, c->stream));
         HCHK(hipEventRecord(c->ev[3], c->stream));
@@ -1457,3 +1510,89 @@ int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, i
 }
 
 }  // extern "C"
+
+// --------------------------------------------------------------------------
+// run_mvs (mvs/src/lib.rs:294-517) without image decoding
+// --------------------------------------------------------------------------
+
+extern "C" int mvs_plan(const mvs_view_t* views, int32_t nviews, const double* xyz, const int32_t* track_off,
+                        const int32_t* track_view, int64_t npoints, const mvs_options_t* opts, int32_t* src_off,
+                        int32_t* srcs, double* zrange, int32_t* fusion_off, int32_t* fusion, uint32_t* covis);
+
+extern "C" int mvs_run_mvs(mvs_ctx* c, const mvs_view_t* views, int32_t nviews, const double* xyz,
+                           const int32_t* track_off, const int32_t* track_view, int64_t npoints,
+                           const mvs_options_t* opts, const char* out_dir, mvs_run_stats_t* st) {
+    if (!c || !views || nviews < 2 || !opts) return fail(MVS_ERR_ARG, "mvs_run_mvs: need at least 2 views");
+    if (npoints < 8)
+        return fail(MVS_ERR_STATE, "sparse reconstruction too small for MVS (" + std::to_string(npoints) +
+                                       " points, need >= 8)");
+    try {
+        const int n = nviews;
+        std::vector<int32_t> src_off(n + 1), srcs((size_t)n * std::max<uint32_t>(opts->num_sources, 1));
+        std::vector<double> zr(2 * n);
+        std::vector<int32_t> fus_off(n + 1), fus((size_t)n * n);
+        int r = mvs_plan(views, n, xyz, track_off, track_view, npoints, opts, src_off.data(), srcs.data(), zr.data(),
+                         fus_off.data(), fus.data(), nullptr);
+        if (r) return fail(r, "mvs_run_mvs: planning failed");
+        if ((r = mvs_upload_views(c, views, n))) return r;
+        std::vector<mvs_job_t> jobs;
+        for (int i = 0; i < n; i++) {
+            if (src_off[i + 1] == src_off[i]) continue;   // no source view: no depth map (lib.rs:355-357)
+            mvs_job_t j{};
+            j.ref = i;
+            j.k = src_off[i + 1] - src_off[i];
+            j.srcs = srcs.data() + src_off[i];
+            j.zmin = zr[2 * i];
+            j.zmax = zr[2 * i + 1];
+            jobs.push_back(j);
+        }
+        mvs_run_stats_t rs{};
+        rs.views = n;
+        rs.views_with_depth = (int32_t)jobs.size();
+        double stereo_ms = 0.0;
+        if (!jobs.empty()) {
+            if ((r = mvs_estimate_depth_batch(c, jobs.data(), (int32_t)jobs.size(), opts, 4))) return r;
+            stereo_ms = c->stats.ms_init + c->stats.ms_pass + c->stats.ms_gate;
+        }
+        std::vector<int32_t> has(n, 0);
+        for (auto& j : jobs) has[j.ref] = 1;
+        const double cell = mvs_fusion_cell(xyz, npoints, opts->voxel_rel);
+        mvs_point_t* pts = nullptr;
+        int64_t npts = 0;
+        if ((r = mvs_fuse(c, has.data(), fus_off.data(), fus.data(), cell, opts, &pts, &npts))) return r;
+        rs.dense_points = npts;
+        rs.cell = cell;
+        rs.stereo_ms = stereo_ms;
+        rs.fuse_ms = c->stats.ms_fuse;
+        rs.n_candidates = c->stats.n_candidates;
+        {
+            std::lock_guard<std::mutex> lk(c->mu);
+            for (int i = 0; i < n && i < 4096; i++) rs.per_view_valid[i] = has[i] ? c->views[i].n_valid : 0;
+        }
+        if (out_dir) {
+            const std::string dense = std::string(out_dir) + "/dense";
+            (void)mkdir(out_dir, 0755);
+            (void)mkdir(dense.c_str(), 0755);
+            r = mvs_write_ply((dense + "/dense.ply").c_str(), pts, npts);
+            if (r) {
+                mvs_free_points(pts);
+                return r;
+            }
+            FILE* f = fopen((dense + "/mvs_stats.json").c_str(), "w");
+            if (f) {
+                // same keys as lib.rs:503-507; seconds of device time here
+                fprintf(f,
+                        "{\"views\": %d, \"views_with_depth\": %d, \"dense_points\": %lld, \"stereo_s\": %.1f, "
+                        "\"fuse_s\": %.1f, \"elapsed_s\": %.1f}",
+                        rs.views, rs.views_with_depth, (long long)npts, stereo_ms / 1000.0, rs.fuse_ms / 1000.0,
+                        (stereo_ms + rs.fuse_ms) / 1000.0);
+                fclose(f);
+            }
+        }
+        mvs_free_points(pts);
+        if (st) *st = rs;
+        return MVS_OK;
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_run_mvs: ") + e.what());
+    }
+}

@@ -39,7 +39,8 @@ struct JobDesc {
     float inv_min, inv_max; // 1/zmax, 1/zmin (stereo.rs:248-249)
     int32_t tiles_x, tiles_y;
     int32_t tile_off;       // first tile of this job in the batch tile space
-    int32_t pad;
+    int32_t pinhole_ok;     // k == 0 sources may skip the distortion polynomial (finite-r2 bound holds)
+    u64* valid_count;       // gate: number of valid pixels (accumulated), or NULL
 };
 
 // A view as the fusion kernels see it (fuse.rs:99-162).

@@ -270,6 +270,130 @@ __device__ __forceinline__ float eval_plane_fast(const u64* __restrict__ s_cen, 
     return sum / (25.0f * (float)nsrc);
 }
 
+// Third evaluator, same values again, fewer VALU operations per tap:
+//   * the per-row abort test RN(sum / D) > thr is decided without a divide:
+//     with m the midpoint of thr and its successor (exact in f64, 25
+//     significant bits) the quotient rounds above thr iff sum > m * D, or
+//     sum == m * D and thr's last mantissa bit is odd (ties to even); m * D
+//     is exact in f64 because D <= 5 * 5 * K has few bits;
+//   * sources with k == 0 skip the distortion polynomial: dist = 1 + 0 * r2
+//     is exactly 1 whenever r2 is finite, which the host guarantees through
+//     a bound on zmax and the translation (JobDesc.pinhole_ok);
+//   * `(x) as i32` is v_cvt_i32_f32, which truncates, saturates and maps NaN
+//     to 0 exactly like Rust, so the bounds test is one unsigned compare;
+//   * popcount / 48 comes from a 49-entry LDS table of the exact quotients.
+__device__ __forceinline__ int cvt_i32_rust(float x) {
+    int r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+struct AbortThr {
+    double m;
+    bool odd, never;
+};
+
+__device__ __forceinline__ AbortThr make_abort_thr(float thr) {
+    AbortThr a;
+    a.never = !(thr < __builtin_inff());
+    const uint32_t b = __float_as_uint(thr);
+    const float succ = __uint_as_float(b + 1u);
+    a.m = 0.5 * ((double)thr + (double)succ);
+    a.odd = (b & 1u) != 0;
+    return a;
+}
+
+// RN(sum / D) > thr for sum >= 0, thr > 0 finite (or never when thr = inf)
+__device__ __forceinline__ bool abort_test(float sum, double D, const AbortThr& a) {
+    const double x = (double)sum, r = a.m * D;
+    return !a.never && (x > r || (x == r && a.odd));
+}
+
+template <bool PIN>
+__device__ __forceinline__ bool tap_index_v3(float z, float xr, float yr, bool vd, const SrcP& P, int& idx) {
+    const float xs = ((P.r0 * xr + P.r1 * yr) + P.r2 * z) + P.t0;
+    const float ys = ((P.r3 * xr + P.r4 * yr) + P.r5 * z) + P.t1;
+    const float zs = ((P.r6 * xr + P.r7 * yr) + P.r8 * z) + P.t2;
+    const float xn = xs / zs, yn = ys / zs;
+    float fu, fv;
+    if (PIN) {
+        fu = P.f * xn + P.cx;
+        fv = P.f * yn + P.cy;
+    } else {
+        const float dist = 1.0f + P.k * (xn * xn + yn * yn);
+        fu = P.f * xn * dist + P.cx;
+        fv = P.f * yn * dist + P.cy;
+    }
+    const int u = cvt_i32_rust(fu), v = cvt_i32_rust(fv);
+    const bool ok = vd && !(zs < 1e-3f) && ((unsigned)u < (unsigned)P.sw) && ((unsigned)v < (unsigned)P.sh);
+    idx = ok ? v * P.sw + u : 0;
+    return ok;
+}
+
+template <int PW, bool PIN>
+__device__ __forceinline__ bool row_v3(const u64* __restrict__ s_cen, int c, int r, const float* gz, const float* gx,
+                                       const float* gy, unsigned okd, const SrcP& P, const float* __restrict__ tab,
+                                       float& sum) {
+    int idx[5];
+    bool ok[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const int t = r * 5 + q;
+        ok[q] = tap_index_v3<PIN>(gz[t], gx[t], gy[t], (okd >> t) & 1u, P, idx[q]);
+    }
+    u64 cs[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) cs[q] = P.cen[idx[q]];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const u64 rc = s_cen[c + (r - 2) * PW + (q - 2)];
+        sum += ok[q] ? tab[__popcll(rc ^ cs[q])] : 1.0f;
+    }
+    return true;
+}
+
+template <int PW>
+__device__ __forceinline__ float eval_plane_v3(const u64* __restrict__ s_cen, const float2* __restrict__ s_ray, int c,
+                                               const SrcDesc* __restrict__ S, int nsrc, float nx, float ny, float nz,
+                                               float d, float thr, const float* __restrict__ tab, bool pin_ok) {
+    float gz[25], gx[25], gy[25];
+    unsigned okd = 0;
+#pragma unroll
+    for (int t = 0; t < 25; ++t) {
+        const float2 ray = s_ray[c + (t / 5 - 2) * PW + (t % 5 - 2)];
+        const float denom = (nx * ray.x + ny * ray.y) + nz;
+        okd |= (!(denom < 0.05f) ? 1u : 0u) << t;
+        const float z = d / denom;
+        gz[t] = z;
+        gx[t] = z * ray.x;
+        gy[t] = z * ray.y;
+    }
+    const AbortThr A = make_abort_thr(thr);
+    float sum = 0.0f;
+    for (int si = 0; si < nsrc; ++si) {
+        const SrcP P = load_src(S[si]);
+        const double fs = (double)(si + 1) * 5.0;
+        if (pin_ok && P.k == 0.0f) {
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                row_v3<PW, true>(s_cen, c, r, gz, gx, gy, okd, P, tab, sum);
+                if (abort_test(sum, (double)(r + 1) * fs, A)) return 1e9f;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                row_v3<PW, false>(s_cen, c, r, gz, gx, gy, okd, P, tab, sum);
+                if (abort_test(sum, (double)(r + 1) * fs, A)) return 1e9f;
+            }
+        }
+    }
+    return sum / (25.0f * (float)nsrc);
+}
+
+__device__ __forceinline__ void fill_pc_table(float* tab) {
+    if (threadIdx.x < 64) tab[threadIdx.x] = pc_over_48((int)threadIdx.x);
+}
+
 // --------------------------------------------------------------------------
 // tile staging
 // --------------------------------------------------------------------------
@@ -304,7 +428,7 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
 
 template <int TW, int TH, int EV>
 __device__ __forceinline__ void init_tile(const JobDesc& J, const SrcDesc* __restrict__ srcs, int x0, int y0, int nh,
-                                          u64* s_cen, float2* s_ray, unsigned& ne) {
+                                          u64* s_cen, float2* s_ray, unsigned& ne, const float* s_tab) {
     constexpr int PW = TW + 4;
     const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
     const int x = x0 + lx, y = y0 + ly;
@@ -317,8 +441,10 @@ __device__ __forceinline__ void init_tile(const JobDesc& J, const SrcDesc* __res
         const float inv = inv_min + (inv_max - inv_min) * (float)hi / (float)(nh - 1);
         const float d = 1.0f / inv;
         const float thr = (best_c + 0.02f) + 0.12f;
-        const float c = EV ? eval_plane_fast<PW>(s_cen, s_ray, R.c, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, thr)
-                           : eval_plane<false>(R, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, thr, nullptr);
+        const float c = EV == 2 ? eval_plane_v3<PW>(s_cen, s_ray, R.c, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, thr, s_tab,
+                                                    J.pinhole_ok)
+                        : EV == 1 ? eval_plane_fast<PW>(s_cen, s_ray, R.c, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, thr)
+                                  : eval_plane<false>(R, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, thr, nullptr);
         ++ne;
         if (c < best_c) { best_c = c; best_d = d; }
     }
@@ -334,7 +460,7 @@ __device__ __forceinline__ void init_tile(const JobDesc& J, const SrcDesc* __res
 template <int TW, int TH, int EV>
 __device__ __forceinline__ void pass_tile(const JobDesc& J, const SrcDesc* __restrict__ srcs, int x0, int y0, int par,
                                           u64 salt, float sig_inv, float sig_n, u64* s_cen, float2* s_ray,
-                                          unsigned& ne) {
+                                          unsigned& ne, const float* s_tab) {
     constexpr int PW = TW + 4;
     const int w = J.w, h = J.h;
     const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
@@ -406,8 +532,10 @@ __device__ __forceinline__ void pass_tile(const JobDesc& J, const SrcDesc* __res
             cnz = nnz;
         }
         const float thr = (best_c + 0.02f) + 0.12f;
-        const float cst = EV ? eval_plane_fast<PW>(s_cen, s_ray, c, S, nsrc, cnx, cny, cnz, d_plane, thr)
-                             : eval_plane<false>(R, S, nsrc, cnx, cny, cnz, d_plane, thr, nullptr);
+        const float cst = EV == 2 ? eval_plane_v3<PW>(s_cen, s_ray, c, S, nsrc, cnx, cny, cnz, d_plane, thr, s_tab,
+                                                      J.pinhole_ok)
+                          : EV == 1 ? eval_plane_fast<PW>(s_cen, s_ray, c, S, nsrc, cnx, cny, cnz, d_plane, thr)
+                                    : eval_plane<false>(R, S, nsrc, cnx, cny, cnz, d_plane, thr, nullptr);
         ++ne;
         if (cst < best_c) { best_c = cst; b0 = cnx; b1 = cny; b2 = cnz; bd = d_plane; }
     }
@@ -444,13 +572,15 @@ __global__ __launch_bounds__(TW* TH) void init_kernel(const JobDesc* __restrict_
     __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
     __shared__ unsigned s_cnt;
+    __shared__ float s_tab[64];
     const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
     const JobDesc& J = jobs[tl.x];
     if (threadIdx.x == 0) s_cnt = 0;
+    fill_pc_table(s_tab);
     stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
     __syncthreads();
     unsigned ne = 0;
-    init_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne);
+    init_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne, s_tab);
     block_count<TW, TH>(ne, &s_cnt, counter);
 }
 
@@ -461,13 +591,15 @@ __global__ __launch_bounds__(TW* TH) void pass_kernel(const JobDesc* __restrict_
     __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
     __shared__ unsigned s_cnt;
+    __shared__ float s_tab[64];
     const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
     const JobDesc& J = jobs[tl.x];
     if (threadIdx.x == 0) s_cnt = 0;
+    fill_pc_table(s_tab);
     stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
     __syncthreads();
     unsigned ne = 0;
-    pass_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, par, salt, sig_inv, sig_n, s_cen, s_ray, ne);
+    pass_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, par, salt, sig_inv, sig_n, s_cen, s_ray, ne, s_tab);
     block_count<TW, TH>(ne, &s_cnt, counter);
 }
 
@@ -483,7 +615,9 @@ __global__ __launch_bounds__(TW* TH) void pass_persistent_kernel(const JobDesc* 
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
     __shared__ unsigned s_cnt;
     __shared__ int s_tile;
+    __shared__ float s_tab[64];
     if (threadIdx.x == 0) s_cnt = 0;
+    fill_pc_table(s_tab);
     unsigned ne = 0;
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(queue, 1);
@@ -494,7 +628,7 @@ __global__ __launch_bounds__(TW* TH) void pass_persistent_kernel(const JobDesc* 
         const JobDesc& J = jobs[tl.x];
         stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
         __syncthreads();
-        pass_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, par, salt, sig_inv, sig_n, s_cen, s_ray, ne);
+        pass_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, par, salt, sig_inv, sig_n, s_cen, s_ray, ne, s_tab);
         __syncthreads();   // LDS tile and s_tile are reused by the next iteration
     }
     block_count<TW, TH>(ne, &s_cnt, counter);
@@ -507,23 +641,31 @@ __global__ __launch_bounds__(TW* TH) void pass_persistent_kernel(const JobDesc* 
 template <int TW, int TH>
 __global__ __launch_bounds__(TW* TH) void gate_kernel(const JobDesc* __restrict__ jobs, const int4* __restrict__ tiles,
                                                       int par, float cost_thr) {
+    __shared__ unsigned s_valid;
     const int4 tl = tiles[blockIdx.x];
     const JobDesc& J = jobs[tl.x];
+    if (threadIdx.x == 0) s_valid = 0;
+    __syncthreads();
     const int x = tl.y + threadIdx.x % TW, y = tl.z + threadIdx.x / TW;
-    if (x >= J.w || y >= J.h) return;
-    const size_t i = (size_t)y * J.w + x;
-    const float4 p = J.plane[par][i];
-    const float cst = J.cost[par][i];
-    const int tex = __popcll(J.census[i]);
-    const bool textured = tex >= 4 && tex <= 44;
-    const bool ok = textured && cst < cost_thr && p.w > J.zmin * 0.5f && p.w < J.zmax * 2.0f;
-    J.depth_out[i] = ok ? p.w : 0.0f;
-    J.cost_out[i] = cst;
-    if (J.normal_out) {
-        J.normal_out[3 * i] = p.x;
-        J.normal_out[3 * i + 1] = p.y;
-        J.normal_out[3 * i + 2] = p.z;
+    if (x < J.w && y < J.h) {
+        const size_t i = (size_t)y * J.w + x;
+        const float4 p = J.plane[par][i];
+        const float cst = J.cost[par][i];
+        const int tex = __popcll(J.census[i]);
+        const bool textured = tex >= 4 && tex <= 44;
+        const bool ok = textured && cst < cost_thr && p.w > J.zmin * 0.5f && p.w < J.zmax * 2.0f;
+        J.depth_out[i] = ok ? p.w : 0.0f;
+        J.cost_out[i] = cst;
+        if (J.normal_out) {
+            J.normal_out[3 * i] = p.x;
+            J.normal_out[3 * i + 1] = p.y;
+            J.normal_out[3 * i + 2] = p.z;
+        }
+        // the reference counts depth > 0 (lib.rs:361)
+        if (ok && p.w > 0.0f) atomicAdd(&s_valid, 1u);
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && J.valid_count && s_valid) atomicAdd(J.valid_count, (u64)s_valid);
 }
 
 // --------------------------------------------------------------------------
@@ -584,12 +726,14 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
     __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
     __shared__ unsigned s_cnt, s_acc;
+    __shared__ float s_tab[64];
     const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
     const JobDesc& J = jobs[tl.x];
     if (threadIdx.x == 0) {
         s_cnt = 0;
         s_acc = 0;
     }
+    fill_pc_table(s_tab);
     stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
     __syncthreads();
     unsigned ne = 0, na = 0;
@@ -613,7 +757,7 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
             J.cost[0][i] = fminf(cst, 1.0f);
         } else {
             unsigned ne2 = 0;
-            init_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne2);
+            init_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne2, s_tab);
             ne += ne2;
         }
     }
@@ -680,6 +824,11 @@ __global__ void debug_cost_kernel(JobDesc J, const SrcDesc* __restrict__ S, cons
     peak[t] = info.peak;
 }
 
+__global__ void debug_cvt_kernel(const float* x, int n, int* out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = cvt_i32_rust(x[t]);
+}
+
 __global__ void debug_rng_kernel(const int* xy, const u64* salt, int n, int ndraw, u64* seed_out, float* uni) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -715,7 +864,9 @@ hipError_t launch_fill_u64(u64* p, int n, u64 v, hipStream_t st) {
     return hipGetLastError();
 }
 
-#define TILE_CASES(X) X(16, 16, 0) X(16, 16, 1) X(32, 8, 0) X(32, 8, 1) X(16, 8, 0) X(16, 8, 1) X(8, 8, 1)
+#define TILE_CASES(X) \
+    X(16, 16, 0) X(16, 16, 1) X(16, 16, 2) X(32, 8, 0) X(32, 8, 1) X(32, 8, 2) X(16, 8, 0) X(16, 8, 1) X(16, 8, 2) \
+    X(8, 8, 1) X(8, 8, 2)
 
 hipError_t launch_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles,
                        int nh, u64* counter, hipStream_t st) {
@@ -805,6 +956,12 @@ hipError_t launch_debug_cost(const JobDesc& J, const SrcDesc* S, const float* q,
                              int* ninv, float* peak, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(debug_cost_kernel, dim3((n + 63) / 64), dim3(64), 0, st, J, S, q, n, cost, ham, ninv, peak);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_cvt(const float* x, int n, int* out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(debug_cvt_kernel, dim3((n + 63) / 64), dim3(64), 0, st, x, n, out);
     return hipGetLastError();
 }
 

@@ -24,7 +24,7 @@ struct VolOut {
 struct TileCfg {
     int tw = 16, th = 16;
     int persistent = 0;   // 1: persistent grid over an atomic tile queue
-    int eval = 1;         // plane_cost evaluator: 0 = reference loop order, 1 = hoisted geometry
+    int eval = 1;         // plane_cost evaluator: 0 = reference loop order, 1 = hoisted geometry, 2 = + no-divide abort
 };
 
 int prepare_tiles(int w, int h, int& tiles_x);
@@ -48,6 +48,7 @@ hipError_t launch_assemble(const uint16_t* const* vols, const uint8_t* const* gr
                            int Wp, uint16_t* out, hipStream_t st);
 hipError_t launch_debug_cost(const JobDesc& J, const SrcDesc* S, const float* q, int n, float* cost, int* ham,
                              int* ninv, float* peak, hipStream_t st);
+hipError_t launch_debug_cvt(const float* x, int n, int* out, hipStream_t st);
 hipError_t launch_debug_rng(const int* xy, const u64* salt, int n, int ndraw, u64* seed_out, float* uni,
                             hipStream_t st);
 

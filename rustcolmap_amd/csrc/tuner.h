@@ -28,14 +28,15 @@ struct Tuner {
         return {sb, kb};
     }
 
-    // "WxH[p][r]": tile shape, p = persistent queue, r = reference-order evaluator
+    // "WxH[p][r|v]": tile shape, p = persistent queue, r = reference-order
+    // evaluator, v = divide-free-abort evaluator (default: hoisted geometry)
     static bool parse(const char* s, TileCfg& c) {
         int w = 0, h = 0, used = 0;
         if (sscanf(s, "%dx%d%n", &w, &h, &used) < 2) return false;
         c.tw = w;
         c.th = h;
         c.persistent = strchr(s + used, 'p') ? 1 : 0;
-        c.eval = strchr(s + used, 'r') ? 0 : 1;
+        c.eval = strchr(s + used, 'r') ? 0 : (strchr(s + used, 'v') ? 2 : 1);
         return true;
     }
 

@@ -121,6 +121,37 @@ class MvsStats:
     ms_learned_net: float = 0.0
 
 
+def views_c(views: Sequence[ViewData]):
+    """mvs_view_t array (and the arrays it points into) for `views`."""
+    arr = (MvsViewC * len(views))()
+    keep = []
+    for i, v in enumerate(views):
+        g = np.ascontiguousarray(v.gray, np.uint8)
+        if g.shape != (v.h, v.w):
+            raise ValueError("view %d: gray shape %s != (h, w) = (%d, %d)" % (i, g.shape, v.h, v.w))
+        keep.append(g)
+        c = arr[i]
+        c.w, c.h = int(v.w), int(v.h)
+        c.f, c.cx, c.cy, c.k = float(v.cam.f), float(v.cam.cx), float(v.cam.cy), float(v.cam.k)
+        R = np.asarray(v.pose.r, np.float64).reshape(9)
+        t = np.asarray(v.pose.t, np.float64).reshape(3)
+        for j in range(9):
+            c.R[j] = float(R[j])
+        for j in range(3):
+            c.t[j] = float(t[j])
+        c.gray = g.ctypes.data_as(C.POINTER(C.c_uint8))
+        if v.rgb is not None:
+            rgb = np.ascontiguousarray(v.rgb, np.uint8)
+            if rgb.size != 3 * v.w * v.h:
+                raise ValueError("view %d: rgb has %d bytes, expected %d" % (i, rgb.size, 3 * v.w * v.h))
+            keep.append(rgb)
+            c.rgb = rgb.ctypes.data_as(C.POINTER(C.c_uint8))
+        else:
+            c.rgb = None
+    return arr, keep
+
+
+
 class MvsContext:
     """One device, one HIP stream, the device-resident views."""
 
@@ -145,31 +176,7 @@ class MvsContext:
 
     # -- views ---------------------------------------------------------------
     def upload_views(self, views: Sequence[ViewData]):
-        arr = (MvsViewC * len(views))()
-        keep = []
-        for i, v in enumerate(views):
-            g = np.ascontiguousarray(v.gray, np.uint8)
-            if g.shape != (v.h, v.w):
-                raise ValueError("view %d: gray shape %s != (h, w) = (%d, %d)" % (i, g.shape, v.h, v.w))
-            keep.append(g)
-            c = arr[i]
-            c.w, c.h = int(v.w), int(v.h)
-            c.f, c.cx, c.cy, c.k = float(v.cam.f), float(v.cam.cx), float(v.cam.cy), float(v.cam.k)
-            R = np.asarray(v.pose.r, np.float64).reshape(9)
-            t = np.asarray(v.pose.t, np.float64).reshape(3)
-            for j in range(9):
-                c.R[j] = float(R[j])
-            for j in range(3):
-                c.t[j] = float(t[j])
-            c.gray = g.ctypes.data_as(C.POINTER(C.c_uint8))
-            if v.rgb is not None:
-                rgb = np.ascontiguousarray(v.rgb, np.uint8)
-                if rgb.size != 3 * v.w * v.h:
-                    raise ValueError("view %d: rgb has %d bytes, expected %d" % (i, rgb.size, 3 * v.w * v.h))
-                keep.append(rgb)
-                c.rgb = rgb.ctypes.data_as(C.POINTER(C.c_uint8))
-            else:
-                c.rgb = None
+        arr, keep = views_c(views)
         check(self._lib.mvs_upload_views(self._h, arr, len(views)), "mvs_upload_views")
         self.views = list(views)
 
@@ -253,8 +260,14 @@ class MvsContext:
             arr[i].srcs = s.ctypes.data_as(C.POINTER(C.c_int32))
             arr[i].zmin, arr[i].zmax = float(zmin), float(zmax)
         best = np.zeros(4, np.int32)
+        ms = np.zeros(32, np.float32)
+        cfg = np.zeros((32, 4), np.int32)
+        nc = np.zeros(1, np.int32)
         oc = opts.to_c()
-        check(self._lib.mvs_autotune(self._h, arr, len(jobs), C.byref(oc), int(passes), ptr(best)), "mvs_autotune")
+        check(self._lib.mvs_autotune(self._h, arr, len(jobs), C.byref(oc), int(passes), ptr(best), ptr(ms), ptr(cfg),
+                                     ptr(nc)), "mvs_autotune")
+        self.last_tune = [("%dx%d%s%s" % (c[0], c[1], "p" if c[2] else "", "rfv"[c[3]] if c[3] != 1 else ""),
+                           float(t)) for c, t in zip(cfg[:nc[0]], ms[:nc[0]])]
         return tuple(int(b) for b in best)
 
     def get_depth(self, view: int) -> DepthMap:
@@ -533,3 +546,53 @@ def estimate_depth_learned(ctx: MvsContext, net: ProposalNet, jobs, opts: MvsOpt
     check(ctx._lib.mvs_estimate_depth_learned(ctx._h, net._h, arr, len(jobs), C.byref(oc), int(vol_k),
                                               C.c_float(conf_thr), C.byref(acc)), "mvs_estimate_depth_learned")
     return (out if download else None), int(acc.value)
+
+
+def _tracks_c(xyz, tracks):
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    off = np.zeros(len(tracks) + 1, np.int32)
+    for i, t in enumerate(tracks):
+        off[i + 1] = off[i] + len(t)
+    flat = np.asarray([v for t in tracks for v in t] or [0], np.int32)
+    return xyz, off, flat
+
+
+def plan(views: Sequence[ViewData], xyz: np.ndarray, tracks: Sequence[Sequence[int]], opts: MvsOptions):
+    """run_mvs's host scheduling (lib.rs:224-282, 326-347, 398-418):
+    returns (sources, (zmin, zmax) per view, fusion order, covisibility)."""
+    lib = _lib.load()
+    n = len(views)
+    arr, keep = views_c(views)
+    xyz, off, flat = _tracks_c(xyz, tracks)
+    so = np.zeros(n + 1, np.int32)
+    sr = np.zeros(max(1, n * max(1, int(opts.num_sources))), np.int32)
+    zr = np.zeros(2 * n, np.float64)
+    fo = np.zeros(n + 1, np.int32)
+    fu = np.zeros(n * n, np.int32)
+    cov = np.zeros((n, n), np.uint32)
+    oc = opts.to_c()
+    r = lib.mvs_plan(arr, n, ptr(xyz), ptr(off), ptr(flat), len(xyz), C.byref(oc), ptr(so), ptr(sr), ptr(zr),
+                     ptr(fo), ptr(fu), ptr(cov))
+    if r != 0:
+        raise _lib.MvsError(r, "mvs_plan", "sparse reconstruction too small for MVS" if r == -4 else "bad input")
+    sources = [list(sr[so[i]:so[i + 1]]) for i in range(n)]
+    fusion = [list(fu[fo[i]:fo[i + 1]]) for i in range(n)]
+    return sources, [(float(zr[2 * i]), float(zr[2 * i + 1])) for i in range(n)], fusion, cov
+
+
+def run_mvs(views: Sequence[ViewData], xyz: np.ndarray, tracks: Sequence[Sequence[int]], out_dir: Optional[str],
+            opts: MvsOptions, ctx: Optional[MvsContext] = None) -> dict:
+    """run_mvs (lib.rs:294-517) for views at working resolution: plan,
+    batched stereo, fusion, <out_dir>/dense/dense.ply and mvs_stats.json.
+    Returns MvsStats' fields plus timings."""
+    ctx = ctx or default_context()
+    arr, keep = views_c(views)
+    xyz, off, flat = _tracks_c(xyz, tracks)
+    st = _lib.MvsRunStatsC()
+    oc = opts.to_c()
+    check(ctx._lib.mvs_run_mvs(ctx._h, arr, len(views), ptr(xyz), ptr(off), ptr(flat), len(xyz), C.byref(oc),
+                               None if out_dir is None else str(out_dir).encode(), C.byref(st)), "mvs_run_mvs")
+    ctx.views = list(views)
+    return {"views": st.views, "views_with_depth": st.views_with_depth, "dense_points": st.dense_points,
+            "per_view_valid": [int(st.per_view_valid[i]) for i in range(min(len(views), 4096))],
+            "n_candidates": st.n_candidates, "cell": st.cell, "stereo_ms": st.stereo_ms, "fuse_ms": st.fuse_ms}

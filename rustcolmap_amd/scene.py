@@ -389,3 +389,39 @@ def select_sources(views: Sequence[View], k: int) -> List[List[int]]:
         cand.sort()
         out.append([j for _, j in cand[:k]])
     return out
+
+
+def sparse_points(sc: Scene, n: int, seed: int = 0, rel_tol: float = 0.01):
+    """A synthetic sparse reconstruction: n surface points sampled from the
+    ground-truth depth of random views, each with the track of views that
+    see it (projection inside the image and ground-truth depth within
+    rel_tol), as COLMAP's points3D would carry.  Returns (xyz (n, 3), tracks)."""
+    rng = np.random.default_rng(seed)
+    views = sc.views
+    pts, tracks = [], []
+    tries = 0
+    while len(pts) < n and tries < 50 * n:
+        tries += 1
+        vi = int(rng.integers(len(views)))
+        v = views[vi]
+        x, y = int(rng.integers(v.w)), int(rng.integers(v.h))
+        z = float(v.gt_depth[y, x])
+        if z <= 0:
+            continue
+        Xc = np.array([(x - v.cx) / v.f * z, (y - v.cy) / v.f * z, z])
+        Xw = v.R.T @ (Xc - v.t)
+        tr = []
+        for ui, u in enumerate(views):
+            q = u.R @ Xw + u.t
+            if q[2] <= 1e-6:
+                continue
+            px, py = u.f * q[0] / q[2] + u.cx, u.f * q[1] / q[2] + u.cy
+            ix, iy = int(round(px)), int(round(py))
+            if 0 <= ix < u.w and 0 <= iy < u.h:
+                zu = float(u.gt_depth[iy, ix])
+                if zu > 0 and abs(zu - q[2]) <= rel_tol * q[2]:
+                    tr.append(ui)
+        if len(tr) >= 2:
+            pts.append(Xw)
+            tracks.append(tr)
+    return np.asarray(pts, np.float64).reshape(-1, 3), tracks

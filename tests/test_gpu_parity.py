@@ -40,6 +40,30 @@ def test_view_census_and_upload(ctx):
         assert np.array_equal(ctx.view_census(i), oracle.census(v.gray))
 
 
+def test_cvt_matches_rust_as_i32(ctx):
+    """v_cvt_i32_f32 as used by the stereo kernels: truncate toward zero,
+    saturate, NaN -> 0 -- Rust's `f32 as i32`."""
+    import ctypes as C
+    from rustcolmap_amd import _lib
+    xs = np.array([0.0, -0.0, 0.5, -0.5, -0.999, -1.0, -1.5, 1.999, 2147483520.0, 2147483648.0, 3e9, -2147483648.0,
+                   -3e9, np.inf, -np.inf, np.nan, -np.nan, 1e-45, -1e-45, 123456.7, -123456.7, 1919.9999,
+                   np.float32(np.uint32(0x7fc00001).view(np.float32))], np.float32)
+    out = np.empty(len(xs), np.int32)
+    _lib.check(_lib.load().mvs_debug_cvt(ctx._h, xs.ctypes.data_as(C.c_void_p), len(xs),
+                                         out.ctypes.data_as(C.c_void_p)), "mvs_debug_cvt")
+
+    def rust(x):
+        if np.isnan(x):
+            return 0
+        if x >= 2147483648.0:
+            return 2147483647
+        if x <= -2147483648.0:
+            return -2147483648
+        return int(np.trunc(x))
+    for x, o in zip(xs, out):
+        assert int(o) == rust(float(x)), (x, o)
+
+
 def test_rng_streams(ctx):
     import oracle
     rng = np.random.default_rng(1)
@@ -246,7 +270,8 @@ def test_fusion_resident_matches_host_maps(ctx):
     assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("shape", ["16x16", "16x16r", "32x8", "32x8p", "16x8r", "16x8p", "8x8", "16x16pr", "8x8p"])
+@pytest.mark.parametrize("shape", ["16x16", "16x16r", "32x8", "32x8p", "16x8r", "16x8p", "8x8", "16x16pr", "8x8p",
+                                   "16x16v", "8x8v", "32x8pv", "16x8v"])
 def test_every_launch_shape_is_exact(ctx, shape, monkeypatch):
     """Tile shape, evaluator and schedule change only speed, never values."""
     import oracle
@@ -282,8 +307,40 @@ def test_autotune_then_exact(ctx):
     jobs = [(r, srcs[r], 1.0, 10.0) for r in range(len(views))]
     best = ctx.autotune(jobs, opts, passes=1)
     _say("autotune picked %s" % (best,))
-    assert best[0] * best[1] in (64, 128, 256) and best[2] in (0, 1) and best[3] in (0, 1)
+    assert best[0] * best[1] in (64, 128, 256) and best[2] in (0, 1) and best[3] in (0, 1, 2)
     out = ctx.estimate_depth_batch(jobs, opts, mode=0)
     for r, s, zmin, zmax in jobs:
         d, c, n, _, _ = ov.estimate_depth(r, s, zmin, zmax, opts)
         assert np.array_equal(bits(out[r].depth), bits(d)), r
+
+
+def test_run_mvs_matches_oracle_pipeline(ctx, tmp_path):
+    """run_mvs end to end (plan from a sparse model, batched stereo, fusion,
+    dense.ply) equals the oracle run on the same plan."""
+    import oracle
+    from rustcolmap_amd.mvs import MvsOptions, plan, run_mvs
+    from rustcolmap_amd.scene import sparse_points
+    sc = scene("corner320")
+    views = view_data(sc)
+    xyz, tracks = sparse_points(sc, 400, seed=1)
+    opts = MvsOptions(num_sources=4, sweeps=2)
+    st = run_mvs(views, xyz, tracks, str(tmp_path), opts, ctx=ctx)
+    src, ranges, fus, _ = plan(views, xyz, tracks, opts)
+    ov = oracle.OracleViews(views)
+    dms = []
+    for i in range(len(views)):
+        if not src[i]:
+            dms.append(None)
+            continue
+        d, _, _, _, _ = ov.estimate_depth(i, src[i], ranges[i][0], ranges[i][1], opts)
+        dms.append(d)
+        assert st["per_view_valid"][i] == int((d > 0).sum()), i
+    cell = oracle.fusion_cell(xyz, opts.voxel_rel)
+    assert st["cell"] == cell
+    exp, ncand, _ = ov.fuse(dms, fus, opts.min_consistent, opts.depth_diff, cell)
+    assert st["dense_points"] == len(exp) and st["n_candidates"] == ncand
+    ply = (tmp_path / "dense" / "dense.ply").read_bytes()
+    ref = tmp_path / "ref.ply"
+    oracle.write_ply(str(ref), exp)
+    assert ply == ref.read_bytes()
+    assert (tmp_path / "dense" / "mvs_stats.json").exists()
