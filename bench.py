@@ -6,16 +6,18 @@ Workload (default): 20 views x 1920x1080, f = 1650, K = 8 support views,
 sigma = 2 DN noise (rustcolmap_amd.scene "room1080"), synthetic data.
 
 A step = the whole stereo stage of the batch (init sweep + 16 passes + gate
-for all 20 views), replayed from its captured HIP graph, with the views'
-census/rays already resident in HBM.  value = depth Mpix.iter/s =
-sum(W*H) * sweeps / t(step); init and gate time are inside t(step).
-
-After the timed region: one directly launched run of the same batch gives
-per-stage device times (HIP events on the library's stream) and the exact
-evaluation counts, from which the pass kernel's algorithmic bytes and
-roofline fraction follow (bytes per pixel and pass = 120 + n_eval * (400 +
-200 K)); fusion is timed for fused points/s and views/s.  The CPU baseline
-is the oracle on a bounded row band of one view, on the host's cores.
+for all 20 views) with the views' census/rays already resident in HBM.
+value = depth Mpix.iter/s = sum(W*H) * sweeps / t(step); init and gate time
+are inside t(step).  Timed steps launch directly on the library's stream
+with HIP events around each stage (mode | 4), so the pass kernel's average
+launch time and the exact evaluation counts come from the timed steps
+themselves; the pass kernel's algorithmic bytes and roofline fraction follow
+(bytes per pixel and pass = 120 + n_eval * (400 + 200 K)).  roofline.traffic
+is the copy-calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE of the same kernel
+on this workload, read from the committed summary under profiles/ (null when
+the tuned kernel is not in it).  `--graph` times graph replays instead.
+Fusion is timed for fused points/s and views/s.  The CPU baseline is the
+oracle on a bounded row band of one view, on the host's cores.
 """
 from __future__ import annotations
 
@@ -35,6 +37,30 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "pass_kernel_room1080_traffic.txt")
+
+
+def profiled_traffic(kernel, path=TRAFFIC_SUMMARY):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, each scaled
+    by the copy-kernel calibration) from a committed summary written by
+    --rocprof-summary; None when the file or the kernel is absent."""
+    if not os.path.exists(path):
+        return None
+    fetch = write = None
+    with open(path) as f:
+        for line in f:
+            if not line.startswith(kernel + " ") or "->" not in line:
+                continue
+            val = float(line.split("->")[1].split()[0])
+            if " FETCH_SIZE " in line:
+                fetch = val
+            elif " WRITE_SIZE " in line:
+                write = val
+    if fetch is None or write is None:
+        return None
+    return fetch + write, os.path.relpath(path, ROOT)
 
 
 def rocprof_summary(path, copy_bytes=1 << 30, only="mvs::"):
@@ -101,6 +127,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--graph", action="store_true", help="time HIP graph replays (stage times from an extra run)")
+    ap.add_argument("--learned", action="store_true",
+                    help="learned initialisation + 2 sweeps against the plain 8-sweep judge (own JSON line)")
+    ap.add_argument("--combined", action="store_true",
+                    help="100 views 1080p on a 360-degree orbit, K=10: learned init + 2 sweeps + fusion (own JSON line)")
+    ap.add_argument("--weights", default=None, help="proposal network weights (.npz)")
+    ap.add_argument("--conf", type=float, default=0.5, help="confidence threshold of the learned initialisation")
     ap.add_argument("--batched", action="store_true",
                     help="uneven batch: 64 views of mixed size, K uniform in [4, 12]; batched graph schedule vs "
                          "per-view launches (prints its own JSON line)")
@@ -120,6 +153,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.batched:
         return batched_bench(args)
+    if args.learned or args.combined:
+        return learned_bench(args)
     t0 = time.time()
     spec = config_spec(args.workload)
     if args.views:
@@ -154,29 +189,47 @@ def main():
         log("autotune candidates (est. ms per full stereo run): " +
             ", ".join("%s %.0f" % (n, t) for n, t in ctx.last_tune))
 
+    mode = args.mode if args.graph else (args.mode & 3) | 4
     for _ in range(args.warmup):
-        ctx.estimate_depth_batch(jobs, opts, mode=args.mode, download=False)
+        ctx.estimate_depth_batch(jobs, opts, mode=mode, download=False)
     torch.cuda.synchronize()
+    acc = {"ms_init": 0.0, "ms_pass": 0.0, "ms_gate": 0.0, "n_eval_init": 0, "n_eval_pass": 0}
     t1 = time.perf_counter()
     for _ in range(args.steps):
-        ctx.estimate_depth_batch(jobs, opts, mode=args.mode, download=False)
+        ctx.estimate_depth_batch(jobs, opts, mode=mode, download=False)
+        if not args.graph:
+            st = ctx.stats()
+            for k_ in acc:
+                acc[k_] += getattr(st, k_)
     torch.cuda.synchronize()
     t_steps = time.perf_counter() - t1
     ms_step = 1000.0 * t_steps / args.steps
     value = npix * args.sweeps / (ms_step / 1000.0) / 1e6
     log("stereo step %.1f ms -> %.1f Mpix.iter/s" % (ms_step, value))
-
-    # per-stage device times and exact evaluation counts (direct launches)
-    ctx.estimate_depth_batch(jobs, opts, mode=(args.mode & 3) | 4, download=False)
-    st = ctx.stats()
+    if args.graph:
+        # per-stage device times and exact evaluation counts (direct launches)
+        ctx.estimate_depth_batch(jobs, opts, mode=(args.mode & 3) | 4, download=False)
+        st = ctx.stats()
+        nrun = 1
+        for k_ in acc:
+            acc[k_] = getattr(st, k_)
+    else:
+        nrun = args.steps
+    ms_init, ms_pass, ms_gate = acc["ms_init"] / nrun, acc["ms_pass"] / nrun, acc["ms_gate"] / nrun
+    n_eval_init, n_eval_pass = acc["n_eval_init"] // nrun, acc["n_eval_pass"] // nrun
     n_passes = 2 * args.sweeps
     b_eval = 400 + 200 * K
-    pass_bytes = n_passes * npix * 120 + st.n_eval_pass * b_eval          # all passes
-    init_bytes = npix * 20 + st.n_eval_init * b_eval
-    avg_pass_ms = st.ms_pass / n_passes
+    pass_bytes = n_passes * npix * 120 + n_eval_pass * b_eval          # all passes of one step
+    init_bytes = npix * 20 + n_eval_init * b_eval
+    avg_pass_ms = ms_pass / n_passes
     achieved = pass_bytes / n_passes / (avg_pass_ms / 1000.0) / 1e9
     log("init %.1f ms, passes %.1f ms (%.2f ms/launch), gate %.2f ms; evals init %d pass %d"
-        % (st.ms_init, st.ms_pass, avg_pass_ms, st.ms_gate, st.n_eval_init, st.n_eval_pass))
+        % (ms_init, ms_pass, avg_pass_ms, ms_gate, n_eval_init, n_eval_pass))
+    st = ctx.stats()
+    launch = (st.tile_w, st.tile_h, st.persistent, st.evaluator)
+    pass_kernel = "mvs::%s<%d, %d, %d>" % ("pass_persistent_kernel" if launch[2] else "pass_kernel", launch[0],
+                                           launch[1], launch[3])
+    traffic = profiled_traffic(pass_kernel)
 
     # fusion of the resident depth maps
     order = select_sources(sc.views, len(views) - 1)
@@ -211,21 +264,120 @@ def main():
                    "launch": None if tuned is None else
                    {"tile": "%dx%d" % tuned[:2], "persistent": tuned[2], "evaluator": tuned[3],
                     "candidates_ms": {n: round(t, 1) for n, t in ctx.last_tune}}},
-        "roofline": {"bound": "hbm", "kernel": "pass_kernel", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": pass_kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_launch": int(pass_bytes / n_passes),
-                     "avg_launch_ms": round(avg_pass_ms, 4)},
-        "stages_ms": {"census_rays": round(ms_prepare, 3), "init": round(st.ms_init, 3),
-                      "passes": round(st.ms_pass, 3), "gate": round(st.ms_gate, 3),
+                     "traffic": None if traffic is None else int(traffic[0]),
+                     "traffic_source": None if traffic is None else traffic[1],
+                     "bytes_per_launch": int(pass_bytes / n_passes), "avg_launch_ms": round(avg_pass_ms, 4),
+                     "launch_time_source": "graph-free run after the timed steps" if args.graph
+                     else "HIP events around the pass launches of the timed steps"},
+        "stages_ms": {"census_rays": round(ms_prepare, 3), "init": round(ms_init, 3),
+                      "passes": round(ms_pass, 3), "gate": round(ms_gate, 3),
                       "fusion": round(sf.ms_fuse, 3)},
-        "mpix_iter_per_s_passes_only": round(npix * args.sweeps / (st.ms_pass / 1000.0) / 1e6, 2),
+        "mpix_iter_per_s_passes_only": round(npix * args.sweeps / (ms_pass / 1000.0) / 1e6, 2),
         "views_per_s": round(views_per_s, 3),
         "fused_points_per_s": round(fused_pps, 1),
         "fused_points": int(len(pts)),
-        "init_roofline_frac": round(init_bytes / (st.ms_init / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
-        if st.ms_init > 0 else None,
+        "init_roofline_frac": round(init_bytes / (ms_init / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
+        if ms_init > 0 else None,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def _agree(ref, other, tol=0.01):
+    """Fraction of pixels valid in `ref` that are valid in `other` with
+    inverse depth within `tol` (relative)."""
+    m = ref > 0
+    if not m.any():
+        return float("nan")
+    o = other[m]
+    ok = (o > 0) & (np.abs(1.0 / np.where(o > 0, o, 1.0) - 1.0 / ref[m]) <= tol / ref[m])
+    return float(ok.mean())
+
+
+def learned_bench(args):
+    """Plain 8-sweep stereo (the judge) vs the learned initialisation
+    followed by 2 sweeps, on the same views; fusion after each."""
+    import torch
+    from rustcolmap_amd.learned import DEFAULT_WEIGHTS, load_weights
+    from rustcolmap_amd.mvs import MvsContext, MvsOptions, ProposalNet, ViewData, estimate_depth_learned
+    from rustcolmap_amd.scene import config_spec, render_scene, select_sources
+    name = "orbit100" if args.combined else args.workload
+    spec = config_spec(name)
+    if args.views:
+        spec.n_views = args.views
+    K = 10 if args.combined else args.k
+    zmax = 20.0 if args.combined else 10.0
+    t0 = time.time()
+    sc = render_scene(spec, device="cuda", with_gt=False)
+    log("rendered %d views of %s in %.1fs" % (len(sc.views), name, time.time() - t0))
+    views = [ViewData.from_scene_view(v) for v in sc.views]
+    K = min(K, len(views) - 1)
+    srcs = select_sources(sc.views, K)
+    jobs = [(r, srcs[r], 1.0, zmax) for r in range(len(views))]
+    npix = sum(v.w * v.h for v in views)
+    ctx = MvsContext(0)
+    ctx.upload_views(views)
+    ms_prepare = ctx.stats().ms_prepare
+    order = select_sources(sc.views, min(len(views) - 1, 2 * K))
+    plain = MvsOptions(num_sources=K, sweeps=8)
+    short = MvsOptions(num_sources=K, sweeps=2)
+    if not args.no_tune:
+        ctx.autotune(jobs, plain, passes=1)
+
+    def timed(fn, steps):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            r = fn()
+        torch.cuda.synchronize()
+        return r, 1000.0 * (time.perf_counter() - t) / steps
+
+    def fuse():
+        return ctx.fuse_resident([True] * len(views), order, args.cell, plain)
+
+    steps = max(1, args.steps)
+    # plain judge
+    _, ms_plain = timed(lambda: ctx.estimate_depth_batch(jobs, plain, mode=0, download=False), steps)
+    judge = {r: ctx.get_depth(r).depth for r in range(len(views))}
+    pts_plain, ms_fuse_plain = timed(fuse, 1)
+    # plain method with 2 sweeps, no network
+    _, ms_short = timed(lambda: ctx.estimate_depth_batch(jobs, short, mode=0, download=False), steps)
+    short_d = {r: ctx.get_depth(r).depth for r in range(len(views))}
+    # learned initialisation + 2 sweeps
+    net = ProposalNet(ctx, seed=0)
+    meta = load_weights(net, args.weights or DEFAULT_WEIGHTS)
+    (_, acc), ms_learned = timed(lambda: estimate_depth_learned(ctx, net, jobs, short, vol_k=4, conf_thr=args.conf,
+                                                                download=False), steps)
+    st = ctx.stats()
+    learned_d = {r: ctx.get_depth(r).depth for r in range(len(views))}
+    pts_learned, ms_fuse_learned = timed(fuse, 1)
+    agree_l = float(np.mean([_agree(judge[r], learned_d[r]) for r in judge]))
+    agree_s = float(np.mean([_agree(judge[r], short_d[r]) for r in judge]))
+    valid = lambda d: float(np.mean([(d[r] > 0).mean() for r in d]))
+    vps_plain = len(views) / ((ms_prepare + ms_plain + ms_fuse_plain) / 1000.0)
+    vps_learned = len(views) / ((ms_prepare + ms_learned + ms_fuse_learned) / 1000.0)
+    out = {"metric": "views/s, learned init + 2 sweeps vs plain 8 sweeps (%s)" % name,
+           "value": round(vps_learned, 3), "unit": "views/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+           "higher_is_better": True, "data": "synthetic",
+           "config": {"workload": "%s: %d views %dx%d K=%d vol_k=4 conf>=%.2f zrange [1, %g]"
+                      % (name, len(views), views[0].w, views[0].h, K, args.conf, zmax)},
+           "plain_views_per_s": round(vps_plain, 3), "speedup_views_per_s": round(vps_learned / vps_plain, 3),
+           "ms": {"census": round(ms_prepare, 2), "plain_8_sweeps": round(ms_plain, 2),
+                  "plain_2_sweeps": round(ms_short, 2), "learned_2_sweeps": round(ms_learned, 2),
+                  "learned_volume": round(st.ms_learned_volume, 2), "learned_net": round(st.ms_learned_net, 2),
+                  "fusion_plain": round(ms_fuse_plain, 2), "fusion_learned": round(ms_fuse_learned, 2)},
+           "agreement_with_plain_8": {"learned_2_sweeps": round(agree_l, 4), "plain_2_sweeps": round(agree_s, 4),
+                                      "definition": "pixels valid in the 8-sweep plain map that are valid with "
+                                                    "inverse depth within 1 %"},
+           "valid_fraction": {"plain_8": round(valid(judge), 4), "plain_2": round(valid(short_d), 4),
+                              "learned_2": round(valid(learned_d), 4)},
+           "proposals_accepted_fraction": round(acc / npix, 4),
+           "fused_points": {"plain_8": int(len(pts_plain)), "learned_2": int(len(pts_learned))},
+           "fused_points_per_s_learned": round(len(pts_learned) / (ms_fuse_learned / 1000.0), 1),
+           "weights": {k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in meta.items()}}
     print(json.dumps(out), flush=True)
 
 

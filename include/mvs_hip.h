@@ -95,6 +95,9 @@ typedef struct mvs_stats {
     int32_t graph_used;
     double ms_learned_volume; /* learned init: sweep volume (part of ms_init) */
     double ms_learned_net;    /* learned init: network forward (part of ms_init) */
+    int32_t tile_w, tile_h;   /* launch shape of the last stereo call */
+    int32_t persistent;       /* 1 = persistent tile queue for the passes */
+    int32_t evaluator;        /* plane_cost evaluator (see mvs_autotune) */
 } mvs_stats_t;
 
 /* MvsOptions::default (mvs/src/lib.rs:56-70). */
@@ -146,9 +149,10 @@ int mvs_estimate_depth_batch(mvs_ctx *ctx, const mvs_job_t *jobs, int32_t njobs,
                              int32_t mode);
 
 /* Launch-shape autotuner (no counterpart in the reference): runs the batch
- * with every candidate (tile 16x16 / 32x8 / 16x8 / 8x8, reference-order or
- * hoisted-geometry evaluator, tile grid or persistent queue) for `passes`
- * passes (plus the divide-free-abort evaluator), and records the fastest
+ * with every candidate (tile 16x16 / 32x8 / 16x8 / 8x8; evaluator 0
+ * reference order, 1 hoisted geometry, 2 divide-free abort, 3 / 4 per-tap
+ * geometry with the scaling-free exact divide, source-0 rows batched or
+ * not; tile grid or persistent queue) for `passes` passes, and records the fastest
  * for the batch's shape class (longest
  * side bucket x K bucket); later batches of that class use it.  Results
  * are identical for every candidate.  best (optional, 4 ints) receives
@@ -206,6 +210,11 @@ int mvs_debug_plane_cost(mvs_ctx *ctx, int32_t ref, const int32_t *srcs, int32_t
  * device (truncate toward zero, saturate, NaN -> 0), for n floats. */
 int mvs_debug_cvt(mvs_ctx *ctx, const float *x, int32_t n, int32_t *out);
 
+/* Test probe: out[i] = a[i] / b[i] through the stereo kernels' scaling-free
+ * division sequence, which is IEEE division when b and 1/b are normal, a / b
+ * is normal, |a| >= 2^-104 and |a / b| < 2^96. */
+int mvs_debug_div(mvs_ctx *ctx, const float *a, const float *b, int32_t n, float *out);
+
 /* Instrumentation: one plain device-to-device copy of `bytes` (a multiple
  * of 16) by the library's copy kernel; a reference for traffic counters. */
 int mvs_debug_copy(mvs_ctx *ctx, int64_t bytes);
@@ -256,7 +265,12 @@ int mvs_run_mvs(mvs_ctx *ctx, const mvs_view_t *views, int32_t nviews, const dou
  * A 5-level U-Net (3x3 convs, channels 32-64-96-128-160, bf16 on MFMA with
  * fp32 accumulation) maps the gray level plus a 32-plane census sweep volume
  * over 4 sources to normalised inverse depth, unit normal and confidence.
+ * The volume and the network run on the coarse grid of every
+ * MVS_NET_STRIDE-th pixel: coarse pixel (cx, cy) is pixel (S*cx, S*cy) with
+ * its full-resolution census window; a w x h view has a
+ * ceil(w/S) x ceil(h/S) coarse grid.
  * ------------------------------------------------------------------------ */
+#define MVS_NET_STRIDE 2
 typedef struct mvs_net mvs_net;
 
 /* Network with seeded He-normal weights, bound to the context's device. */
@@ -267,16 +281,18 @@ int64_t mvs_net_num_params(mvs_net *net);
 int mvs_net_get_params(mvs_net *net, float *out);
 int mvs_net_set_params(mvs_net *net, const float *in);
 
-/* 32-plane fronto-parallel sweep volume of one job (its k sources): for
- * plane h = 0..31, inv = 1/zmax + (1/zmin - 1/zmax) * h / 31 and the
- * plane_cost of (0, 0, 1, 1/inv) with no abort; out = w*h*32 floats. */
-int mvs_sweep_volume(mvs_ctx *ctx, const mvs_job_t *job, float *out);
+/* 32-plane fronto-parallel sweep volume of one job (its k sources) on the
+ * grid of every stride-th pixel (stride 1 or 2): for plane h = 0..31,
+ * inv = 1/zmax + (1/zmin - 1/zmax) * h / 31 and the plane_cost at pixel
+ * (stride*cx, stride*cy) of (0, 0, 1, 1/inv) with no abort;
+ * out = ceil(w/stride) * ceil(h/stride) * 32 floats. */
+int mvs_sweep_volume(mvs_ctx *ctx, const mvs_job_t *job, int32_t stride, float *out);
 
 /* Adds training samples: for each job (a view of the context, usually 4
- * sources), the sweep volume is computed and kept on the device with the
- * host target (w*h*5 floats: normalised inverse depth (1/z - 1/zmax) /
- * (1/zmin - 1/zmax), unit normal, validity 0/1).  Returns the sample count
- * (>= 0) or an error. */
+ * sources), the coarse sweep volume is computed and kept on the device with
+ * the host target on the coarse grid (cw*ch*5 floats: normalised inverse
+ * depth (1/z - 1/zmax) / (1/zmin - 1/zmax), unit normal, validity 0/1).
+ * Returns the sample count (>= 0) or an error. */
 int mvs_net_add_samples(mvs_net *net, const mvs_job_t *jobs, int32_t njobs, const float *const *targets);
 int mvs_net_clear_samples(mvs_net *net);
 
@@ -286,8 +302,8 @@ int mvs_net_clear_samples(mvs_net *net);
  * (optional): mean depth L1, mean normal term, mean BCE, valid fraction. */
 int mvs_net_train_step(mvs_net *net, const int32_t *idx, int32_t n, float lr, float *loss4);
 
-/* Prediction for one job: w*h*5 floats (inverse depth normalised to the
- * job's range, unit normal, confidence). */
+/* Prediction for one job on the coarse grid: cw*ch*5 floats (inverse depth
+ * normalised to the job's range, unit normal, confidence). */
 int mvs_net_predict(mvs_net *net, const mvs_job_t *job, float *pred);
 
 /* Gradient probe for tests: forward + loss + backward on a host batch
@@ -296,9 +312,11 @@ int mvs_net_predict(mvs_net *net, const mvs_job_t *job, float *pred);
 int mvs_net_debug_grads(mvs_net *net, const float *input, const float *target, int32_t n, int32_t Hp, int32_t Wp,
                         int32_t h, int32_t w, float *grads, float *pred, float *loss4);
 
-/* estimate_depth with the learned initialisation: sweep volume over the
- * first vol_k sources, network, then per pixel the proposed plane where
- * confidence >= conf_thr (else the plain plane sweep), then 2 * opts->sweeps
+/* estimate_depth with the learned initialisation: coarse sweep volume over
+ * the first vol_k sources, network, then per pixel the plane proposed at its
+ * coarse pixel (passing through the predicted depth on that pixel's ray)
+ * where confidence >= conf_thr and both rays face it (n . r >= 0.1), else the
+ * plain plane sweep; then 2 * opts->sweeps
  * passes and the gate as in mvs_estimate_depth_batch.  Results stay
  * resident and are copied to the jobs' host pointers where non-NULL.
  * n_accepted (optional): pixels that started from a proposed plane. */

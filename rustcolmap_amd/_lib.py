@@ -51,6 +51,7 @@ class MvsStatsC(C.Structure):
         ("ms_pass", C.c_double), ("ms_gate", C.c_double), ("ms_fuse", C.c_double),
         ("n_candidates", C.c_int64), ("n_points", C.c_int64), ("n_pass_launches", C.c_int32),
         ("graph_used", C.c_int32), ("ms_learned_volume", C.c_double), ("ms_learned_net", C.c_double),
+        ("tile_w", C.c_int32), ("tile_h", C.c_int32), ("persistent", C.c_int32), ("evaluator", C.c_int32),
     ]
 
 
@@ -90,6 +91,7 @@ _SIGS = {
     "mvs_debug_rng": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, _P, _P]),
     "mvs_debug_copy": (C.c_int, [_P, C.c_int64]),
     "mvs_debug_cvt": (C.c_int, [_P, _P, C.c_int32, _P]),
+    "mvs_debug_div": (C.c_int, [_P, _P, _P, C.c_int32, _P]),
     "mvs_plan": (C.c_int, [C.POINTER(MvsViewC), C.c_int32, _P, _P, _P, C.c_int64, C.POINTER(MvsOptionsC), _P, _P,
                           _P, _P, _P, _P]),
     "mvs_run_mvs": (C.c_int, [_P, C.POINTER(MvsViewC), C.c_int32, _P, _P, _P, C.c_int64, C.POINTER(MvsOptionsC),
@@ -99,7 +101,7 @@ _SIGS = {
     "mvs_net_num_params": (C.c_int64, [_P]),
     "mvs_net_get_params": (C.c_int, [_P, _P]),
     "mvs_net_set_params": (C.c_int, [_P, _P]),
-    "mvs_sweep_volume": (C.c_int, [_P, C.POINTER(MvsJobC), _P]),
+    "mvs_sweep_volume": (C.c_int, [_P, C.POINTER(MvsJobC), C.c_int32, _P]),
     "mvs_net_add_samples": (C.c_int, [_P, C.POINTER(MvsJobC), C.c_int32, _P]),
     "mvs_net_clear_samples": (C.c_int, [_P]),
     "mvs_net_train_step": (C.c_int, [_P, _P, C.c_int32, C.c_float, _P]),

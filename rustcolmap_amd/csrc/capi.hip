@@ -416,7 +416,11 @@ static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_op
                 const SrcDesc sd = make_src(rv, c->views[jb.srcs[s2]]);
                 for (int i = 0; i < 3; i++) ok = ok && fabsf(sd.ts[i]) < 1e12f;
             }
-            J.pinhole_ok = ok ? 1 : 0;
+            // bit 1: every valid tap's divides stay inside the domain where the
+            // scaling-free division sequence is IEEE division (zmin >= 1e-6
+            // bounds the plane distance below; the bounds above bound it and
+            // the projected coordinates above)
+            J.pinhole_ok = ok ? (J.zmin >= 1e-6f ? 3 : 1) : 0;
         }
         J.tiles_x = (rv.w + TW - 1) / TW;
         J.tiles_y = (rv.h + TH - 1) / TH;
@@ -589,6 +593,10 @@ static int run_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_opt
     c->stats.n_pixels = P.n_pixels;
     c->stats.taps_per_eval_sum = P.taps;
     c->stats.n_pass_launches = P.run_passes;
+    c->stats.tile_w = P.cfg.tw;
+    c->stats.tile_h = P.cfg.th;
+    c->stats.persistent = P.cfg.persistent;
+    c->stats.evaluator = P.cfg.eval;
     if (n_eval_out) {
         for (int p = 0; p < 1 + P.run_passes; p++) n_eval_out[p] = (int64_t)cnt[p];
     }
@@ -671,9 +679,10 @@ int mvs_autotune(mvs_ctx* c, const mvs_job_t* jobs, int32_t njobs, const mvs_opt
     try {
         std::lock_guard<std::mutex> lk(c->mu);
         if (int r = set_device(c)) return r;
-        static const int cand[][4] = {{16, 16, 0, 1}, {16, 16, 1, 1}, {32, 8, 0, 1}, {16, 8, 0, 1}, {8, 8, 0, 1},
-                                      {8, 8, 1, 1},   {16, 16, 0, 2}, {16, 16, 1, 2}, {32, 8, 0, 2}, {16, 8, 0, 2},
-                                      {8, 8, 0, 2},   {8, 8, 1, 2},   {16, 16, 0, 0}, {16, 8, 0, 0}};
+        static const int cand[][4] = {{16, 16, 0, 1}, {16, 8, 0, 1}, {8, 8, 0, 1},  {8, 8, 1, 1},  {16, 16, 0, 2},
+                                      {16, 8, 0, 2},  {8, 8, 0, 2},  {8, 8, 1, 2},  {16, 16, 0, 0}, {16, 8, 0, 0},
+                                      {16, 16, 0, 3}, {16, 16, 0, 4}, {16, 8, 0, 3}, {16, 8, 0, 4}, {8, 8, 0, 3},
+                                      {8, 8, 0, 4},   {8, 8, 1, 3},  {8, 8, 1, 4},  {32, 8, 0, 3}, {16, 16, 1, 4}};
         const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
         const int n_passes = (int)std::max<uint32_t>(opts->sweeps, 1) * 2;
         double best_t = 1e300;
@@ -1010,6 +1019,23 @@ int mvs_debug_cvt(mvs_ctx* c, const float* x, int32_t n, int32_t* out) {
     return MVS_OK;
 }
 
+int mvs_debug_div(mvs_ctx* c, const float* a, const float* b, int32_t n, float* out) {
+    if (!c || !a || !b || !out || n < 0) return fail(MVS_ERR_ARG, "mvs_debug_div: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int r = set_device(c)) return r;
+    const size_t nb = (((size_t)n * 4 + 255) & ~(size_t)255);
+    HCHK(c->dbg.need(3 * nb + 64));
+    float* da = (float*)c->dbg.p;
+    float* db = (float*)(c->dbg.p + nb);
+    float* dout = (float*)(c->dbg.p + 2 * nb);
+    HCHK(hipMemcpyAsync(da, a, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    HCHK(hipMemcpyAsync(db, b, (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    HCHK(launch_debug_div(da, db, n, dout, c->stream));
+    HCHK(hipMemcpyAsync(out, dout, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+    return MVS_OK;
+}
+
 int mvs_debug_rng(mvs_ctx* c, const int32_t* xy, const uint64_t* salt, int32_t n, int32_t ndraw, uint64_t* seed_out,
                   float* uniform_out) {
     if (!c || !xy || !salt || n < 0 || ndraw < 0 || !seed_out || (ndraw > 0 && !uniform_out))
@@ -1038,12 +1064,17 @@ int mvs_debug_rng(mvs_ctx* c, const int32_t* xy, const uint64_t* salt, int32_t n
 // learned plane proposals
 // --------------------------------------------------------------------------
 
+// One training sample on the network's coarse grid (h x w = the view's
+// size divided by MVS_NET_STRIDE, rounded up); the gray image stays at the
+// view's full width wf.
 struct NetSample {
-    int view = 0, h = 0, w = 0;
+    int view = 0, h = 0, w = 0, wf = 0;
     DBuf<uint16_t> vol;     // [h*w][32] bf16 sweep features
-    DBuf<uint8_t> gray;     // [h*w]
+    DBuf<uint8_t> gray;     // full-resolution gray
     DBuf<float> target;     // [h*w][5]
 };
+
+static inline int coarse(int v) { return (v + MVS_NET_STRIDE - 1) / MVS_NET_STRIDE; }
 
 struct mvs_net {
     mvs_ctx* ctx = nullptr;
@@ -1084,23 +1115,32 @@ static int upload_plan(mvs_ctx* c, BatchPlan& P) {
     return MVS_OK;
 }
 
-// Sweep volume of each job into per-job outputs (device pointers).
-static int run_volume(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, int njobs, const std::vector<VolOut>& outs) {
+// Sweep volume of each job, on the grid of every stride-th pixel, into
+// per-job outputs (device pointers).  Tiles are 16 x 16 coarse pixels.
+static int run_volume(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, int njobs, const std::vector<VolOut>& outs,
+                      int stride) {
     mvs_options_t o;
     mvs_options_default(&o);
     BatchPlan P;
-    if (int r = plan_16(c, jobs, njobs, &o, P, 1)) return r;
+    if (int r = plan_16(c, jobs, njobs, &o, P, 4)) return r;
+    P.tiles.clear();
+    for (int j = 0; j < njobs; j++) {
+        const int cw = (P.jobs[j].w + stride - 1) / stride, ch = (P.jobs[j].h + stride - 1) / stride;
+        for (int ty = 0; ty < (ch + 15) / 16; ty++)
+            for (int tx = 0; tx < (cw + 15) / 16; tx++) P.tiles.push_back(make_int4(j, tx * 16, ty * 16, 0));
+    }
     if (int r = upload_plan(c, P)) return r;
     HCHK(nt->vol_outs.need(outs.size()));
     HCHK(hipMemcpyAsync(nt->vol_outs.p, outs.data(), outs.size() * sizeof(VolOut), hipMemcpyHostToDevice, c->stream));
-    HCHK(launch_volume(c->jobs.p, c->srcs.p, c->tiles.p, (int)P.tiles.size(), 32, nt->vol_outs.p, c->stream));
+    HCHK(launch_volume(c->jobs.p, c->srcs.p, c->tiles.p, (int)P.tiles.size(), 32, stride, nt->vol_outs.p, c->stream));
     return MVS_OK;
 }
 
 // Forward of the network over n same-size samples given device vol/gray
-// pointers; predictions written to dpred ([n][h][w][5]).
+// pointers (coarse h x w grid, gray rows of full width wf); predictions
+// written to dpred ([n][h][w][5]).
 static int net_predict_batch(mvs_net* nt, const std::vector<const uint16_t*>& vols,
-                             const std::vector<const uint8_t*>& grays, int h, int w, float* dpred, bool train,
+                             const std::vector<const uint8_t*>& grays, int h, int w, int wf, float* dpred, bool train,
                              const float* dtarget, float* loss4) {
     mvs_ctx* c = nt->ctx;
     const int n = (int)vols.size();
@@ -1114,7 +1154,7 @@ static int net_predict_batch(mvs_net* nt, const std::vector<const uint16_t*>& vo
     }
     HCHK(hipMemcpyAsync(nt->ptrs.p, tab.data(), tab.size() * sizeof(uintptr_t), hipMemcpyHostToDevice, c->stream));
     HCHK(launch_assemble((const uint16_t* const*)nt->ptrs.p, (const uint8_t* const*)(nt->ptrs.p + n), n, h, w, Hp, Wp,
-                         nt->net.input(), c->stream));
+                         MVS_NET_STRIDE, wf, nt->net.input(), c->stream));
     HCHK(nt->net.forward());
     HCHK(nt->net.head(train, h, w, dpred, dtarget, 0.02f));
     if (train) {
@@ -1192,14 +1232,15 @@ int mvs_net_set_params(mvs_net* nt, const float* in) {
     return MVS_OK;
 }
 
-int mvs_sweep_volume(mvs_ctx* c, const mvs_job_t* job, float* out) {
+int mvs_sweep_volume(mvs_ctx* c, const mvs_job_t* job, int32_t stride, float* out) {
     if (!c || !job || !out) return fail(MVS_ERR_ARG, "mvs_sweep_volume: NULL argument");
+    if (stride != 1 && stride != 2) return fail(MVS_ERR_ARG, "mvs_sweep_volume: stride must be 1 or 2");
     try {
         std::lock_guard<std::mutex> lk(c->mu);
         if (int r = set_device(c)) return r;
         if (job->ref < 0 || job->ref >= (int)c->views.size()) return fail(MVS_ERR_ARG, "mvs_sweep_volume: bad ref");
         const ViewState& v = c->views[job->ref];
-        const size_t np = (size_t)v.w * v.h;
+        const size_t np = (size_t)((v.w + stride - 1) / stride) * ((v.h + stride - 1) / stride);
         mvs_net tmp;
         tmp.ctx = c;
         DBuf<float> vf;
@@ -1207,7 +1248,7 @@ int mvs_sweep_volume(mvs_ctx* c, const mvs_job_t* job, float* out) {
         std::vector<VolOut> outs(1);
         outs[0].vol = nullptr;
         outs[0].volf = vf.p;
-        int r = run_volume(c, &tmp, job, 1, outs);
+        int r = run_volume(c, &tmp, job, 1, outs, stride);
         if (r == MVS_OK) {
             hipError_t e = hipMemcpyAsync(out, vf.p, np * 32 * 4, hipMemcpyDeviceToHost, c->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1235,19 +1276,20 @@ int mvs_net_add_samples(mvs_net* nt, const mvs_job_t* jobs, int32_t njobs, const
             const ViewState& v = c->views[jobs[j].ref];
             NetSample* s = new NetSample();
             s->view = jobs[j].ref;
-            s->h = v.h;
-            s->w = v.w;
-            const size_t np = (size_t)v.w * v.h;
+            s->h = coarse(v.h);
+            s->w = coarse(v.w);
+            s->wf = v.w;
+            const size_t np = (size_t)s->w * s->h;
             HCHK(s->vol.need(np * 32));
-            HCHK(s->gray.need(np));
+            HCHK(s->gray.need((size_t)v.w * v.h));
             HCHK(s->target.need(np * 5));
-            HCHK(hipMemcpyAsync(s->gray.p, v.gray.p, np, hipMemcpyDeviceToDevice, c->stream));
+            HCHK(hipMemcpyAsync(s->gray.p, v.gray.p, (size_t)v.w * v.h, hipMemcpyDeviceToDevice, c->stream));
             HCHK(hipMemcpyAsync(s->target.p, targets[j], np * 5 * 4, hipMemcpyHostToDevice, c->stream));
             outs[j].vol = s->vol.p;
             outs[j].volf = nullptr;
             fresh.push_back(s);
         }
-        if (int r = run_volume(c, nt, jobs, njobs, outs)) return r;
+        if (int r = run_volume(c, nt, jobs, njobs, outs, MVS_NET_STRIDE)) return r;
         HCHK(hipStreamSynchronize(c->stream));
         for (NetSample* s : fresh) nt->samples.push_back(s);
         return (int)nt->samples.size();
@@ -1279,14 +1321,15 @@ int mvs_net_train_step(mvs_net* nt, const int32_t* idx, int32_t n, float lr, flo
         if (int r = set_device(c)) return r;
         std::vector<const uint16_t*> vols;
         std::vector<const uint8_t*> grays;
-        int h = -1, w = -1;
+        int h = -1, w = -1, wf = -1;
         for (int i = 0; i < n; i++) {
             if (idx[i] < 0 || idx[i] >= (int)nt->samples.size()) return fail(MVS_ERR_ARG, "train_step: bad sample");
             NetSample* s = nt->samples[idx[i]];
             if (h < 0) {
                 h = s->h;
                 w = s->w;
-            } else if (s->h != h || s->w != w) {
+                wf = s->wf;
+            } else if (s->h != h || s->w != w || s->wf != wf) {
                 return fail(MVS_ERR_ARG, "train_step: samples of one batch must share their size");
             }
             vols.push_back(s->vol.p);
@@ -1298,7 +1341,7 @@ int mvs_net_train_step(mvs_net* nt, const int32_t* idx, int32_t n, float lr, flo
             HCHK(hipMemcpyAsync(nt->net.target + (size_t)i * np * 5, nt->samples[idx[i]]->target.p, np * 5 * 4,
                                 hipMemcpyDeviceToDevice, c->stream));
         float l4[4] = {0, 0, 0, 0};
-        if (int r = net_predict_batch(nt, vols, grays, h, w, nullptr, true, nt->net.target, l4)) return r;
+        if (int r = net_predict_batch(nt, vols, grays, h, w, wf, nullptr, true, nt->net.target, l4)) return r;
         HCHK(nt->net.adam(lr));
         HCHK(hipStreamSynchronize(c->stream));
         if (loss4) {
@@ -1322,16 +1365,18 @@ int mvs_net_predict(mvs_net* nt, const mvs_job_t* job, float* pred) {
         if (int r = set_device(c)) return r;
         if (job->ref < 0 || job->ref >= (int)c->views.size()) return fail(MVS_ERR_ARG, "mvs_net_predict: bad ref");
         const ViewState& v = c->views[job->ref];
-        const size_t np = (size_t)v.w * v.h;
+        const size_t np = (size_t)coarse(v.w) * coarse(v.h);
         HCHK(nt->vol_tmp.need(np * 32));
         HCHK(nt->preds.need(np * 5));
         std::vector<VolOut> outs(1);
         outs[0].vol = nt->vol_tmp.p;
         outs[0].volf = nullptr;
-        if (int r = run_volume(c, nt, job, 1, outs)) return r;
+        if (int r = run_volume(c, nt, job, 1, outs, MVS_NET_STRIDE)) return r;
         std::vector<const uint16_t*> vols(1, nt->vol_tmp.p);
         std::vector<const uint8_t*> grays(1, v.gray.p);
-        if (int r = net_predict_batch(nt, vols, grays, v.h, v.w, nt->preds.p, false, nullptr, nullptr)) return r;
+        if (int r = net_predict_batch(nt, vols, grays, coarse(v.h), coarse(v.w), v.w, nt->preds.p, false, nullptr,
+                                      nullptr))
+            return r;
         HCHK(hipMemcpyAsync(pred, nt->preds.p, np * 5 * 4, hipMemcpyDeviceToHost, c->stream));
         HCHK(hipStreamSynchronize(c->stream));
         return MVS_OK;
@@ -1390,7 +1435,7 @@ int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, i
                 return fail(MVS_ERR_ARG, "mvs_estimate_depth_learned: bad ref");
             vj[j].k = std::min(jobs[j].k, vol_k);
             off[j] = total;
-            total += (size_t)c->views[jobs[j].ref].w * c->views[jobs[j].ref].h;
+            total += (size_t)coarse(c->views[jobs[j].ref].w) * coarse(c->views[jobs[j].ref].h);
         }
         HCHK(nt->vol_tmp.need(total * 32));
         HCHK(nt->preds.need(total * 5));
@@ -1400,7 +1445,7 @@ int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, i
             outs[j].vol = nt->vol_tmp.p + off[j] * 32;
             outs[j].volf = nullptr;
         }
-        if (int r = run_volume(c, nt, vj.data(), njobs, outs)) return r;
+        if (int r = run_volume(c, nt, vj.data(), njobs, outs, MVS_NET_STRIDE)) return r;
         HCHK(hipEventRecord(c->ev[1], c->stream));
         // 2. network over same-size groups, in chunks that bound activation memory
         std::vector<int> order(njobs);
@@ -1411,7 +1456,8 @@ int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, i
         });
         for (size_t i = 0; i < order.size();) {
             const ViewState& v0 = c->views[jobs[order[i]].ref];
-            const int64_t px = (int64_t)round16(v0.w) * round16(v0.h);
+            const int cw0 = coarse(v0.w), ch0 = coarse(v0.h);
+            const int64_t px = (int64_t)round16(cw0) * round16(ch0);
             const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(16, ((int64_t)24 << 20) / px));
             std::vector<const uint16_t*> vols;
             std::vector<const uint8_t*> grays;
@@ -1427,17 +1473,18 @@ int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, i
             DBuf<float> chunk_pred;
             float* dst = nt->preds.p + off[order[i]] * 5;
             bool contiguous = true;
+            const size_t cnp = (size_t)cw0 * ch0;
             for (size_t q = i; q < k; q++)
-                if (off[order[q]] != off[order[i]] + (q - i) * (size_t)v0.w * v0.h) contiguous = false;
+                if (off[order[q]] != off[order[i]] + (q - i) * cnp) contiguous = false;
             if (!contiguous) {
-                HCHK(chunk_pred.need((k - i) * (size_t)v0.w * v0.h * 5));
+                HCHK(chunk_pred.need((k - i) * cnp * 5));
                 dst = chunk_pred.p;
             }
-            if (int r = net_predict_batch(nt, vols, grays, v0.h, v0.w, dst, false, nullptr, nullptr)) return r;
+            if (int r = net_predict_batch(nt, vols, grays, ch0, cw0, v0.w, dst, false, nullptr, nullptr)) return r;
             if (!contiguous) {
                 for (size_t q = i; q < k; q++)
-                    HCHK(hipMemcpyAsync(nt->preds.p + off[order[q]] * 5, dst + (q - i) * (size_t)v0.w * v0.h * 5,
-                                        (size_t)v0.w * v0.h * 5 * 4, hipMemcpyDeviceToDevice, c->stream));
+                    HCHK(hipMemcpyAsync(nt->preds.p + off[order[q]] * 5, dst + (q - i) * cnp * 5, cnp * 5 * 4,
+                                        hipMemcpyDeviceToDevice, c->stream));
                 HCHK(hipStreamSynchronize(c->stream));
                 chunk_pred.release();
             }
@@ -1446,7 +1493,7 @@ int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, i
         HCHK(hipEventRecord(c->ev[2], c->stream));
         // 3. proposal init with the full source sets, then the passes and the gate
         BatchPlan P;
-        if (int r = plan_16(c, jobs, njobs, opts, P, 1)) return r;
+        if (int r = plan_16(c, jobs, njobs, opts, P, 4)) return r;
         if (int r = upload_plan(c, P)) return r;
         const int ncnt = 1 + P.n_passes;
         HCHK(nt->ptrs.need(njobs));
@@ -1456,7 +1503,8 @@ int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, i
         HCHK(launch_fill_u64(c->counters.p, counter_words(P), 0ULL, c->stream));
         u64* acc_counter = c->counters.p + valid_base(P) - 4;
         HCHK(launch_proposal_init(P.cfg, c->jobs.p, c->srcs.p, c->tiles.p, (int)P.tiles.size(), P.nh,
-                                  (const float* const*)nt->ptrs.p, conf_thr, c->counters.p, acc_counter, c->stream));
+                                  (const float* const*)nt->ptrs.p, MVS_NET_STRIDE, conf_thr, c->counters.p,
+                                  acc_counter, c->stream));
         HCHK(hipEventRecord(c->ev[3], c->stream));
         for (int p = 0; p < P.run_passes; p++) {
             const float t = (float)p / (float)P.n_passes;
@@ -1492,6 +1540,10 @@ int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, i
         c->stats.ms_learned_net = t_net;
         c->stats.n_pass_launches = P.run_passes;
         c->stats.graph_used = 0;
+        c->stats.tile_w = P.cfg.tw;
+        c->stats.tile_h = P.cfg.th;
+        c->stats.persistent = P.cfg.persistent;
+        c->stats.evaluator = P.cfg.eval;
         if (n_accepted) *n_accepted = (int64_t)acc;
         for (int j = 0; j < njobs; j++) {
             ViewState& rv = c->views[jobs[j].ref];

@@ -390,6 +390,171 @@ __device__ __forceinline__ float eval_plane_v3(const u64* __restrict__ s_cen, co
     return sum / (25.0f * (float)nsrc);
 }
 
+// Correctly rounded a / b from the hardware reciprocal: one Newton step on
+// 1/b, then two FMA quotient corrections -- the compiler's IEEE division
+// sequence without its operand scaling (v_div_scale) and special-case fixup
+// (v_div_fixup).  Both are identities when b and 1/b are normal, a / b is
+// normal, |a| >= 2^-104 and exponent(a) - exponent(b) < 96, so the result
+// then equals a / b bit for bit.  For |a| < 2^-104 the quotient can be off
+// by a few units of a number below 2^-94, which cannot move
+// trunc(f * x/z + c) and so no census index.  The host enables this only for
+// jobs whose bounds keep every valid tap inside that domain (JobDesc.pinhole_ok
+// bit 1: zmin >= 1e-6, zmax < 1e6, |ray| < 1e3, |ts| < 1e12).
+__device__ __forceinline__ float div_rn_fast(float a, float b) {
+    float r = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, r, 1.0f);
+    r = __builtin_fmaf(e, r, r);
+    float q = a * r;
+    float t = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(t, r, q);
+    t = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(t, r, q);
+}
+
+// Fourth evaluator (same values): no hoisted window geometry, since an
+// evaluation almost never gets past the first row of its second source (the
+// prefix average there divides the 30 summed taps by 10) -- per-tap
+// geometry is recomputed for later sources instead of holding 75 registers,
+// which keeps occupancy high.  Divides use div_rn_fast; fronto-parallel
+// planes (the init sweep) take z = d, exact because (0 * rx + 0 * ry) + 1 is
+// 1 for finite rays.  With ROWS20, the source-0 rows 1-4 issue their 20
+// gathers together and apply the four prefix tests in row order afterwards
+// (the value is 1e9 iff one of them fires, as before).
+template <bool FRONTO, bool PIN>
+__device__ __forceinline__ bool tap_v4(float2 ray, float nx, float ny, float nz, float d, const SrcP& P, int& idx) {
+    float z, xr, yr;
+    bool vd;
+    if (FRONTO) {
+        z = d;
+        vd = true;
+    } else {
+        const float denom = (nx * ray.x + ny * ray.y) + nz;
+        vd = !(denom < 0.05f);
+        z = div_rn_fast(d, denom);
+    }
+    xr = z * ray.x;
+    yr = z * ray.y;
+    const float xs = ((P.r0 * xr + P.r1 * yr) + P.r2 * z) + P.t0;
+    const float ys = ((P.r3 * xr + P.r4 * yr) + P.r5 * z) + P.t1;
+    const float zs = ((P.r6 * xr + P.r7 * yr) + P.r8 * z) + P.t2;
+    const float xn = div_rn_fast(xs, zs), yn = div_rn_fast(ys, zs);
+    float fu, fv;
+    if (PIN) {
+        fu = P.f * xn + P.cx;
+        fv = P.f * yn + P.cy;
+    } else {
+        const float dist = 1.0f + P.k * (xn * xn + yn * yn);
+        fu = P.f * xn * dist + P.cx;
+        fv = P.f * yn * dist + P.cy;
+    }
+    const int u = cvt_i32_rust(fu), v = cvt_i32_rust(fv);
+    const bool ok = vd && !(zs < 1e-3f) && ((unsigned)u < (unsigned)P.sw) && ((unsigned)v < (unsigned)P.sh);
+    idx = ok ? v * P.sw + u : 0;
+    return ok;
+}
+
+template <int PW, bool FRONTO, bool PIN>
+__device__ __forceinline__ void row_v4(const u64* __restrict__ s_cen, const float2* __restrict__ s_ray, int c, int r,
+                                       float nx, float ny, float nz, float d, const SrcP& P,
+                                       const float* __restrict__ tab, float& sum) {
+    int idx[5];
+    bool ok[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+        ok[q] = tap_v4<FRONTO, PIN>(s_ray[c + (r - 2) * PW + (q - 2)], nx, ny, nz, d, P, idx[q]);
+    u64 cs[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) cs[q] = P.cen[idx[q]];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const u64 rc = s_cen[c + (r - 2) * PW + (q - 2)];
+        sum += ok[q] ? tab[__popcll(rc ^ cs[q])] : 1.0f;
+    }
+}
+
+// Source-0 rows 1-4 with all 20 gathers issued before the sums.
+template <int PW, bool FRONTO, bool PIN>
+__device__ __forceinline__ bool rows20_v4(const u64* __restrict__ s_cen, const float2* __restrict__ s_ray, int c,
+                                          float nx, float ny, float nz, float d, const SrcP& P,
+                                          const float* __restrict__ tab, const AbortThr& A, float& sum) {
+    int idx[20];
+    unsigned okm = 0;
+#pragma unroll
+    for (int t = 0; t < 20; ++t) {
+        const int r = 1 + t / 5, q = t % 5;
+        const bool ok = tap_v4<FRONTO, PIN>(s_ray[c + (r - 2) * PW + (q - 2)], nx, ny, nz, d, P, idx[t]);
+        okm |= (ok ? 1u : 0u) << t;
+    }
+    u64 cs[20];
+#pragma unroll
+    for (int t = 0; t < 20; ++t) cs[t] = P.cen[idx[t]];
+    bool ab = false;
+#pragma unroll
+    for (int t = 0; t < 20; ++t) {
+        const int r = 1 + t / 5, q = t % 5;
+        const u64 rc = s_cen[c + (r - 2) * PW + (q - 2)];
+        sum += ((okm >> t) & 1u) ? tab[__popcll(rc ^ cs[t])] : 1.0f;
+        if (q == 4) ab = ab || abort_test(sum, (double)(r + 1) * 5.0, A);
+    }
+    return ab;
+}
+
+template <int PW, bool FRONTO, bool PIN, bool ROWS20>
+__device__ __forceinline__ float eval_v4_body(const u64* __restrict__ s_cen, const float2* __restrict__ s_ray, int c,
+                                              const SrcDesc* __restrict__ S, int nsrc, float nx, float ny, float nz,
+                                              float d, const AbortThr& A, const float* __restrict__ tab) {
+    float sum = 0.0f;
+    {
+        const SrcP P = load_src(S[0]);
+        row_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, 0, nx, ny, nz, d, P, tab, sum);
+        if (abort_test(sum, 5.0, A)) return 1e9f;
+        if (ROWS20) {
+            if (rows20_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, nx, ny, nz, d, P, tab, A, sum)) return 1e9f;
+        } else {
+#pragma unroll
+            for (int r = 1; r < 5; ++r) {
+                row_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, r, nx, ny, nz, d, P, tab, sum);
+                if (abort_test(sum, (double)(r + 1) * 5.0, A)) return 1e9f;
+            }
+        }
+    }
+    for (int si = 1; si < nsrc; ++si) {
+        const SrcP P = load_src(S[si]);
+        const double fs = (double)(si + 1) * 5.0;
+        // an opaque copy of the window index keeps the compiler from hoisting
+        // the 25 ray and 25 census LDS reads out of this loop (100 registers)
+        int cc = c;
+        asm volatile("" : "+v"(cc));
+        if (P.k == 0.0f) {
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                row_v4<PW, FRONTO, true>(s_cen, s_ray, cc, r, nx, ny, nz, d, P, tab, sum);
+                if (abort_test(sum, (double)(r + 1) * fs, A)) return 1e9f;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                row_v4<PW, FRONTO, false>(s_cen, s_ray, cc, r, nx, ny, nz, d, P, tab, sum);
+                if (abort_test(sum, (double)(r + 1) * fs, A)) return 1e9f;
+            }
+        }
+    }
+    return sum / (25.0f * (float)nsrc);
+}
+
+// Entry of the fourth evaluator; jobs outside the fast-divide domain (bit 1
+// of pinhole_ok clear) take the reference-order evaluator, which holds no
+// hoisted state either.
+template <int PW, bool FRONTO, bool ROWS20>
+__device__ __forceinline__ float eval_plane_v4(const u64* __restrict__ s_cen, const float2* __restrict__ s_ray, int c,
+                                               const SrcDesc* __restrict__ S, int nsrc, float nx, float ny, float nz,
+                                               float d, float thr, const float* __restrict__ tab, int pin_flags) {
+    if (!(pin_flags & 2)) return eval_plane<false>(LdsRef{s_cen, s_ray, c, PW}, S, nsrc, nx, ny, nz, d, thr, nullptr);
+    const AbortThr A = make_abort_thr(thr);
+    if (S[0].k == 0.0f) return eval_v4_body<PW, FRONTO, true, ROWS20>(s_cen, s_ray, c, S, nsrc, nx, ny, nz, d, A, tab);
+    return eval_v4_body<PW, FRONTO, false, ROWS20>(s_cen, s_ray, c, S, nsrc, nx, ny, nz, d, A, tab);
+}
+
 __device__ __forceinline__ void fill_pc_table(float* tab) {
     if (threadIdx.x < 64) tab[threadIdx.x] = pc_over_48((int)threadIdx.x);
 }
@@ -441,8 +606,10 @@ __device__ __forceinline__ void init_tile(const JobDesc& J, const SrcDesc* __res
         const float inv = inv_min + (inv_max - inv_min) * (float)hi / (float)(nh - 1);
         const float d = 1.0f / inv;
         const float thr = (best_c + 0.02f) + 0.12f;
-        const float c = EV == 2 ? eval_plane_v3<PW>(s_cen, s_ray, R.c, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, thr, s_tab,
-                                                    J.pinhole_ok)
+        const float c = EV >= 3 ? eval_plane_v4<PW, true, EV == 3>(s_cen, s_ray, R.c, S, J.nsrc, 0.0f, 0.0f, 1.0f, d,
+                                                                  thr, s_tab, J.pinhole_ok)
+                        : EV == 2 ? eval_plane_v3<PW>(s_cen, s_ray, R.c, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, thr, s_tab,
+                                                      J.pinhole_ok != 0)
                         : EV == 1 ? eval_plane_fast<PW>(s_cen, s_ray, R.c, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, thr)
                                   : eval_plane<false>(R, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, thr, nullptr);
         ++ne;
@@ -532,8 +699,10 @@ __device__ __forceinline__ void pass_tile(const JobDesc& J, const SrcDesc* __res
             cnz = nnz;
         }
         const float thr = (best_c + 0.02f) + 0.12f;
-        const float cst = EV == 2 ? eval_plane_v3<PW>(s_cen, s_ray, c, S, nsrc, cnx, cny, cnz, d_plane, thr, s_tab,
-                                                      J.pinhole_ok)
+        const float cst = EV >= 3 ? eval_plane_v4<PW, false, EV == 3>(s_cen, s_ray, c, S, nsrc, cnx, cny, cnz, d_plane,
+                                                                      thr, s_tab, J.pinhole_ok)
+                          : EV == 2 ? eval_plane_v3<PW>(s_cen, s_ray, c, S, nsrc, cnx, cny, cnz, d_plane, thr, s_tab,
+                                                        J.pinhole_ok != 0)
                           : EV == 1 ? eval_plane_fast<PW>(s_cen, s_ray, c, S, nsrc, cnx, cny, cnz, d_plane, thr)
                                     : eval_plane<false>(R, S, nsrc, cnx, cny, cnz, d_plane, thr, nullptr);
         ++ne;
@@ -679,34 +848,49 @@ __device__ __forceinline__ uint16_t f2bf16(float f) {
     return (uint16_t)(u >> 16);
 }
 
-// Fronto-parallel census sweep volume: nplanes planes uniform in inverse
-// depth over [1/zmax, 1/zmin] (the init sweep's spacing with nplanes
-// hypotheses), each costed by plane_cost over the job's sources with no
-// abort (hint = +inf).  fp32 costs (optional) and bf16 network features
-// (cost - 0.5) * 2 are written per pixel.
-template <int TW, int TH>
+// Fronto-parallel census sweep volume on a coarse grid: coarse pixel
+// (cx, cy) is full-resolution pixel (S*cx, S*cy) with its full-resolution
+// 5x5 window; nplanes planes uniform in inverse depth over [1/zmax, 1/zmin]
+// (the init sweep's spacing with nplanes hypotheses), each costed by
+// plane_cost over the job's sources with no abort (hint = +inf).  fp32 costs
+// (optional) and bf16 network features (cost - 0.5) * 2 are written per
+// coarse pixel.  Tiles are (job, cx0, cy0) in coarse coordinates.
+template <int TW, int TH, int S>
 __global__ __launch_bounds__(TW* TH) void volume_kernel(const JobDesc* __restrict__ jobs,
                                                         const SrcDesc* __restrict__ srcs,
                                                         const int4* __restrict__ tiles, int ntiles, int nplanes,
                                                         const VolOut* __restrict__ outs) {
-    constexpr int PW = TW + 4;
-    __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
-    __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
+    constexpr int PW = S * TW + 4, PH = S * TH + 4;
+    __shared__ u64 s_cen[PW * PH];
+    __shared__ float2 s_ray[PW * PH];
+    __shared__ float s_tab[64];
     const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
     const JobDesc& J = jobs[tl.x];
-    stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
+    const int w = J.w, h = J.h;
+    fill_pc_table(s_tab);
+    const int fx0 = S * tl.y - 2, fy0 = S * tl.z - 2;
+    for (int i = threadIdx.x; i < PW * PH; i += TW * TH) {
+        const int ay = i / PW, ax = i % PW;
+        const int gx = min(max(fx0 + ax, 0), w - 1);
+        const int gy = min(max(fy0 + ay, 0), h - 1);
+        const size_t g = (size_t)gy * w + gx;
+        s_cen[i] = J.census[g];
+        s_ray[i] = J.rays[g];
+    }
     __syncthreads();
+    const int cw = (w + S - 1) / S, ch = (h + S - 1) / S;
     const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
     const int x = tl.y + lx, y = tl.z + ly;
-    if (x >= J.w || y >= J.h) return;
-    const int c = (ly + 2) * PW + (lx + 2);
-    const SrcDesc* S = srcs + J.src_off;
+    if (x >= cw || y >= ch) return;
+    const int c = (S * ly + 2) * PW + (S * lx + 2);
+    const SrcDesc* Sd = srcs + J.src_off;
     const VolOut& O = outs[tl.x];
-    const size_t i = (size_t)y * J.w + x;
+    const size_t i = (size_t)y * cw + x;
     for (int hi = 0; hi < nplanes; ++hi) {
         const float inv = J.inv_min + (J.inv_max - J.inv_min) * (float)hi / (float)(nplanes - 1);
         const float d = 1.0f / inv;
-        const float cst = eval_plane_fast<PW>(s_cen, s_ray, c, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, __builtin_inff());
+        const float cst = eval_plane_v4<PW, true, false>(s_cen, s_ray, c, Sd, J.nsrc, 0.0f, 0.0f, 1.0f, d,
+                                                         __builtin_inff(), s_tab, J.pinhole_ok);
         if (O.volf) O.volf[i * nplanes + hi] = cst;
         if (O.vol) O.vol[i * 32 + hi] = f2bf16((cst - 0.5f) * 2.0f);
     }
@@ -720,8 +904,9 @@ template <int TW, int TH, int EV>
 __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __restrict__ jobs,
                                                                const SrcDesc* __restrict__ srcs,
                                                                const int4* __restrict__ tiles, int ntiles, int nh,
-                                                               const float* const* __restrict__ preds, float conf_thr,
-                                                               u64* __restrict__ counter, u64* __restrict__ accepted) {
+                                                               const float* const* __restrict__ preds, int stride,
+                                                               float conf_thr, u64* __restrict__ counter,
+                                                               u64* __restrict__ accepted) {
     constexpr int PW = TW + 4;
     __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
@@ -742,15 +927,21 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
     if (x < J.w && y < J.h) {
         const int c = (ly + 2) * PW + (lx + 2);
         const size_t i = (size_t)y * J.w + x;
-        const float* pr = preds[tl.x] + 5 * i;
+        const int cw = (J.w + stride - 1) / stride;
+        const float* pr = preds[tl.x] + 5 * ((size_t)(y / stride) * cw + (x / stride));
         const float inv_n = pr[0], nx = pr[1], ny = pr[2], nz = pr[3], conf = pr[4];
         const float2 ri = s_ray[c];
         const float a = (nx * ri.x + ny * ri.y) + nz;
-        if (conf >= conf_thr && a >= 0.1f && inv_n >= 0.0f && inv_n <= 1.0f) {
+        // the proposed plane passes through depth 1/inv along the ray of the
+        // coarse pixel (stride*(x/stride), stride*(y/stride)), which lies in
+        // this tile because tiles start at multiples of 16
+        const float2 rc = s_ray[c - (x % stride) - (y % stride) * PW];
+        const float ac = (nx * rc.x + ny * rc.y) + nz;
+        if (conf >= conf_thr && a >= 0.1f && ac >= 0.1f && inv_n >= 0.0f && inv_n <= 1.0f) {
             const float inv = J.inv_min + (J.inv_max - J.inv_min) * inv_n;
-            const float z = 1.0f / inv;
-            const float cst = eval_plane_fast<PW>(s_cen, s_ray, c, srcs + J.src_off, J.nsrc, nx, ny, nz, z * a,
-                                                  __builtin_inff());
+            const float z = ((1.0f / inv) * ac) / a;
+            const float cst = eval_plane_v4<PW, false, false>(s_cen, s_ray, c, srcs + J.src_off, J.nsrc, nx, ny, nz,
+                                                              z * a, __builtin_inff(), s_tab, J.pinhole_ok);
             ++ne;
             ++na;
             J.plane[0][i] = make_float4(nx, ny, nz, z);
@@ -773,7 +964,8 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
 // Network input [n][Hp][Wp][64] bf16: channels 0-31 sweep features, 32 the
 // normalised gray level, 33-63 zero; zero outside the h x w image.
 __global__ void assemble_input_kernel(const uint16_t* const* __restrict__ vols, const uint8_t* const* __restrict__ grays,
-                                      int n, int h, int w, int Hp, int Wp, uint16_t* __restrict__ out) {
+                                      int n, int h, int w, int Hp, int Wp, int stride, int wfull,
+                                      uint16_t* __restrict__ out) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // one thread per (pixel, 8 channels)
     const int64_t total = (int64_t)n * Hp * Wp * 8;
     if (t >= total) return;
@@ -789,7 +981,8 @@ __global__ void assemble_input_kernel(const uint16_t* const* __restrict__ vols, 
         if (g < 4) {
             v = *(const us8*)(vols[b] + i * 32 + 8 * g);
         } else if (g == 4) {
-            v[0] = f2bf16((float)grays[b][i] * (1.0f / 255.0f) - 0.5f);
+            const size_t gi = (size_t)(stride * y) * wfull + (size_t)(stride * x);
+            v[0] = f2bf16((float)grays[b][gi] * (1.0f / 255.0f) - 0.5f);
         }
     }
     *(us8*)(out + (size_t)p * 64 + 8 * g) = v;
@@ -822,6 +1015,11 @@ __global__ void debug_cost_kernel(JobDesc J, const SrcDesc* __restrict__ S, cons
     ham[t] = info.ham;
     ninv[t] = info.ninv;
     peak[t] = info.peak;
+}
+
+__global__ void debug_div_kernel(const float* a, const float* b, int n, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = div_rn_fast(a[i], b[i]);
 }
 
 __global__ void debug_cvt_kernel(const float* x, int n, int* out) {
@@ -866,7 +1064,7 @@ hipError_t launch_fill_u64(u64* p, int n, u64 v, hipStream_t st) {
 
 #define TILE_CASES(X) \
     X(16, 16, 0) X(16, 16, 1) X(16, 16, 2) X(32, 8, 0) X(32, 8, 1) X(32, 8, 2) X(16, 8, 0) X(16, 8, 1) X(16, 8, 2) \
-    X(8, 8, 1) X(8, 8, 2)
+    X(8, 8, 1) X(8, 8, 2) X(16, 16, 3) X(16, 16, 4) X(16, 8, 3) X(16, 8, 4) X(8, 8, 3) X(8, 8, 4) X(32, 8, 3)
 
 hipError_t launch_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles,
                        int nh, u64* counter, hipStream_t st) {
@@ -924,20 +1122,33 @@ int persistent_blocks_per_cu(const TileCfg& cfg) {
 }
 
 hipError_t launch_volume(const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles, int nplanes,
-                         const VolOut* outs, hipStream_t st) {
-    hipLaunchKernelGGL((volume_kernel<16, 16>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles, nplanes, outs);
+                         int stride, const VolOut* outs, hipStream_t st) {
+    if (stride == 1)
+        hipLaunchKernelGGL((volume_kernel<16, 16, 1>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles, nplanes,
+                           outs);
+    else if (stride == 2)
+        hipLaunchKernelGGL((volume_kernel<16, 16, 2>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles, nplanes,
+                           outs);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 hipError_t launch_proposal_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles,
-                                int ntiles, int nh, const float* const* preds, float conf_thr, u64* counter,
-                                u64* accepted, hipStream_t st) {
+                                int ntiles, int nh, const float* const* preds, int stride, float conf_thr,
+                                u64* counter, u64* accepted, hipStream_t st) {
     if (cfg.tw == 16 && cfg.th == 16 && cfg.eval == 0) {
         hipLaunchKernelGGL((proposal_init_kernel<16, 16, 0>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles,
-                           nh, preds, conf_thr, counter, accepted);
+                           nh, preds, stride, conf_thr, counter, accepted);
+    } else if (cfg.tw == 16 && cfg.th == 16 && cfg.eval == 2) {
+        hipLaunchKernelGGL((proposal_init_kernel<16, 16, 2>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles,
+                           nh, preds, stride, conf_thr, counter, accepted);
+    } else if (cfg.tw == 16 && cfg.th == 16 && cfg.eval == 4) {
+        hipLaunchKernelGGL((proposal_init_kernel<16, 16, 4>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles,
+                           nh, preds, stride, conf_thr, counter, accepted);
     } else if (cfg.tw == 16 && cfg.th == 16) {
         hipLaunchKernelGGL((proposal_init_kernel<16, 16, 1>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles,
-                           nh, preds, conf_thr, counter, accepted);
+                           nh, preds, stride, conf_thr, counter, accepted);
     } else {
         return hipErrorInvalidValue;
     }
@@ -945,10 +1156,10 @@ hipError_t launch_proposal_init(const TileCfg& cfg, const JobDesc* jobs, const S
 }
 
 hipError_t launch_assemble(const uint16_t* const* vols, const uint8_t* const* grays, int n, int h, int w, int Hp,
-                           int Wp, uint16_t* out, hipStream_t st) {
+                           int Wp, int stride, int wfull, uint16_t* out, hipStream_t st) {
     const int64_t total = (int64_t)n * Hp * Wp * 8;
     hipLaunchKernelGGL(assemble_input_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, vols, grays, n,
-                       h, w, Hp, Wp, out);
+                       h, w, Hp, Wp, stride, wfull, out);
     return hipGetLastError();
 }
 
@@ -956,6 +1167,12 @@ hipError_t launch_debug_cost(const JobDesc& J, const SrcDesc* S, const float* q,
                              int* ninv, float* peak, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(debug_cost_kernel, dim3((n + 63) / 64), dim3(64), 0, st, J, S, q, n, cost, ham, ninv, peak);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_div(const float* a, const float* b, int n, float* out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(debug_div_kernel, dim3((n + 255) / 256), dim3(256), 0, st, a, b, n, out);
     return hipGetLastError();
 }
 

@@ -40,15 +40,16 @@ hipError_t launch_gate(const TileCfg& cfg, const JobDesc* jobs, const int4* tile
                        float cost_thr, hipStream_t st);
 int persistent_blocks_per_cu(const TileCfg& cfg);
 hipError_t launch_volume(const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles, int nplanes,
-                         const VolOut* outs, hipStream_t st);
+                         int stride, const VolOut* outs, hipStream_t st);
 hipError_t launch_proposal_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles,
-                                int ntiles, int nh, const float* const* preds, float conf_thr, u64* counter,
-                                u64* accepted, hipStream_t st);
+                                int ntiles, int nh, const float* const* preds, int stride, float conf_thr,
+                                u64* counter, u64* accepted, hipStream_t st);
 hipError_t launch_assemble(const uint16_t* const* vols, const uint8_t* const* grays, int n, int h, int w, int Hp,
-                           int Wp, uint16_t* out, hipStream_t st);
+                           int Wp, int stride, int wfull, uint16_t* out, hipStream_t st);
 hipError_t launch_debug_cost(const JobDesc& J, const SrcDesc* S, const float* q, int n, float* cost, int* ham,
                              int* ninv, float* peak, hipStream_t st);
 hipError_t launch_debug_cvt(const float* x, int n, int* out, hipStream_t st);
+hipError_t launch_debug_div(const float* a, const float* b, int n, float* out, hipStream_t st);
 hipError_t launch_debug_rng(const int* xy, const u64* salt, int n, int ndraw, u64* seed_out, float* uni,
                             hipStream_t st);
 

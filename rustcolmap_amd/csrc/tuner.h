@@ -28,15 +28,18 @@ struct Tuner {
         return {sb, kb};
     }
 
-    // "WxH[p][r|v]": tile shape, p = persistent queue, r = reference-order
-    // evaluator, v = divide-free-abort evaluator (default: hoisted geometry)
+    // "WxH[p][r|v|w|x]": tile shape, p = persistent queue, evaluator r =
+    // reference order, v = divide-free abort, w = no hoisting + fast exact
+    // divide with the source-0 rows 1-4 batched, x = as w row by row
+    // (default: hoisted geometry)
     static bool parse(const char* s, TileCfg& c) {
         int w = 0, h = 0, used = 0;
         if (sscanf(s, "%dx%d%n", &w, &h, &used) < 2) return false;
         c.tw = w;
         c.th = h;
         c.persistent = strchr(s + used, 'p') ? 1 : 0;
-        c.eval = strchr(s + used, 'r') ? 0 : (strchr(s + used, 'v') ? 2 : 1);
+        const char* t = s + used;
+        c.eval = strchr(t, 'r') ? 0 : strchr(t, 'v') ? 2 : strchr(t, 'w') ? 3 : strchr(t, 'x') ? 4 : 1;
         return true;
     }
 

@@ -4,8 +4,9 @@ Training scenes are seeded random variants of the synthetic room (random
 camera arc, distance, height and target, 3-10 seeded spheres/boxes, random
 texture scale and noise), 8 views each.  Targets are the plain patch-match's
 own 8-sweep output on those scenes (normalised inverse depth, accepted
-normal, validity = the reference's gate), computed on the GPU; the network
-input is the 32-plane census sweep volume over 4 sources plus the gray level.
+normal, validity = the reference's gate) at every second pixel, computed on
+the GPU; the network input is the 32-plane census sweep volume over 4
+sources plus the gray level on the same coarse grid.
 
     python -m rustcolmap_amd.learned --scenes 2000 --steps 3000 --out weights.npz
 """
@@ -20,7 +21,7 @@ from typing import Callable, List, Optional
 
 import numpy as np
 
-from .mvs import MvsContext, MvsOptions, ProposalNet, ViewData
+from .mvs import NET_STRIDE, MvsContext, MvsOptions, ProposalNet, ViewData
 from .scene import SceneSpec, render_scene, select_sources
 
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
@@ -71,7 +72,7 @@ def build_training_set(ctx: MvsContext, net: ProposalNet, n_scenes: int, size=(6
         refs = list(range(step // 2, len(views), step))[:views_per_scene]
         jobs = [(r, srcs[r], zmin, zmax) for r in refs]
         out = ctx.estimate_depth_batch(jobs, opts, mode=2 | 4)
-        tg = [targets_from_depth(out[r], zmin, zmax) for r in refs]
+        tg = [targets_from_depth(out[r], zmin, zmax)[::NET_STRIDE, ::NET_STRIDE] for r in refs]
         n = net.add_samples(jobs, tg)
         if log and (si + 1) % max(1, n_scenes // 10) == 0:
             log("training set: %d/%d scenes, %d samples, %.1fs" % (si + 1, n_scenes, n, time.time() - t0))
@@ -95,15 +96,28 @@ def train(net: ProposalNet, n_samples: int, steps: int, batch: int = 8, lr: floa
     return hist
 
 
+def to_bf16_bits(p: np.ndarray) -> np.ndarray:
+    """Round-to-nearest-even bfloat16 bit patterns of float32 values."""
+    u = np.ascontiguousarray(p, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def from_bf16_bits(b: np.ndarray) -> np.ndarray:
+    return (b.astype(np.uint32) << 16).view(np.float32)
+
+
 def save_weights(net: ProposalNet, path: str, meta: dict):
+    """Weights as bfloat16 bit patterns (the convolutions run on bf16
+    weights, so inference is unchanged by the rounding)."""
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
-    np.savez_compressed(path, params=net.get_params(), **{k: np.asarray(v) for k, v in meta.items()})
+    np.savez_compressed(path, params_bf16=to_bf16_bits(net.get_params()),
+                        **{k: np.asarray(v) for k, v in meta.items()})
 
 
 def load_weights(net: ProposalNet, path: str = DEFAULT_WEIGHTS) -> dict:
     with np.load(path, allow_pickle=False) as z:
-        net.set_params(z["params"])
-        return {k: z[k] for k in z.files if k != "params"}
+        net.set_params(from_bf16_bits(z["params_bf16"]) if "params_bf16" in z.files else z["params"])
+        return {k: z[k] for k in z.files if k not in ("params", "params_bf16")}
 
 
 def main(argv=None):

@@ -119,6 +119,10 @@ class MvsStats:
     graph_used: int = 0
     ms_learned_volume: float = 0.0
     ms_learned_net: float = 0.0
+    tile_w: int = 0
+    tile_h: int = 0
+    persistent: int = 0
+    evaluator: int = 0
 
 
 def views_c(views: Sequence[ViewData]):
@@ -266,7 +270,7 @@ class MvsContext:
         oc = opts.to_c()
         check(self._lib.mvs_autotune(self._h, arr, len(jobs), C.byref(oc), int(passes), ptr(best), ptr(ms), ptr(cfg),
                                      ptr(nc)), "mvs_autotune")
-        self.last_tune = [("%dx%d%s%s" % (c[0], c[1], "p" if c[2] else "", "rfv"[c[3]] if c[3] != 1 else ""),
+        self.last_tune = [("%dx%d%s%s" % (c[0], c[1], "p" if c[2] else "", "r.vwx"[c[3]] if c[3] != 1 else ""),
                            float(t)) for c, t in zip(cfg[:nc[0]], ms[:nc[0]])]
         return tuple(int(b) for b in best)
 
@@ -442,8 +446,17 @@ def _jobs_c(jobs):
     return arr, keep
 
 
+NET_STRIDE = 2   # MVS_NET_STRIDE: the network's grid is every second pixel
+
+
+def coarse_shape(h: int, w: int, stride: int = NET_STRIDE):
+    return (h + stride - 1) // stride, (w + stride - 1) // stride
+
+
 class ProposalNet:
-    """The learned plane-proposal network (BF16 U-Net on MFMA) of a context."""
+    """The learned plane-proposal network (BF16 U-Net on MFMA) of a context.
+    It works on the coarse grid of every NET_STRIDE-th pixel: targets and
+    predictions are (ceil(h/2), ceil(w/2), 5)."""
 
     def __init__(self, ctx: MvsContext, seed: int = 0):
         self.ctx = ctx
@@ -498,7 +511,7 @@ class ProposalNet:
     def predict(self, job) -> np.ndarray:
         arr, keep = _jobs_c([job])
         v = self.ctx.views[job[0]]
-        out = np.empty((v.h, v.w, 5), np.float32)
+        out = np.empty(coarse_shape(v.h, v.w) + (5,), np.float32)
         check(self._lib.mvs_net_predict(self._h, arr, ptr(out)), "mvs_net_predict")
         return out
 
@@ -515,12 +528,13 @@ class ProposalNet:
         return grads, pred, loss
 
 
-def sweep_volume(ctx: MvsContext, job) -> np.ndarray:
-    """32-plane census sweep volume of a job: (h, w, 32) plane costs."""
+def sweep_volume(ctx: MvsContext, job, stride: int = 1) -> np.ndarray:
+    """32-plane census sweep volume of a job on the grid of every stride-th
+    pixel: (ceil(h/stride), ceil(w/stride), 32) plane costs."""
     arr, keep = _jobs_c([job])
     v = ctx.views[job[0]]
-    out = np.empty((v.h, v.w, 32), np.float32)
-    check(ctx._lib.mvs_sweep_volume(ctx._h, arr, ptr(out)), "mvs_sweep_volume")
+    out = np.empty(coarse_shape(v.h, v.w, stride) + (32,), np.float32)
+    check(ctx._lib.mvs_sweep_volume(ctx._h, arr, int(stride), ptr(out)), "mvs_sweep_volume")
     return out
 
 

@@ -1,7 +1,8 @@
 """GPU tests of the learned plane-proposal path.
 
 * the 32-plane sweep volume equals the oracle's plane_cost (hint = +inf) bit
-  for bit -- it is plane_cost at fronto-parallel planes;
+  for bit at full resolution and on the stride-2 grid -- it is plane_cost at
+  fronto-parallel planes;
 * the BF16/MFMA forward matches the CPU reference (oracle/net_ref.py) run
   with the same bf16 roundings, to bf16 tolerance;
 * the GPU gradients of every parameter tensor match fp64 CPU autograd
@@ -34,6 +35,10 @@ def test_sweep_volume_is_plane_cost(ctx):
     srcs = select_sources(sc.views, 4)
     ref = 5
     vol = sweep_volume(ctx, (ref, srcs[ref], 1.0, 10.0))
+    vol2 = sweep_volume(ctx, (ref, srcs[ref], 1.0, 10.0), stride=2)
+    assert vol.shape == (240, 320, 32) and vol2.shape == (120, 160, 32)
+    # the coarse grid samples the full-resolution volume
+    assert np.array_equal(bits(vol2), bits(vol[::2, ::2]))
     ws = ov.workspace(ref, srcs[ref], 1.0, 10.0)
     inv_min, inv_max = np.float32(1.0) / np.float32(10.0), np.float32(1.0) / np.float32(1.0)
     rng = np.random.default_rng(0)

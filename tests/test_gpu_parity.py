@@ -64,6 +64,41 @@ def test_cvt_matches_rust_as_i32(ctx):
         assert int(o) == rust(float(x)), (x, o)
 
 
+def test_fast_divide_is_ieee_division(ctx):
+    """The stereo kernels' scaling-free division equals IEEE f32 division
+    bit for bit over its domain (b and 1/b normal, a / b normal,
+    |a| >= 2^-104): log-uniform operands over the ranges the taps produce and
+    beyond, exact multiples, and quotients one product away from
+    representable numbers."""
+    import ctypes as C
+    from rustcolmap_amd import _lib
+    rng = np.random.default_rng(11)
+    n = 1 << 22
+    parts = []
+    for lo_a, hi_a, lo_b, hi_b in ((-104, 60, -14, 60), (-20, 45, -10, 10), (-3, 3, -5, 1)):
+        a = np.exp2(rng.uniform(lo_a, hi_a, n)) * rng.choice([-1.0, 1.0], n)
+        b = np.exp2(rng.uniform(lo_b, hi_b, n))
+        parts.append((a.astype(np.float32), b.astype(np.float32)))
+    q = (rng.uniform(1, 2, n) * np.exp2(rng.integers(-30, 30, n))).astype(np.float32)
+    b = (rng.uniform(0.05, 1000.0, n)).astype(np.float32)
+    parts.append(((q * b).astype(np.float32), b))
+    ones = np.float32(np.uint32(0x3fffffff).view(np.float32))     # all-ones significand
+    parts.append((rng.uniform(-1e6, 1e6, 1024).astype(np.float32), np.full(1024, ones, np.float32)))
+    a = np.concatenate([p[0] for p in parts])
+    b = np.concatenate([p[1] for p in parts])
+    with np.errstate(all="ignore"):
+        ref = a / b
+    keep = (np.abs(a) >= np.float32(2.0 ** -104)) & (np.abs(ref) >= np.float32(2.0 ** -126)) & \
+           (np.abs(ref) < np.float32(2.0 ** 96)) & (b >= np.float32(2.0 ** -126)) & (b < np.float32(2.0 ** 126))
+    a, b, ref = a[keep], b[keep], ref[keep]
+    out = np.empty_like(a)
+    _lib.check(_lib.load().mvs_debug_div(ctx._h, a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p), len(a),
+                                         out.ctypes.data_as(C.c_void_p)), "mvs_debug_div")
+    bad = np.flatnonzero(out.view(np.uint32) != ref.view(np.uint32))
+    _say("fast divide: %d quotients, %d differ" % (len(a), len(bad)))
+    assert len(bad) == 0, [(float(a[i]), float(b[i]), float(out[i]), float(ref[i])) for i in bad[:5]]
+
+
 def test_rng_streams(ctx):
     import oracle
     rng = np.random.default_rng(1)
@@ -271,7 +306,8 @@ def test_fusion_resident_matches_host_maps(ctx):
 
 
 @pytest.mark.parametrize("shape", ["16x16", "16x16r", "32x8", "32x8p", "16x8r", "16x8p", "8x8", "16x16pr", "8x8p",
-                                   "16x16v", "8x8v", "32x8pv", "16x8v"])
+                                   "16x16v", "8x8v", "32x8pv", "16x8v", "16x16w", "16x16x", "16x8w", "16x8px",
+                                   "8x8w", "8x8px", "8x8pw", "32x8w", "16x16pw"])
 def test_every_launch_shape_is_exact(ctx, shape, monkeypatch):
     """Tile shape, evaluator and schedule change only speed, never values."""
     import oracle
