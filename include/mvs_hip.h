@@ -151,8 +151,9 @@ int mvs_estimate_depth_batch(mvs_ctx *ctx, const mvs_job_t *jobs, int32_t njobs,
 /* Launch-shape autotuner (no counterpart in the reference): runs the batch
  * with every candidate (tile 16x16 / 32x8 / 16x8 / 8x8; evaluator 0
  * reference order, 1 hoisted geometry, 2 divide-free abort, 3 / 4 per-tap
- * geometry with the scaling-free exact divide, source-0 rows batched or
- * not; tile grid or persistent queue) for `passes` passes, and records the fastest
+ * geometry with the scaling-free exact divide and the strengthened abort,
+ * source-0 rows 1-2 batched or not; tile grid or persistent queue) for
+ * `passes` passes, and records the fastest
  * for the batch's shape class (longest
  * side bucket x K bucket); later batches of that class use it.  Results
  * are identical for every candidate.  best (optional, 4 ints) receives
@@ -273,7 +274,9 @@ int mvs_run_mvs(mvs_ctx *ctx, const mvs_view_t *views, int32_t nviews, const dou
 #define MVS_NET_STRIDE 2
 typedef struct mvs_net mvs_net;
 
-/* Network with seeded He-normal weights, bound to the context's device. */
+/* Network with seeded He-normal weights, bound to the context's device.
+ * Destroying the context first frees the network's device memory; the
+ * handle then only accepts mvs_net_destroy (other calls: MVS_ERR_STATE). */
 int mvs_net_create(mvs_ctx *ctx, uint64_t seed, mvs_net **out);
 void mvs_net_destroy(mvs_net *net);
 int64_t mvs_net_num_params(mvs_net *net);
@@ -297,8 +300,9 @@ int mvs_net_add_samples(mvs_net *net, const mvs_job_t *jobs, int32_t njobs, cons
 int mvs_net_clear_samples(mvs_net *net);
 
 /* One Adam step on samples idx[0..n) (same image size): forward, loss
- * (L1 inverse depth + 0.5 (1 - cos) normal on valid pixels, 0.25 BCE
- * confidence against "within 2 % of the target"), backward.  loss4
+ * (mean over valid pixels of L1 inverse depth + 0.5 (1 - cos) normal, plus
+ * 0.25 x mean over all pixels of the BCE of the confidence against "within
+ * 2 % of the target"), backward.  loss4
  * (optional): mean depth L1, mean normal term, mean BCE, valid fraction. */
 int mvs_net_train_step(mvs_net *net, const int32_t *idx, int32_t n, float lr, float *loss4);
 

@@ -82,7 +82,9 @@ def forward(params, x_nhwc, round_acts=True, round_weights=True, dtype=torch.flo
 
 def head_loss(logits, target, h, w, w_depth=1.0, w_normal=0.5, w_conf=0.25, conf_tol=0.02):
     """Loss of the library's head on the h x w image inside the padded logits.
-    target: (N, h, w, 5) = inverse depth (normalised), normal (3), validity."""
+    target: (N, h, w, 5) = inverse depth (normalised), normal (3), validity.
+    Depth L1 and normal (1 - cos) are means over the valid pixels, the
+    confidence BCE a mean over all pixels."""
     o = logits[:, :h, :w, :]
     t = torch.as_tensor(np.asarray(target), dtype=o.dtype)
     inv = torch.sigmoid(o[..., 0])
@@ -96,8 +98,8 @@ def head_loss(logits, target, h, w, w_depth=1.0, w_normal=0.5, w_conf=0.25, conf
     ln = (v * (1.0 - (n * t[..., 1:4]).sum(-1))).sum()
     c = torch.clamp(conf, 1e-7, 1 - 1e-7)
     lc = -(good * torch.log(c) + (1 - good) * torch.log(1 - c)).sum()
-    scale = 1.0 / (o.shape[0] * h * w)
-    total = scale * (w_depth * ld + w_normal * ln + w_conf * lc)
+    nv = max(float(v.sum()), 1.0)
+    total = (w_depth * ld + w_normal * ln) / nv + w_conf * lc / (o.shape[0] * h * w)
     return total, (ld, ln, lc, v.sum())
 
 

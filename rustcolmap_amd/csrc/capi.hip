@@ -102,6 +102,9 @@ struct GraphCache {
     }
 };
 
+struct mvs_net;
+static void orphan_net(mvs_net* nt);
+
 struct mvs_ctx {
     int device = 0;
     int n_cu = 256;
@@ -126,6 +129,7 @@ struct mvs_ctx {
     DBuf<uint8_t> dbg;
     mvs_stats_t stats;
     Tuner tuner;
+    std::vector<mvs_net*> nets;   // proposal networks bound to this context
 };
 
 static int set_device(mvs_ctx* c) {
@@ -179,6 +183,10 @@ void mvs_ctx_destroy(mvs_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    // networks still alive lose their device memory and their context; a
+    // later mvs_net_destroy only frees the handle
+    for (mvs_net* nt : c->nets) orphan_net(nt);
+    c->nets.clear();
     c->graph.release();
     for (auto& v : c->views) v.release();
     c->prep.release(); c->jobs.release(); c->srcs.release(); c->tiles.release();
@@ -1069,6 +1077,7 @@ int mvs_debug_rng(mvs_ctx* c, const int32_t* xy, const uint64_t* salt, int32_t n
 // view's full width wf.
 struct NetSample {
     int view = 0, h = 0, w = 0, wf = 0;
+    int64_t n_valid = 0;    // valid target pixels
     DBuf<uint16_t> vol;     // [h*w][32] bf16 sweep features
     DBuf<uint8_t> gray;     // full-resolution gray
     DBuf<float> target;     // [h*w][5]
@@ -1089,6 +1098,35 @@ struct mvs_net {
 };
 
 static inline int round16(int v) { return (v + 15) & ~15; }
+
+// Frees every device buffer of a network (its context's device current).
+static void release_net(mvs_net* nt) {
+    for (NetSample* s : nt->samples) {
+        s->vol.release();
+        s->gray.release();
+        s->target.release();
+        delete s;
+    }
+    nt->samples.clear();
+    nt->net.release();
+    nt->vol_tmp.release();
+    nt->volf_tmp.release();
+    nt->vol_outs.release();
+    nt->ptrs.release();
+    nt->preds.release();
+    nt->cnt.release();
+}
+
+static void orphan_net(mvs_net* nt) {
+    release_net(nt);
+    nt->ctx = nullptr;
+}
+
+#define NET_CTX(nt, name)                                                              \
+    do {                                                                               \
+        if (!(nt)) return fail(MVS_ERR_ARG, name ": NULL network");                    \
+        if (!(nt)->ctx) return fail(MVS_ERR_STATE, name ": the network's context is gone"); \
+    } while (0)
 
 static hipError_t launch_zero_grads(mvs_net* nt) { return net::launch_zero(nt->net.g, nt->net.n_params, nt->ctx->stream); }
 
@@ -1141,7 +1179,7 @@ static int run_volume(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, int njobs,
 // written to dpred ([n][h][w][5]).
 static int net_predict_batch(mvs_net* nt, const std::vector<const uint16_t*>& vols,
                              const std::vector<const uint8_t*>& grays, int h, int w, int wf, float* dpred, bool train,
-                             const float* dtarget, float* loss4) {
+                             const float* dtarget, float* loss4, int64_t n_valid = 0) {
     mvs_ctx* c = nt->ctx;
     const int n = (int)vols.size();
     const int Hp = round16(h), Wp = round16(w);
@@ -1156,7 +1194,7 @@ static int net_predict_batch(mvs_net* nt, const std::vector<const uint16_t*>& vo
     HCHK(launch_assemble((const uint16_t* const*)nt->ptrs.p, (const uint8_t* const*)(nt->ptrs.p + n), n, h, w, Hp, Wp,
                          MVS_NET_STRIDE, wf, nt->net.input(), c->stream));
     HCHK(nt->net.forward());
-    HCHK(nt->net.head(train, h, w, dpred, dtarget, 0.02f));
+    HCHK(nt->net.head(train, h, w, dpred, dtarget, 0.02f, n_valid));
     if (train) {
         HCHK(nt->net.backward());
         if (loss4) {
@@ -1183,6 +1221,7 @@ int mvs_net_create(mvs_ctx* c, uint64_t seed, mvs_net** out) {
             delete nt;
             return fail(MVS_ERR_HIP, std::string("mvs_net_create: ") + hipGetErrorString(e));
         }
+        c->nets.push_back(nt);
         *out = nt;
         return MVS_OK;
     } catch (const std::exception& e) {
@@ -1192,29 +1231,21 @@ int mvs_net_create(mvs_ctx* c, uint64_t seed, mvs_net** out) {
 
 void mvs_net_destroy(mvs_net* nt) {
     if (!nt) return;
-    std::lock_guard<std::mutex> lk(nt->ctx->mu);
-    (void)hipSetDevice(nt->ctx->device);
-    (void)hipStreamSynchronize(nt->ctx->stream);
-    for (NetSample* s : nt->samples) {
-        s->vol.release();
-        s->gray.release();
-        s->target.release();
-        delete s;
+    if (mvs_ctx* c = nt->ctx) {
+        std::lock_guard<std::mutex> lk(c->mu);
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        release_net(nt);
+        c->nets.erase(std::remove(c->nets.begin(), c->nets.end(), nt), c->nets.end());
     }
-    nt->net.release();
-    nt->vol_tmp.release();
-    nt->volf_tmp.release();
-    nt->vol_outs.release();
-    nt->ptrs.release();
-    nt->preds.release();
-    nt->cnt.release();
     delete nt;
 }
 
 int64_t mvs_net_num_params(mvs_net* nt) { return nt ? nt->net.n_params : -1; }
 
 int mvs_net_get_params(mvs_net* nt, float* out) {
-    if (!nt || !out) return fail(MVS_ERR_ARG, "mvs_net_get_params: NULL argument");
+    NET_CTX(nt, "mvs_net_get_params");
+    if (!out) return fail(MVS_ERR_ARG, "mvs_net_get_params: NULL argument");
     std::lock_guard<std::mutex> lk(nt->ctx->mu);
     if (int r = set_device(nt->ctx)) return r;
     HCHK(hipMemcpyAsync(out, nt->net.p, nt->net.n_params * 4, hipMemcpyDeviceToHost, nt->ctx->stream));
@@ -1223,7 +1254,8 @@ int mvs_net_get_params(mvs_net* nt, float* out) {
 }
 
 int mvs_net_set_params(mvs_net* nt, const float* in) {
-    if (!nt || !in) return fail(MVS_ERR_ARG, "mvs_net_set_params: NULL argument");
+    NET_CTX(nt, "mvs_net_set_params");
+    if (!in) return fail(MVS_ERR_ARG, "mvs_net_set_params: NULL argument");
     std::lock_guard<std::mutex> lk(nt->ctx->mu);
     if (int r = set_device(nt->ctx)) return r;
     HCHK(hipMemcpyAsync(nt->net.p, in, nt->net.n_params * 4, hipMemcpyHostToDevice, nt->ctx->stream));
@@ -1263,7 +1295,8 @@ int mvs_sweep_volume(mvs_ctx* c, const mvs_job_t* job, int32_t stride, float* ou
 }
 
 int mvs_net_add_samples(mvs_net* nt, const mvs_job_t* jobs, int32_t njobs, const float* const* targets) {
-    if (!nt || !jobs || njobs < 1 || !targets) return fail(MVS_ERR_ARG, "mvs_net_add_samples: bad arguments");
+    NET_CTX(nt, "mvs_net_add_samples");
+    if (!jobs || njobs < 1 || !targets) return fail(MVS_ERR_ARG, "mvs_net_add_samples: bad arguments");
     mvs_ctx* c = nt->ctx;
     try {
         std::lock_guard<std::mutex> lk(c->mu);
@@ -1285,6 +1318,7 @@ int mvs_net_add_samples(mvs_net* nt, const mvs_job_t* jobs, int32_t njobs, const
             HCHK(s->target.need(np * 5));
             HCHK(hipMemcpyAsync(s->gray.p, v.gray.p, (size_t)v.w * v.h, hipMemcpyDeviceToDevice, c->stream));
             HCHK(hipMemcpyAsync(s->target.p, targets[j], np * 5 * 4, hipMemcpyHostToDevice, c->stream));
+            for (size_t q = 0; q < np; q++) s->n_valid += targets[j][5 * q + 4] > 0.5f ? 1 : 0;
             outs[j].vol = s->vol.p;
             outs[j].volf = nullptr;
             fresh.push_back(s);
@@ -1299,7 +1333,7 @@ int mvs_net_add_samples(mvs_net* nt, const mvs_job_t* jobs, int32_t njobs, const
 }
 
 int mvs_net_clear_samples(mvs_net* nt) {
-    if (!nt) return fail(MVS_ERR_ARG, "mvs_net_clear_samples: NULL");
+    NET_CTX(nt, "mvs_net_clear_samples");
     std::lock_guard<std::mutex> lk(nt->ctx->mu);
     (void)hipSetDevice(nt->ctx->device);
     (void)hipStreamSynchronize(nt->ctx->stream);
@@ -1314,7 +1348,8 @@ int mvs_net_clear_samples(mvs_net* nt) {
 }
 
 int mvs_net_train_step(mvs_net* nt, const int32_t* idx, int32_t n, float lr, float* loss4) {
-    if (!nt || !idx || n < 1) return fail(MVS_ERR_ARG, "mvs_net_train_step: bad arguments");
+    NET_CTX(nt, "mvs_net_train_step");
+    if (!idx || n < 1) return fail(MVS_ERR_ARG, "mvs_net_train_step: bad arguments");
     mvs_ctx* c = nt->ctx;
     try {
         std::lock_guard<std::mutex> lk(c->mu);
@@ -1322,6 +1357,7 @@ int mvs_net_train_step(mvs_net* nt, const int32_t* idx, int32_t n, float lr, flo
         std::vector<const uint16_t*> vols;
         std::vector<const uint8_t*> grays;
         int h = -1, w = -1, wf = -1;
+        int64_t n_valid = 0;
         for (int i = 0; i < n; i++) {
             if (idx[i] < 0 || idx[i] >= (int)nt->samples.size()) return fail(MVS_ERR_ARG, "train_step: bad sample");
             NetSample* s = nt->samples[idx[i]];
@@ -1334,6 +1370,7 @@ int mvs_net_train_step(mvs_net* nt, const int32_t* idx, int32_t n, float lr, flo
             }
             vols.push_back(s->vol.p);
             grays.push_back(s->gray.p);
+            n_valid += s->n_valid;
         }
         const size_t np = (size_t)h * w;
         HCHK(nt->net.ensure(n, round16(h), round16(w), true));
@@ -1341,7 +1378,8 @@ int mvs_net_train_step(mvs_net* nt, const int32_t* idx, int32_t n, float lr, flo
             HCHK(hipMemcpyAsync(nt->net.target + (size_t)i * np * 5, nt->samples[idx[i]]->target.p, np * 5 * 4,
                                 hipMemcpyDeviceToDevice, c->stream));
         float l4[4] = {0, 0, 0, 0};
-        if (int r = net_predict_batch(nt, vols, grays, h, w, wf, nullptr, true, nt->net.target, l4)) return r;
+        if (int r = net_predict_batch(nt, vols, grays, h, w, wf, nullptr, true, nt->net.target, l4, n_valid))
+            return r;
         HCHK(nt->net.adam(lr));
         HCHK(hipStreamSynchronize(c->stream));
         if (loss4) {
@@ -1358,7 +1396,8 @@ int mvs_net_train_step(mvs_net* nt, const int32_t* idx, int32_t n, float lr, flo
 }
 
 int mvs_net_predict(mvs_net* nt, const mvs_job_t* job, float* pred) {
-    if (!nt || !job || !pred) return fail(MVS_ERR_ARG, "mvs_net_predict: NULL argument");
+    NET_CTX(nt, "mvs_net_predict");
+    if (!job || !pred) return fail(MVS_ERR_ARG, "mvs_net_predict: NULL argument");
     mvs_ctx* c = nt->ctx;
     try {
         std::lock_guard<std::mutex> lk(c->mu);
@@ -1387,7 +1426,8 @@ int mvs_net_predict(mvs_net* nt, const mvs_job_t* job, float* pred) {
 
 int mvs_net_debug_grads(mvs_net* nt, const float* input, const float* target, int32_t n, int32_t Hp, int32_t Wp,
                         int32_t h, int32_t w, float* grads, float* pred, float* loss4) {
-    if (!nt || !input || !target || n < 1 || Hp % 16 || Wp % 16 || h > Hp || w > Wp)
+    NET_CTX(nt, "mvs_net_debug_grads");
+    if (!input || !target || n < 1 || Hp % 16 || Wp % 16 || h > Hp || w > Wp)
         return fail(MVS_ERR_ARG, "mvs_net_debug_grads: bad arguments");
     mvs_ctx* c = nt->ctx;
     try {
@@ -1406,7 +1446,9 @@ int mvs_net_debug_grads(mvs_net* nt, const float* input, const float* target, in
         HCHK(hipMemcpyAsync(nt->net.target, target, (size_t)n * h * w * 5 * 4, hipMemcpyHostToDevice, c->stream));
         HCHK(launch_zero_grads(nt));
         HCHK(nt->net.forward());
-        HCHK(nt->net.head(true, h, w, nt->net.pred, nt->net.target, 0.02f));
+        int64_t n_valid = 0;
+        for (size_t q = 0; q < (size_t)n * h * w; q++) n_valid += target[5 * q + 4] > 0.5f ? 1 : 0;
+        HCHK(nt->net.head(true, h, w, nt->net.pred, nt->net.target, 0.02f, n_valid));
         HCHK(nt->net.backward());
         if (grads) HCHK(hipMemcpyAsync(grads, nt->net.g, nt->net.n_params * 4, hipMemcpyDeviceToHost, c->stream));
         if (pred) HCHK(hipMemcpyAsync(pred, nt->net.pred, (size_t)n * h * w * 5 * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1423,6 +1465,7 @@ int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, i
                                const mvs_options_t* opts, int32_t vol_k, float conf_thr, int64_t* n_accepted) {
     if (!c || !nt || !jobs || njobs < 1 || !opts || vol_k < 1)
         return fail(MVS_ERR_ARG, "mvs_estimate_depth_learned: bad arguments");
+    if (nt->ctx != c) return fail(MVS_ERR_STATE, "mvs_estimate_depth_learned: the network belongs to another context");
     try {
         std::lock_guard<std::mutex> lk(c->mu);
         if (int r = set_device(c)) return r;

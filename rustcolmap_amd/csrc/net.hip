@@ -199,7 +199,8 @@ hipError_t Net::forward() {
     return hipSuccess;
 }
 
-hipError_t Net::head(bool train, int h_valid, int w_valid, float* pred_out, const float* tgt, float conf_tol) {
+hipError_t Net::head(bool train, int h_valid, int w_valid, float* pred_out, const float* tgt, float conf_tol,
+                     int64_t n_valid) {
     HeadArgs a{};
     a.feat = act[D0];
     a.npix = (int64_t)N * H * W;
@@ -216,7 +217,8 @@ hipError_t Net::head(bool train, int h_valid, int w_valid, float* pred_out, cons
     a.w_depth = 1.0f;
     a.w_normal = 0.5f;
     a.w_conf = 0.25f;
-    a.scale = 1.0f / (float)((int64_t)N * h_valid * w_valid);
+    a.scale_c = 1.0f / (float)((int64_t)N * h_valid * w_valid);
+    a.scale_v = 1.0f / (float)(n_valid > 0 ? n_valid : 1);
     a.conf_tol = conf_tol;
     if (train) {
         hipError_t e = launch_zero(loss, 4, stream);

@@ -35,7 +35,10 @@ struct Net {
     hipError_t ensure(int N, int H, int W, bool train);
     uint16_t* input() { return act[0]; }
     hipError_t forward();
-    hipError_t head(bool train, int h_valid, int w_valid, float* pred_out, const float* tgt, float conf_tol);
+    // n_valid: valid target pixels of the batch (normalises the depth and
+    // normal terms; the confidence term is a mean over all pixels)
+    hipError_t head(bool train, int h_valid, int w_valid, float* pred_out, const float* tgt, float conf_tol,
+                    int64_t n_valid);
     hipError_t backward();
     hipError_t adam(float lr);
     void release_acts();

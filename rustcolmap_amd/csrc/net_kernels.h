@@ -47,7 +47,9 @@ struct HeadArgs {
     float* gfeat;          // [N][H][W][32] fp32 gradient of feat (NULL = inference)
     float* gw;             // [5*32 + 5]
     float* loss;           // [4]: sum L1 depth, sum normal, sum BCE, valid count
-    float w_depth, w_normal, w_conf, scale, conf_tol;
+    float w_depth, w_normal, w_conf, conf_tol;
+    float scale_v;         // depth and normal terms: 1 / valid pixels of the batch
+    float scale_c;         // confidence term: 1 / pixels of the batch
 };
 
 struct AdamArgs {

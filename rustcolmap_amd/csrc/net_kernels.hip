@@ -307,8 +307,8 @@ __global__ __launch_bounds__(256) void head_kernel(HeadArgs a) {
                 }
                 const float ct = good ? 1.f : 0.f;
                 lc = -(ct * __logf(fmaxf(conf, 1e-7f)) + (1.f - ct) * __logf(fmaxf(1.f - conf, 1e-7f)));
-                go[4] = a.w_conf * (conf - ct);
-                for (int k = 0; k < 5; ++k) go[k] *= a.scale;
+                go[4] = a.w_conf * (conf - ct) * a.scale_c;
+                for (int k = 0; k < 4; ++k) go[k] *= a.scale_v;
             }
             float* gf = a.gfeat + (size_t)p * 32;
             for (int c = 0; c < 32; ++c) {

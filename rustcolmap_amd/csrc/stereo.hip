@@ -411,15 +411,20 @@ __device__ __forceinline__ float div_rn_fast(float a, float b) {
     return __builtin_fmaf(t, r, q);
 }
 
-// Fourth evaluator (same values): no hoisted window geometry, since an
-// evaluation almost never gets past the first row of its second source (the
-// prefix average there divides the 30 summed taps by 10) -- per-tap
-// geometry is recomputed for later sources instead of holding 75 registers,
-// which keeps occupancy high.  Divides use div_rn_fast; fronto-parallel
-// planes (the init sweep) take z = d, exact because (0 * rx + 0 * ry) + 1 is
-// 1 for finite rays.  With ROWS20, the source-0 rows 1-4 issue their 20
-// gathers together and apply the four prefix tests in row order afterwards
-// (the value is 1e9 iff one of them fires, as before).
+// Fourth evaluator (same values): no hoisted window geometry, per-tap
+// geometry recomputed for every source (an evaluation rarely gets past its
+// first source, so holding 75 hoisted registers would only cost occupancy).
+// Divides use div_rn_fast; fronto-parallel planes (the init sweep) take
+// z = d, exact because (0 * rx + 0 * ry) + 1 is 1 for finite rays.
+//
+// Strengthened abort, same values: every tap adds a non-negative amount, so
+// the f32 running sum never decreases and RN(sum / D) is monotone in sum.
+// The first row of source s+1 tests sum / (5 (s+2)), a smaller divisor than
+// the later rows of source s use (e.g. 10 against 15, 20, 25 for s = 0).  So
+// once RN(sum / 5 (s+2)) > thr the evaluation is certain to abort and its
+// value is 1e9 whichever test fires first: each row end tests against the
+// smallest divisor of any test still ahead.  With PAIR, rows 1 and 2 of
+// source 0 issue their ten gathers together.
 template <bool FRONTO, bool PIN>
 __device__ __forceinline__ bool tap_v4(float2 ray, float nx, float ny, float nz, float d, const SrcP& P, int& idx) {
     float z, xr, yr;
@@ -472,55 +477,68 @@ __device__ __forceinline__ void row_v4(const u64* __restrict__ s_cen, const floa
     }
 }
 
-// Source-0 rows 1-4 with all 20 gathers issued before the sums.
+// Source-0 rows 1 and 2 with their ten gathers issued together; the two
+// row tests are applied in order afterwards (1e9 iff either fires).
 template <int PW, bool FRONTO, bool PIN>
-__device__ __forceinline__ bool rows20_v4(const u64* __restrict__ s_cen, const float2* __restrict__ s_ray, int c,
+__device__ __forceinline__ bool rows12_v4(const u64* __restrict__ s_cen, const float2* __restrict__ s_ray, int c,
                                           float nx, float ny, float nz, float d, const SrcP& P,
-                                          const float* __restrict__ tab, const AbortThr& A, float& sum) {
-    int idx[20];
+                                          const float* __restrict__ tab, const AbortThr& A, double D1, double D2,
+                                          float& sum) {
+    int idx[10];
     unsigned okm = 0;
 #pragma unroll
-    for (int t = 0; t < 20; ++t) {
+    for (int t = 0; t < 10; ++t) {
         const int r = 1 + t / 5, q = t % 5;
         const bool ok = tap_v4<FRONTO, PIN>(s_ray[c + (r - 2) * PW + (q - 2)], nx, ny, nz, d, P, idx[t]);
         okm |= (ok ? 1u : 0u) << t;
     }
-    u64 cs[20];
+    u64 cs[10];
 #pragma unroll
-    for (int t = 0; t < 20; ++t) cs[t] = P.cen[idx[t]];
+    for (int t = 0; t < 10; ++t) cs[t] = P.cen[idx[t]];
     bool ab = false;
 #pragma unroll
-    for (int t = 0; t < 20; ++t) {
+    for (int t = 0; t < 10; ++t) {
         const int r = 1 + t / 5, q = t % 5;
         const u64 rc = s_cen[c + (r - 2) * PW + (q - 2)];
         sum += ((okm >> t) & 1u) ? tab[__popcll(rc ^ cs[t])] : 1.0f;
-        if (q == 4) ab = ab || abort_test(sum, (double)(r + 1) * 5.0, A);
+        if (t == 4) ab = abort_test(sum, D1, A);
+        if (t == 9) ab = ab || abort_test(sum, D2, A);
     }
     return ab;
 }
 
-template <int PW, bool FRONTO, bool PIN, bool ROWS20>
+template <int PW, bool FRONTO, bool PIN, bool PAIR>
 __device__ __forceinline__ float eval_v4_body(const u64* __restrict__ s_cen, const float2* __restrict__ s_ray, int c,
                                               const SrcDesc* __restrict__ S, int nsrc, float nx, float ny, float nz,
                                               float d, const AbortThr& A, const float* __restrict__ tab) {
     float sum = 0.0f;
     {
+        // divisors of source 0's row tests, capped by source 1's first test
+        const double cap = nsrc > 1 ? 10.0 : 1e30;
         const SrcP P = load_src(S[0]);
         row_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, 0, nx, ny, nz, d, P, tab, sum);
         if (abort_test(sum, 5.0, A)) return 1e9f;
-        if (ROWS20) {
-            if (rows20_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, nx, ny, nz, d, P, tab, A, sum)) return 1e9f;
+        if (PAIR) {
+            if (rows12_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, nx, ny, nz, d, P, tab, A, fmin(10.0, cap),
+                                           fmin(15.0, cap), sum))
+                return 1e9f;
         } else {
 #pragma unroll
-            for (int r = 1; r < 5; ++r) {
+            for (int r = 1; r < 3; ++r) {
                 row_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, r, nx, ny, nz, d, P, tab, sum);
-                if (abort_test(sum, (double)(r + 1) * 5.0, A)) return 1e9f;
+                if (abort_test(sum, fmin((double)(r + 1) * 5.0, cap), A)) return 1e9f;
             }
+        }
+#pragma unroll
+        for (int r = 3; r < 5; ++r) {
+            row_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, r, nx, ny, nz, d, P, tab, sum);
+            if (abort_test(sum, fmin((double)(r + 1) * 5.0, cap), A)) return 1e9f;
         }
     }
     for (int si = 1; si < nsrc; ++si) {
         const SrcP P = load_src(S[si]);
         const double fs = (double)(si + 1) * 5.0;
+        const double cap = si + 1 < nsrc ? (double)(si + 2) * 5.0 : 1e30;
         // an opaque copy of the window index keeps the compiler from hoisting
         // the 25 ray and 25 census LDS reads out of this loop (100 registers)
         int cc = c;
@@ -529,13 +547,13 @@ __device__ __forceinline__ float eval_v4_body(const u64* __restrict__ s_cen, con
 #pragma unroll
             for (int r = 0; r < 5; ++r) {
                 row_v4<PW, FRONTO, true>(s_cen, s_ray, cc, r, nx, ny, nz, d, P, tab, sum);
-                if (abort_test(sum, (double)(r + 1) * fs, A)) return 1e9f;
+                if (abort_test(sum, fmin((double)(r + 1) * fs, cap), A)) return 1e9f;
             }
         } else {
 #pragma unroll
             for (int r = 0; r < 5; ++r) {
                 row_v4<PW, FRONTO, false>(s_cen, s_ray, cc, r, nx, ny, nz, d, P, tab, sum);
-                if (abort_test(sum, (double)(r + 1) * fs, A)) return 1e9f;
+                if (abort_test(sum, fmin((double)(r + 1) * fs, cap), A)) return 1e9f;
             }
         }
     }
@@ -545,14 +563,14 @@ __device__ __forceinline__ float eval_v4_body(const u64* __restrict__ s_cen, con
 // Entry of the fourth evaluator; jobs outside the fast-divide domain (bit 1
 // of pinhole_ok clear) take the reference-order evaluator, which holds no
 // hoisted state either.
-template <int PW, bool FRONTO, bool ROWS20>
+template <int PW, bool FRONTO, bool PAIR>
 __device__ __forceinline__ float eval_plane_v4(const u64* __restrict__ s_cen, const float2* __restrict__ s_ray, int c,
                                                const SrcDesc* __restrict__ S, int nsrc, float nx, float ny, float nz,
                                                float d, float thr, const float* __restrict__ tab, int pin_flags) {
     if (!(pin_flags & 2)) return eval_plane<false>(LdsRef{s_cen, s_ray, c, PW}, S, nsrc, nx, ny, nz, d, thr, nullptr);
     const AbortThr A = make_abort_thr(thr);
-    if (S[0].k == 0.0f) return eval_v4_body<PW, FRONTO, true, ROWS20>(s_cen, s_ray, c, S, nsrc, nx, ny, nz, d, A, tab);
-    return eval_v4_body<PW, FRONTO, false, ROWS20>(s_cen, s_ray, c, S, nsrc, nx, ny, nz, d, A, tab);
+    if (S[0].k == 0.0f) return eval_v4_body<PW, FRONTO, true, PAIR>(s_cen, s_ray, c, S, nsrc, nx, ny, nz, d, A, tab);
+    return eval_v4_body<PW, FRONTO, false, PAIR>(s_cen, s_ray, c, S, nsrc, nx, ny, nz, d, A, tab);
 }
 
 __device__ __forceinline__ void fill_pc_table(float* tab) {

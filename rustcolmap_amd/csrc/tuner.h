@@ -29,9 +29,9 @@ struct Tuner {
     }
 
     // "WxH[p][r|v|w|x]": tile shape, p = persistent queue, evaluator r =
-    // reference order, v = divide-free abort, w = no hoisting + fast exact
-    // divide with the source-0 rows 1-4 batched, x = as w row by row
-    // (default: hoisted geometry)
+    // reference order, v = divide-free abort, w = per-tap geometry, exact
+    // scaling-free divide and strengthened abort with the source-0 rows 1-2
+    // batched, x = as w row by row (default: hoisted geometry)
     static bool parse(const char* s, TileCfg& c) {
         int w = 0, h = 0, used = 0;
         if (sscanf(s, "%dx%d%n", &w, &h, &used) < 2) return false;

@@ -102,3 +102,10 @@ def test_head_loss_closed_form():
     assert abs(float(ln) - ((1 - 2 / (2 + 1e-6)) + (1 - 4 / (5 + 1e-6)))) < 1e-12
     assert abs(float(lc) - 2 * np.log(2)) < 1e-12 and float(nv) == 2
     assert abs(float(total) - 0.5 * (0.25 + 0.5 * float(ln) + 0.25 * 2 * np.log(2))) < 1e-12
+    # one valid pixel of two: depth and normal are means over valid pixels,
+    # the confidence term a mean over all pixels
+    t[0, 0, 1, 4] = 0.0
+    total, (ld, ln, lc, nv) = net_ref.head_loss(logits, t, 1, 2)
+    ln0 = 1 - 2 / (2 + 1e-6)
+    assert float(nv) == 1 and abs(float(ld) - 0.25) < 1e-12 and abs(float(ln) - ln0) < 1e-12
+    assert abs(float(total) - (0.25 + 0.5 * ln0 + 0.25 * 2 * np.log(2) / 2)) < 1e-12
