@@ -268,6 +268,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic is None else int(traffic[0]),
                      "traffic_source": None if traffic is None else traffic[1],
+                     "measured_hbm_frac": None if traffic is None else
+                     round(traffic[0] / (avg_pass_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
                      "bytes_per_launch": int(pass_bytes / n_passes), "avg_launch_ms": round(avg_pass_ms, 4),
                      "launch_time_source": "graph-free run after the timed steps" if args.graph
                      else "HIP events around the pass launches of the timed steps"},
