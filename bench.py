@@ -134,6 +134,8 @@ def main():
                     help="100 views 1080p on a 360-degree orbit, K=10: learned init + 2 sweeps + fusion (own JSON line)")
     ap.add_argument("--weights", default=None, help="proposal network weights (.npz)")
     ap.add_argument("--conf", type=float, default=0.5, help="confidence threshold of the learned initialisation")
+    ap.add_argument("--conf-sweep", type=lambda v: [float(x) for x in v.split(",") if x], default=[0.3, 0.1],
+                    help="further confidence thresholds reported beside --conf")
     ap.add_argument("--batched", action="store_true",
                     help="uneven batch: 64 views of mixed size, K uniform in [4, 12]; batched graph schedule vs "
                          "per-view launches (prints its own JSON line)")
@@ -348,37 +350,49 @@ def learned_bench(args):
     # plain method with 2 sweeps, no network
     _, ms_short = timed(lambda: ctx.estimate_depth_batch(jobs, short, mode=0, download=False), steps)
     short_d = {r: ctx.get_depth(r).depth for r in range(len(views))}
-    # learned initialisation + 2 sweeps
+    # learned initialisation + 2 sweeps, at each confidence threshold
     net = ProposalNet(ctx, seed=0)
     meta = load_weights(net, args.weights or DEFAULT_WEIGHTS)
-    (_, acc), ms_learned = timed(lambda: estimate_depth_learned(ctx, net, jobs, short, vol_k=4, conf_thr=args.conf,
-                                                                download=False), steps)
-    st = ctx.stats()
-    learned_d = {r: ctx.get_depth(r).depth for r in range(len(views))}
-    pts_learned, ms_fuse_learned = timed(fuse, 1)
-    agree_l = float(np.mean([_agree(judge[r], learned_d[r]) for r in judge]))
     agree_s = float(np.mean([_agree(judge[r], short_d[r]) for r in judge]))
     valid = lambda d: float(np.mean([(d[r] > 0).mean() for r in d]))
     vps_plain = len(views) / ((ms_prepare + ms_plain + ms_fuse_plain) / 1000.0)
-    vps_learned = len(views) / ((ms_prepare + ms_learned + ms_fuse_learned) / 1000.0)
+    vps_short = len(views) / ((ms_prepare + ms_short + ms_fuse_plain) / 1000.0)
+    sweep = {}
+    first = None
+    for conf in [args.conf] + [c for c in args.conf_sweep if c != args.conf]:
+        (_, acc), ms_learned = timed(lambda: estimate_depth_learned(ctx, net, jobs, short, vol_k=4, conf_thr=conf,
+                                                                    download=False), steps)
+        st = ctx.stats()
+        learned_d = {r: ctx.get_depth(r).depth for r in range(len(views))}
+        pts_learned, ms_fuse_learned = timed(fuse, 1)
+        agree_l = float(np.mean([_agree(judge[r], learned_d[r]) for r in judge]))
+        vps_learned = len(views) / ((ms_prepare + ms_learned + ms_fuse_learned) / 1000.0)
+        row = {"views_per_s": round(vps_learned, 3), "speedup_vs_plain_8": round(vps_learned / vps_plain, 3),
+               "agreement_with_plain_8": round(agree_l, 4), "valid_fraction": round(valid(learned_d), 4),
+               "accepted_fraction": round(acc / npix, 4), "ms_stereo": round(ms_learned, 2),
+               "ms_volume": round(st.ms_learned_volume, 2), "ms_net": round(st.ms_learned_net, 2),
+               "ms_init_total": round(st.ms_init, 2), "ms_passes": round(st.ms_pass, 2),
+               "ms_fusion": round(ms_fuse_learned, 2), "fused_points": int(len(pts_learned))}
+        log("conf >= %.2f: %s" % (conf, row))
+        sweep["%.2f" % conf] = row
+        if first is None:
+            first = row
     out = {"metric": "views/s, learned init + 2 sweeps vs plain 8 sweeps (%s)" % name,
-           "value": round(vps_learned, 3), "unit": "views/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+           "value": first["views_per_s"], "unit": "views/s", "n_gpus": 1, "steps": steps, "warmup": 1,
            "higher_is_better": True, "data": "synthetic",
            "config": {"workload": "%s: %d views %dx%d K=%d vol_k=4 conf>=%.2f zrange [1, %g]"
                       % (name, len(views), views[0].w, views[0].h, K, args.conf, zmax)},
-           "plain_views_per_s": round(vps_plain, 3), "speedup_views_per_s": round(vps_learned / vps_plain, 3),
-           "ms": {"census": round(ms_prepare, 2), "plain_8_sweeps": round(ms_plain, 2),
-                  "plain_2_sweeps": round(ms_short, 2), "learned_2_sweeps": round(ms_learned, 2),
-                  "learned_volume": round(st.ms_learned_volume, 2), "learned_net": round(st.ms_learned_net, 2),
-                  "fusion_plain": round(ms_fuse_plain, 2), "fusion_learned": round(ms_fuse_learned, 2)},
-           "agreement_with_plain_8": {"learned_2_sweeps": round(agree_l, 4), "plain_2_sweeps": round(agree_s, 4),
+           "plain_views_per_s": round(vps_plain, 3), "speedup_views_per_s": first["speedup_vs_plain_8"],
+           "agreement_with_plain_8": {"learned_2_sweeps": first["agreement_with_plain_8"],
+                                      "plain_2_sweeps": round(agree_s, 4),
                                       "definition": "pixels valid in the 8-sweep plain map that are valid with "
                                                     "inverse depth within 1 %"},
-           "valid_fraction": {"plain_8": round(valid(judge), 4), "plain_2": round(valid(short_d), 4),
-                              "learned_2": round(valid(learned_d), 4)},
-           "proposals_accepted_fraction": round(acc / npix, 4),
-           "fused_points": {"plain_8": int(len(pts_plain)), "learned_2": int(len(pts_learned))},
-           "fused_points_per_s_learned": round(len(pts_learned) / (ms_fuse_learned / 1000.0), 1),
+           "plain_2_sweeps": {"views_per_s": round(vps_short, 3), "agreement_with_plain_8": round(agree_s, 4),
+                              "valid_fraction": round(valid(short_d), 4), "ms_stereo": round(ms_short, 2)},
+           "plain_8_sweeps": {"views_per_s": round(vps_plain, 3), "valid_fraction": round(valid(judge), 4),
+                              "ms_stereo": round(ms_plain, 2), "ms_census": round(ms_prepare, 2),
+                              "ms_fusion": round(ms_fuse_plain, 2), "fused_points": int(len(pts_plain))},
+           "learned_by_confidence": sweep,
            "weights": {k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in meta.items()}}
     print(json.dumps(out), flush=True)
 

@@ -1534,9 +1534,13 @@ int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, i
             i = k;
         }
         HCHK(hipEventRecord(c->ev[2], c->stream));
-        // 3. proposal init with the full source sets, then the passes and the gate
+        // 3. proposal init with the full source sets, then the passes and the
+        // gate, in the launch shape the tuner holds for this batch (16x16
+        // with the fourth evaluator where the proposal kernel lacks it)
         BatchPlan P;
-        if (int r = plan_16(c, jobs, njobs, opts, P, 4)) return r;
+        if (int r = plan_batch(c, jobs, njobs, opts, 0, -1, P)) return r;
+        if (!proposal_supported(P.cfg))
+            if (int r = plan_16(c, jobs, njobs, opts, P, 4)) return r;
         if (int r = upload_plan(c, P)) return r;
         const int ncnt = 1 + P.n_passes;
         HCHK(nt->ptrs.need(njobs));

@@ -105,6 +105,7 @@ __device__ __forceinline__ float eval_plane(const Ref& R, const SrcDesc* __restr
     int ham = 0, ninv = 0;
     float peak = -__builtin_inff();
     bool aborted = false;
+#pragma nounroll
     for (int si = 0; si < nsrc; ++si) {
         const SrcDesc& s = S[si];
         const u64* __restrict__ cen = s.census;
@@ -244,6 +245,7 @@ __device__ __forceinline__ float eval_plane_fast(const u64* __restrict__ s_cen, 
             if (sum / ((float)(r + 1) * 5.0f) > thr) return 1e9f;
         }
     }
+#pragma nounroll
     for (int si = 1; si < nsrc; ++si) {
         const SrcP P = load_src(S[si]);
         const float fs = (float)(si + 1);
@@ -370,6 +372,7 @@ __device__ __forceinline__ float eval_plane_v3(const u64* __restrict__ s_cen, co
     }
     const AbortThr A = make_abort_thr(thr);
     float sum = 0.0f;
+#pragma nounroll
     for (int si = 0; si < nsrc; ++si) {
         const SrcP P = load_src(S[si]);
         const double fs = (double)(si + 1) * 5.0;
@@ -535,6 +538,9 @@ __device__ __forceinline__ float eval_v4_body(const u64* __restrict__ s_cen, con
             if (abort_test(sum, fmin((double)(r + 1) * 5.0, cap), A)) return 1e9f;
         }
     }
+    // not unrolled: evaluations rarely reach a second source, and unrolling by
+    // the run-time source count multiplies the kernel's code about tenfold
+#pragma nounroll
     for (int si = 1; si < nsrc; ++si) {
         const SrcP P = load_src(S[si]);
         const double fs = (double)(si + 1) * 5.0;
@@ -1152,25 +1158,30 @@ hipError_t launch_volume(const JobDesc* jobs, const SrcDesc* srcs, const int4* t
     return hipGetLastError();
 }
 
+#define PROPOSAL_CASES(X) \
+    X(16, 16, 0) X(16, 16, 1) X(16, 16, 2) X(16, 16, 3) X(16, 16, 4) X(16, 8, 3) X(16, 8, 4) X(8, 8, 3) X(8, 8, 4) \
+    X(32, 8, 3)
+
+bool proposal_supported(const TileCfg& cfg) {
+#define X(W, H, E) \
+    if (cfg.tw == W && cfg.th == H && cfg.eval == E) return true;
+    PROPOSAL_CASES(X)
+#undef X
+    return false;
+}
+
 hipError_t launch_proposal_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles,
                                 int ntiles, int nh, const float* const* preds, int stride, float conf_thr,
                                 u64* counter, u64* accepted, hipStream_t st) {
-    if (cfg.tw == 16 && cfg.th == 16 && cfg.eval == 0) {
-        hipLaunchKernelGGL((proposal_init_kernel<16, 16, 0>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles,
-                           nh, preds, stride, conf_thr, counter, accepted);
-    } else if (cfg.tw == 16 && cfg.th == 16 && cfg.eval == 2) {
-        hipLaunchKernelGGL((proposal_init_kernel<16, 16, 2>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles,
-                           nh, preds, stride, conf_thr, counter, accepted);
-    } else if (cfg.tw == 16 && cfg.th == 16 && cfg.eval == 4) {
-        hipLaunchKernelGGL((proposal_init_kernel<16, 16, 4>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles,
-                           nh, preds, stride, conf_thr, counter, accepted);
-    } else if (cfg.tw == 16 && cfg.th == 16) {
-        hipLaunchKernelGGL((proposal_init_kernel<16, 16, 1>), dim3(ntiles), dim3(256), 0, st, jobs, srcs, tiles, ntiles,
-                           nh, preds, stride, conf_thr, counter, accepted);
-    } else {
-        return hipErrorInvalidValue;
+#define X(W, H, E)                                                                                              \
+    if (cfg.tw == W && cfg.th == H && cfg.eval == E) {                                                          \
+        hipLaunchKernelGGL((proposal_init_kernel<W, H, E>), dim3(ntiles), dim3(W * H), 0, st, jobs, srcs, tiles, \
+                           ntiles, nh, preds, stride, conf_thr, counter, accepted);                             \
+        return hipGetLastError();                                                                               \
     }
-    return hipGetLastError();
+    PROPOSAL_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_assemble(const uint16_t* const* vols, const uint8_t* const* grays, int n, int h, int w, int Hp,

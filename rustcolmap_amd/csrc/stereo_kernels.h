@@ -41,6 +41,9 @@ hipError_t launch_gate(const TileCfg& cfg, const JobDesc* jobs, const int4* tile
 int persistent_blocks_per_cu(const TileCfg& cfg);
 hipError_t launch_volume(const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles, int nplanes,
                          int stride, const VolOut* outs, hipStream_t st);
+// launch shapes the learned initialisation kernel is built for (tile
+// origins must be even so a pixel's coarse pixel lies in its tile)
+bool proposal_supported(const TileCfg& cfg);
 hipError_t launch_proposal_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles,
                                 int ntiles, int nh, const float* const* preds, int stride, float conf_thr,
                                 u64* counter, u64* accepted, hipStream_t st);
