@@ -195,6 +195,11 @@ double mvs_fusion_cell(const double *xyz, int64_t n, double voxel_rel);
 /* write_ply_binary (mvs/src/fuse.rs:221-239). */
 int mvs_write_ply(const char *path, const mvs_point_t *pts, int64_t n);
 
+/* dense_web.json of run_mvs (mvs/src/lib.rs:437-485): outliers beyond the
+ * robust radius dropped, at most 400000 points kept by a fixed stride,
+ * positions with 4 decimals.  Writes nothing for n = 0.  Host only. */
+int mvs_write_dense_web_json(const char *path, const mvs_point_t *pts, int64_t n);
+
 /* Counters and device timings of the last call on this context. */
 int mvs_get_stats(mvs_ctx *ctx, mvs_stats_t *out);
 
@@ -255,7 +260,7 @@ typedef struct mvs_run_stats {   /* MvsStats (lib.rs:72-79) + timings */
 
 /* run_mvs for views at working resolution: plan, batched stereo of every
  * view with sources, fusion with the robust cell, and (out_dir != NULL)
- * <out_dir>/dense/dense.ply and mvs_stats.json. */
+ * <out_dir>/dense/dense.ply, dense_web.json and mvs_stats.json. */
 int mvs_run_mvs(mvs_ctx *ctx, const mvs_view_t *views, int32_t nviews, const double *xyz, const int32_t *track_off,
                 const int32_t *track_view, int64_t npoints, const mvs_options_t *opts, const char *out_dir,
                 mvs_run_stats_t *stats);

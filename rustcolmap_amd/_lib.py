@@ -86,6 +86,7 @@ _SIGS = {
     "mvs_free_points": (None, [C.POINTER(MvsPointC)]),
     "mvs_fusion_cell": (C.c_double, [_P, C.c_int64, C.c_double]),
     "mvs_write_ply": (C.c_int, [C.c_char_p, _P, C.c_int64]),
+    "mvs_write_dense_web_json": (C.c_int, [C.c_char_p, _P, C.c_int64]),
     "mvs_get_stats": (C.c_int, [_P, C.POINTER(MvsStatsC)]),
     "mvs_debug_plane_cost": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, C.c_int32, _P, _P, _P, _P]),
     "mvs_debug_rng": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, _P, _P]),

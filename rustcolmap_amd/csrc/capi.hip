@@ -907,6 +907,64 @@ double mvs_fusion_cell(const double* xyz, int64_t n, double voxel_rel) {
     return cell > 1e-9 ? cell : 1e-9;
 }
 
+// Decimated JSON copy of a fused cloud for web viewers, as run_mvs writes it
+// beside dense.ply (mvs/src/lib.rs:437-485): centroid by f64 sums in point
+// order, distances to it, median and 95th-percentile distance of the sorted
+// list, keep radius med + 6 max(|p95 - med|, max(med, 1e-9) / 2), then every
+// ceil(kept / 400000)-th kept point; positions to 4 decimals, colours as
+// "[r, g,b]".  No file for an empty cloud.
+int mvs_write_dense_web_json(const char* path, const mvs_point_t* pts, int64_t n) {
+    if (!path || (n > 0 && !pts) || n < 0) return fail(MVS_ERR_ARG, "mvs_write_dense_web_json: bad arguments");
+    if (n == 0) return MVS_OK;   // the reference writes no file for an empty cloud
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        sx += pts[i].x;
+        sy += pts[i].y;
+        sz += pts[i].z;
+    }
+    const double cx = sx / (double)n, cy = sy / (double)n, cz = sz / (double)n;
+    std::vector<double> dist((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        const double dx = pts[i].x - cx, dy = pts[i].y - cy, dz = pts[i].z - cz;
+        dist[(size_t)i] = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    std::vector<double> sorted(dist);
+    std::sort(sorted.begin(), sorted.end());
+    const double med = sorted[(size_t)n / 2];
+    const size_t i95 = std::min((size_t)((double)n * 0.95), (size_t)n - 1);
+    const double p95 = sorted[i95];
+    const double r_keep = med + 6.0 * std::max(fabs(p95 - med), std::max(med, 1e-9) * 0.5);
+    std::vector<int64_t> kept;
+    kept.reserve((size_t)n);
+    for (int64_t i = 0; i < n; i++)
+        if (dist[(size_t)i] <= r_keep) kept.push_back(i);
+    const size_t cap = 400000;
+    const size_t stride = (kept.size() + cap - 1) / cap;
+    std::vector<int64_t> web;
+    for (size_t k = 0; k < kept.size(); k += stride) web.push_back(kept[k]);
+    std::string s;
+    s.reserve(web.size() * 48 + 64);
+    char buf[256];
+    s += "{\"n\":" + std::to_string(web.size()) + ",\"pos\":[";
+    for (size_t k = 0; k < web.size(); k++) {
+        const mvs_point_t& p = pts[web[k]];
+        snprintf(buf, sizeof buf, "%s[%.4f,%.4f,%.4f]", k ? "," : "", p.x, p.y, p.z);
+        s += buf;
+    }
+    s += "],\"col\":[";
+    for (size_t k = 0; k < web.size(); k++) {
+        const mvs_point_t& p = pts[web[k]];
+        snprintf(buf, sizeof buf, "%s[%u, %u,%u]", k ? "," : "", (unsigned)p.r, (unsigned)p.g, (unsigned)p.b);
+        s += buf;
+    }
+    s += "]}";
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(MVS_ERR_ARG, std::string("mvs_write_dense_web_json: cannot create ") + path);
+    const bool ok = fwrite(s.data(), 1, s.size(), f) == s.size();
+    if (fclose(f) != 0 || !ok) return fail(MVS_ERR_ARG, std::string("mvs_write_dense_web_json: write failed: ") + path);
+    return MVS_OK;
+}
+
 int mvs_write_ply(const char* path, const mvs_point_t* pts, int64_t n) {
     if (!path || (n > 0 && !pts)) return fail(MVS_ERR_ARG, "mvs_write_ply: bad arguments");
     FILE* fp = fopen(path, "wb");
@@ -1673,6 +1731,7 @@ extern "C" int mvs_run_mvs(mvs_ctx* c, const mvs_view_t* views, int32_t nviews, 
             (void)mkdir(out_dir, 0755);
             (void)mkdir(dense.c_str(), 0755);
             r = mvs_write_ply((dense + "/dense.ply").c_str(), pts, npts);
+            if (r == MVS_OK) r = mvs_write_dense_web_json((dense + "/dense_web.json").c_str(), pts, npts);
             if (r) {
                 mvs_free_points(pts);
                 return r;
