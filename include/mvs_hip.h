@@ -291,8 +291,10 @@ int mvs_net_set_params(mvs_net *net, const float *in);
 
 /* 32-plane fronto-parallel sweep volume of one job (its k sources) on the
  * grid of every stride-th pixel (stride 1 or 2): for plane h = 0..31,
- * inv = 1/zmax + (1/zmin - 1/zmax) * h / 31 and the plane_cost at pixel
- * (stride*cx, stride*cy) of (0, 0, 1, 1/inv) with no abort;
+ * inv = 1/zmax + (1/zmin - 1/zmax) * h / 31, the census cost of the plane
+ * (0, 0, 1, 1/inv) over the 5x5 window at pixel (stride*cx, stride*cy) and
+ * the k sources -- plane_cost's Hamming sum plus 48 per invalid tap, with no
+ * abort, summed in integers -- divided by 48 * 25 * k;
  * out = ceil(w/stride) * ceil(h/stride) * 32 floats. */
 int mvs_sweep_volume(mvs_ctx *ctx, const mvs_job_t *job, int32_t stride, float *out);
 

@@ -428,7 +428,7 @@ __device__ __forceinline__ float div_rn_fast(float a, float b) {
 // value is 1e9 whichever test fires first: each row end tests against the
 // smallest divisor of any test still ahead.  With PAIR, rows 1 and 2 of
 // source 0 issue their ten gathers together.
-template <bool FRONTO, bool PIN>
+template <bool FRONTO, bool PIN, bool FD = true>
 __device__ __forceinline__ bool tap_v4(float2 ray, float nx, float ny, float nz, float d, const SrcP& P, int& idx) {
     float z, xr, yr;
     bool vd;
@@ -438,14 +438,14 @@ __device__ __forceinline__ bool tap_v4(float2 ray, float nx, float ny, float nz,
     } else {
         const float denom = (nx * ray.x + ny * ray.y) + nz;
         vd = !(denom < 0.05f);
-        z = div_rn_fast(d, denom);
+        z = FD ? div_rn_fast(d, denom) : d / denom;
     }
     xr = z * ray.x;
     yr = z * ray.y;
     const float xs = ((P.r0 * xr + P.r1 * yr) + P.r2 * z) + P.t0;
     const float ys = ((P.r3 * xr + P.r4 * yr) + P.r5 * z) + P.t1;
     const float zs = ((P.r6 * xr + P.r7 * yr) + P.r8 * z) + P.t2;
-    const float xn = div_rn_fast(xs, zs), yn = div_rn_fast(ys, zs);
+    const float xn = FD ? div_rn_fast(xs, zs) : xs / zs, yn = FD ? div_rn_fast(ys, zs) : ys / zs;
     float fu, fv;
     if (PIN) {
         fu = P.f * xn + P.cx;
@@ -879,26 +879,38 @@ __device__ __forceinline__ uint16_t f2bf16(float f) {
 
 // Fronto-parallel census sweep volume on a coarse grid: coarse pixel
 // (cx, cy) is full-resolution pixel (S*cx, S*cy) with its full-resolution
-// 5x5 window; nplanes planes uniform in inverse depth over [1/zmax, 1/zmin]
-// (the init sweep's spacing with nplanes hypotheses), each costed by
-// plane_cost over the job's sources with no abort (hint = +inf).  fp32 costs
-// (optional) and bf16 network features (cost - 0.5) * 2 are written per
-// coarse pixel.  Tiles are (job, cx0, cy0) in coarse coordinates.
+// 5x5 window (clamped at the image border); nplanes planes uniform in inverse
+// depth over [1/zmax, 1/zmin] (the init sweep's spacing with nplanes
+// hypotheses).  The value of a plane is the window's census cost summed
+// exactly in integers over the job's sources -- the Hamming distances of the
+// valid taps plus 48 per invalid tap, i.e. plane_cost's Hamming sum and
+// invalid count with no abort -- divided once by 48 * 25 * K.  On a
+// fronto-parallel plane every tap lies at depth d whichever window it belongs
+// to, so a pixel's tap cost is computed once per plane and shared by the 25
+// windows that contain it: each plane costs the tile and its apron into LDS,
+// then each coarse pixel sums its window there.  fp32 values (optional) and
+// bf16 network features (value - 0.5) * 2 are written per coarse pixel.
+template <bool PIN, bool FD>
+__device__ __forceinline__ unsigned fronto_tap_cost(u64 cref, float2 ray, float d, const SrcP& P) {
+    int idx;
+    const bool ok = tap_v4<true, PIN, FD>(ray, 0.0f, 0.0f, 1.0f, d, P, idx);
+    return ok ? (unsigned)__popcll(cref ^ P.cen[idx]) : 48u;
+}
+
 template <int TW, int TH, int S>
 __global__ __launch_bounds__(TW* TH) void volume_kernel(const JobDesc* __restrict__ jobs,
                                                         const SrcDesc* __restrict__ srcs,
                                                         const int4* __restrict__ tiles, int ntiles, int nplanes,
                                                         const VolOut* __restrict__ outs) {
-    constexpr int PW = S * TW + 4, PH = S * TH + 4;
-    __shared__ u64 s_cen[PW * PH];
-    __shared__ float2 s_ray[PW * PH];
-    __shared__ float s_tab[64];
+    constexpr int PW = S * TW + 4, PH = S * TH + 4, NP = PW * PH;
+    __shared__ u64 s_cen[NP];
+    __shared__ float2 s_ray[NP];
+    __shared__ unsigned short s_cost[NP];
     const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
     const JobDesc& J = jobs[tl.x];
     const int w = J.w, h = J.h;
-    fill_pc_table(s_tab);
     const int fx0 = S * tl.y - 2, fy0 = S * tl.z - 2;
-    for (int i = threadIdx.x; i < PW * PH; i += TW * TH) {
+    for (int i = threadIdx.x; i < NP; i += TW * TH) {
         const int ay = i / PW, ax = i % PW;
         const int gx = min(max(fx0 + ax, 0), w - 1);
         const int gy = min(max(fy0 + ay, 0), h - 1);
@@ -906,22 +918,45 @@ __global__ __launch_bounds__(TW* TH) void volume_kernel(const JobDesc* __restric
         s_cen[i] = J.census[g];
         s_ray[i] = J.rays[g];
     }
-    __syncthreads();
     const int cw = (w + S - 1) / S, ch = (h + S - 1) / S;
     const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
     const int x = tl.y + lx, y = tl.z + ly;
-    if (x >= cw || y >= ch) return;
+    const bool mine = x < cw && y < ch;
     const int c = (S * ly + 2) * PW + (S * lx + 2);
     const SrcDesc* Sd = srcs + J.src_off;
     const VolOut& O = outs[tl.x];
-    const size_t i = (size_t)y * cw + x;
+    const size_t oi = (size_t)y * cw + x;
+    const bool fast = (J.pinhole_ok & 2) != 0, pin_ok = (J.pinhole_ok & 1) != 0;
+    const float denom = (float)(48 * 25 * J.nsrc);
     for (int hi = 0; hi < nplanes; ++hi) {
         const float inv = J.inv_min + (J.inv_max - J.inv_min) * (float)hi / (float)(nplanes - 1);
         const float d = 1.0f / inv;
-        const float cst = eval_plane_v4<PW, true, false>(s_cen, s_ray, c, Sd, J.nsrc, 0.0f, 0.0f, 1.0f, d,
-                                                         __builtin_inff(), s_tab, J.pinhole_ok);
-        if (O.volf) O.volf[i * nplanes + hi] = cst;
-        if (O.vol) O.vol[i * 32 + hi] = f2bf16((cst - 0.5f) * 2.0f);
+        __syncthreads();   // staging done / previous plane's sums read
+        for (int i = threadIdx.x; i < NP; i += TW * TH) {
+            const u64 cref = s_cen[i];
+            const float2 ray = s_ray[i];
+            unsigned tot = 0;
+            for (int si = 0; si < J.nsrc; ++si) {
+                const SrcP P = load_src(Sd[si]);
+                const bool pin = pin_ok && P.k == 0.0f;
+                tot += fast ? (pin ? fronto_tap_cost<true, true>(cref, ray, d, P)
+                                   : fronto_tap_cost<false, true>(cref, ray, d, P))
+                            : (pin ? fronto_tap_cost<true, false>(cref, ray, d, P)
+                                   : fronto_tap_cost<false, false>(cref, ray, d, P));
+            }
+            s_cost[i] = (unsigned short)tot;
+        }
+        __syncthreads();
+        if (mine) {
+            unsigned sum = 0;
+#pragma unroll
+            for (int dy = -2; dy <= 2; ++dy)
+#pragma unroll
+                for (int dx = -2; dx <= 2; ++dx) sum += s_cost[c + dy * PW + dx];
+            const float v = (float)sum / denom;
+            if (O.volf) O.volf[oi * nplanes + hi] = v;
+            if (O.vol) O.vol[oi * 32 + hi] = f2bf16((v - 0.5f) * 2.0f);
+        }
     }
 }
 

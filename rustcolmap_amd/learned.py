@@ -6,7 +6,9 @@ texture scale and noise), 8 views each.  Targets are the plain patch-match's
 own 8-sweep output on those scenes (normalised inverse depth, accepted
 normal, validity = the reference's gate) at every second pixel, computed on
 the GPU; the network input is the 32-plane census sweep volume over 4
-sources plus the gray level on the same coarse grid.
+sources plus the gray level on the same coarse grid.  The confidence learns
+whether a proposal is acceptable: within 2 % of a valid target, or at a
+pixel where the plain method finds no valid depth anyway.
 
     python -m rustcolmap_amd.learned --scenes 2000 --steps 3000 --out weights.npz
 """

@@ -1,8 +1,8 @@
 """GPU tests of the learned plane-proposal path.
 
-* the 32-plane sweep volume equals the oracle's plane_cost (hint = +inf) bit
-  for bit at full resolution and on the stride-2 grid -- it is plane_cost at
-  fronto-parallel planes;
+* the 32-plane sweep volume equals the oracle plane_cost's exact window
+  census cost (Hamming sum + 48 per invalid tap, no abort) bit for bit, at
+  full resolution and on the stride-2 grid;
 * the BF16/MFMA forward matches the CPU reference (oracle/net_ref.py) run
   with the same bf16 roundings, to bf16 tolerance;
 * the GPU gradients of every parameter tensor match fp64 CPU autograd
@@ -24,7 +24,10 @@ def _say(msg):
     print(msg, flush=True)
 
 
-def test_sweep_volume_is_plane_cost(ctx):
+def test_sweep_volume_is_window_census_cost(ctx):
+    """Each volume entry is the plane's census cost over the 5x5 window and
+    the sources summed exactly -- the oracle plane_cost's Hamming sum plus 48
+    per invalid tap, evaluated with no abort -- divided once by 48 * 25 * K."""
     import oracle
     from rustcolmap_amd.mvs import sweep_volume
     from rustcolmap_amd.scene import select_sources
@@ -42,12 +45,13 @@ def test_sweep_volume_is_plane_cost(ctx):
     ws = ov.workspace(ref, srcs[ref], 1.0, 10.0)
     inv_min, inv_max = np.float32(1.0) / np.float32(10.0), np.float32(1.0) / np.float32(1.0)
     rng = np.random.default_rng(0)
-    for (x, y) in [(0, 0), (319, 239)] + [tuple(p) for p in rng.integers(0, [320, 240], size=(40, 2))]:
+    for (x, y) in [(0, 0), (319, 239), (1, 238)] + [tuple(p) for p in rng.integers(0, [320, 240], size=(40, 2))]:
         for h in range(32):
             inv = np.float32(inv_min + np.float32(np.float32(inv_max - inv_min) * np.float32(h)) / np.float32(31))
             d = np.float32(np.float32(1.0) / inv)
-            c = ws.plane_cost(x, y, 0.0, 0.0, 1.0, d, np.inf)
-            assert np.float32(vol[y, x, h]).view(np.uint32) == np.float32(c).view(np.uint32), (x, y, h)
+            _, info = ws.plane_cost(x, y, 0.0, 0.0, 1.0, d, np.inf, info=True)
+            want = np.float32(info.ham_sum + 48 * info.n_invalid) / np.float32(48 * 25 * 4)
+            assert np.float32(vol[y, x, h]).view(np.uint32) == want.view(np.uint32), (x, y, h)
 
 
 def _rand_batch(rng, n, Hp, Wp, h, w):
