@@ -447,30 +447,44 @@ def batched_bench(args):
 
 
 def cpu_baseline(views, srcs, opts, seconds):
-    """Oracle (C restatement of the reference CPU path) on a row band of
-    view 0, scaled to the same metric."""
+    """Oracle (C restatement of the reference CPU path, rayon-like threads
+    over rows) on whole reference views of the same batch in turn, then a
+    row band of the next one, until about `seconds` of wall time; scaled to
+    the same metric (pixels x sweeps / time)."""
     import oracle
     ncores = min(16, os.cpu_count() or 1)
-    ov = oracle.OracleViews(views[: 1] + [views[s] for s in srcs[0]])
-    local_srcs = list(range(1, 1 + len(srcs[0])))
-    v = views[0]
-    rows = 8
-    t = time.perf_counter()
-    ov.estimate_depth(0, local_srcs, 1.0, 10.0, opts, nthreads=ncores, rows=(v.h // 2, v.h // 2 + rows))
-    dt = time.perf_counter() - t
-    rows2 = int(min(v.h, max(rows, rows * seconds / max(dt, 1e-3))))
-    rows2 = max(ncores, rows2 - rows2 % ncores)
-    lo = max(0, v.h // 2 - rows2 // 2)
-    hi = min(v.h, lo + rows2)
-    t = time.perf_counter()
-    ov.estimate_depth(0, local_srcs, 1.0, 10.0, opts, nthreads=ncores, rows=(lo, hi))
-    dt = time.perf_counter() - t
-    val = (hi - lo) * v.w * opts.sweeps / dt / 1e6
-    log("cpu baseline: %d rows x %d px, %d threads, %.1fs -> %.4f Mpix.iter/s" % (hi - lo, v.w, ncores, dt, val))
+
+    def run(r, lo, hi):
+        ov = oracle.OracleViews([views[r]] + [views[s] for s in srcs[r]])
+        t = time.perf_counter()
+        ov.estimate_depth(0, list(range(1, 1 + len(srcs[r]))), 1.0, 10.0, opts, nthreads=ncores, rows=(lo, hi))
+        return time.perf_counter() - t
+
+    v0 = views[0]
+    probe = 2 * ncores
+    per_row = run(0, v0.h // 2, v0.h // 2 + probe) / probe
+    budget = seconds
+    done_px = 0
+    elapsed = 0.0
+    parts = []
+    r = 0
+    while r < len(views) and elapsed < budget:
+        v = views[r]
+        rows = int(min(v.h, max(ncores, (budget - elapsed) / max(per_row, 1e-6))))
+        rows -= rows % ncores if rows < v.h else 0
+        lo = (v.h - rows) // 2
+        dt = run(r, lo, lo + rows)
+        elapsed += dt
+        done_px += rows * v.w
+        per_row = dt / rows
+        parts.append("view %d rows %d-%d" % (r, lo, lo + rows) if rows < v.h else "view %d" % r)
+        r += 1
+    val = done_px * opts.sweeps / elapsed / 1e6
+    log("cpu baseline: %s, %d threads, %.1fs -> %.4f Mpix.iter/s" % (", ".join(parts), ncores, elapsed, val))
     return {"value": round(val, 5), "unit": "Mpix*iter/s", "cores": ncores, "kind": "port",
-            "sample": "oracle estimate_depth (init + %d passes, K=%d) on rows %d-%d of view 0 (%dx%d)"
-                      % (2 * opts.sweeps, len(local_srcs), lo, hi, v.w, v.h),
-            "seconds": round(dt, 2)}
+            "sample": "oracle estimate_depth (init + %d passes, K=%d, %dx%d): %s"
+                      % (2 * opts.sweeps, len(srcs[0]), v0.w, v0.h, ", ".join(parts)),
+            "seconds": round(elapsed, 2)}
 
 
 if __name__ == "__main__":
