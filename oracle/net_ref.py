@@ -84,8 +84,8 @@ def head_loss(logits, target, h, w, w_depth=1.0, w_normal=0.5, w_conf=0.25, conf
     """Loss of the library's head on the h x w image inside the padded logits.
     target: (N, h, w, 5) = inverse depth (normalised), normal (3), validity.
     Depth L1 and normal (1 - cos) are means over the valid pixels, the
-    confidence BCE a mean over all pixels, against "acceptable": within
-    conf_tol of a valid target, or an invalid target."""
+    confidence BCE a mean over all pixels, against "within conf_tol of a
+    valid target"."""
     o = logits[:, :h, :w, :]
     t = torch.as_tensor(np.asarray(target), dtype=o.dtype)
     inv = torch.sigmoid(o[..., 0])
@@ -94,7 +94,7 @@ def head_loss(logits, target, h, w, w_depth=1.0, w_normal=0.5, w_conf=0.25, conf
     conf = torch.sigmoid(o[..., 4])
     v = (t[..., 4] > 0.5).to(o.dtype)
     err = inv - t[..., 0]
-    good = ((v == 0) | (err.abs().detach() <= conf_tol * torch.clamp(t[..., 0], min=1e-3))).to(o.dtype)
+    good = ((v > 0) & (err.abs().detach() <= conf_tol * torch.clamp(t[..., 0], min=1e-3))).to(o.dtype)
     ld = (v * err.abs()).sum()
     ln = (v * (1.0 - (n * t[..., 1:4]).sum(-1))).sum()
     c = torch.clamp(conf, 1e-7, 1 - 1e-7)

@@ -294,10 +294,9 @@ __global__ __launch_bounds__(256) void head_kernel(HeadArgs a) {
                 const float* tg = a.target + 5 * q;   // inv, nx, ny, nz, valid
                 const bool v = tg[4] > 0.5f;
                 const float err = inv - tg[0];
-                // confidence target: the proposal is acceptable -- within
-                // conf_tol of a valid target, or the plain method finds no
-                // valid depth there anyway
-                const bool good = !v || fabsf(err) <= a.conf_tol * fmaxf(tg[0], 1e-3f);
+                // confidence target: a valid target the proposal lies within
+                // conf_tol of (relative, in inverse depth)
+                const bool good = v && fabsf(err) <= a.conf_tol * fmaxf(tg[0], 1e-3f);
                 if (v) {
                     ld = fabsf(err);
                     go[0] = a.w_depth * (err > 0 ? 1.f : (err < 0 ? -1.f : 0.f)) * inv * (1.f - inv);

@@ -109,14 +109,14 @@ def test_head_loss_closed_form():
     ln0 = 1 - 2 / (2 + 1e-6)
     assert float(nv) == 1 and abs(float(ld) - 0.25) < 1e-12 and abs(float(ln) - ln0) < 1e-12
     assert abs(float(total) - (0.25 + 0.5 * ln0 + 0.25 * 2 * np.log(2) / 2)) < 1e-12
-    # an invalid target pixel is an acceptable proposal (confidence target 1);
-    # a valid one 2 % off is not
+    # the confidence target is 1 only at a valid target within 2 %: an
+    # invalid pixel and a valid one 3 % off are negatives
     logits[0, 0, 0, 4] = 1.0                                          # pixel 0: conf = sigmoid(1)
     logits[0, 0, 1, 4] = 2.0                                          # pixel 1 (invalid): sigmoid(2)
     s1, s2 = 1 / (1 + np.exp(-1.0)), 1 / (1 + np.exp(-2.0))
     t[0, 0, 0] = [0.5 / 1.03, 0, 0, 1, 1]                             # inv 0.5 is 3 % above the target
     total, (ld, ln, lc, nv) = net_ref.head_loss(logits, t, 1, 2)
-    assert abs(float(lc) - (-np.log(1 - s1) - np.log(s2))) < 1e-12
-    t[0, 0, 0] = [0.5 / 1.01, 0, 0, 1, 1]                             # within 2 %: acceptable
+    assert abs(float(lc) - (-np.log(1 - s1) - np.log(1 - s2))) < 1e-12
+    t[0, 0, 0] = [0.5 / 1.01, 0, 0, 1, 1]                             # within 2 %: positive
     total, (ld, ln, lc, nv) = net_ref.head_loss(logits, t, 1, 2)
-    assert abs(float(lc) - (-np.log(s1) - np.log(s2))) < 1e-12
+    assert abs(float(lc) - (-np.log(s1) - np.log(1 - s2))) < 1e-12
