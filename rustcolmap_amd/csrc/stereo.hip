@@ -644,6 +644,91 @@ __device__ __forceinline__ void init_tile(const JobDesc& J, const SrcDesc* __res
     J.cost[0][i] = fminf(best_c, 1.0f);
 }
 
+// Init sweep with shared tap costs (evaluators 3 and 4), same values as
+// init_tile.  The 48 hypotheses are fronto-parallel, so a window tap lies at
+// depth d whichever window it belongs to and its term of plane_cost --
+// tab[popcount] at a valid tap, 1 = tab[48] at an invalid one -- depends on
+// the tap's own pixel only.  Per hypothesis and source the block computes
+// the code of every pixel of its tile and apron once into LDS; each pixel
+// then adds its 25 terms in plane_cost's order (source, row, column) with
+// the same (strengthened) abort tests.  A source is costed only while some
+// pixel of the block still needs it (block vote).  `want` selects the pixels
+// that run the sweep (all of them, or those the learned initialisation did
+// not settle); every thread of the block must call this.
+template <bool PIN, bool FD>
+__device__ __forceinline__ unsigned fronto_code(u64 cref, float2 ray, float d, const SrcP& P) {
+    int idx;
+    const bool ok = tap_v4<true, PIN, FD>(ray, 0.0f, 0.0f, 1.0f, d, P, idx);
+    return ok ? (unsigned)__popcll(cref ^ P.cen[idx]) : 48u;
+}
+
+template <int TW, int TH>
+__device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* __restrict__ srcs, int x0, int y0,
+                                              int nh, bool want, const u64* s_cen, const float2* s_ray,
+                                              unsigned char* s_code, unsigned& ne, const float* s_tab) {
+    constexpr int PW = TW + 4, NP = (TW + 4) * (TH + 4), NT = TW * TH;
+    const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
+    const int x = x0 + lx, y = y0 + ly;
+    const bool mine = want && x < J.w && y < J.h;
+    const int c = (ly + 2) * PW + (lx + 2);
+    const SrcDesc* S = srcs + J.src_off;
+    const int nsrc = J.nsrc;
+    const bool fast = (J.pinhole_ok & 2) != 0, pin_ok = (J.pinhole_ok & 1) != 0;
+    const float inv_min = J.inv_min, inv_max = J.inv_max;
+    float best_c = __builtin_inff(), best_d = 0.0f;
+    for (int hi = 0; hi < nh; ++hi) {
+        const float inv = inv_min + (inv_max - inv_min) * (float)hi / (float)(nh - 1);
+        const float d = 1.0f / inv;
+        const float thr = (best_c + 0.02f) + 0.12f;
+        const AbortThr A = make_abort_thr(thr);
+        float sum = 0.0f;
+        bool live = mine, aborted = false;
+#pragma nounroll
+        for (int si = 0; si < nsrc; ++si) {
+            // the vote is also the barrier after the previous source's sums
+            if (!__syncthreads_or(live)) break;
+            const SrcP P = load_src(S[si]);
+            const bool pin = pin_ok && P.k == 0.0f;
+            for (int i = threadIdx.x; i < NP; i += NT) {
+                const unsigned v = fast ? (pin ? fronto_code<true, true>(s_cen[i], s_ray[i], d, P)
+                                               : fronto_code<false, true>(s_cen[i], s_ray[i], d, P))
+                                        : (pin ? fronto_code<true, false>(s_cen[i], s_ray[i], d, P)
+                                               : fronto_code<false, false>(s_cen[i], s_ray[i], d, P));
+                s_code[i] = (unsigned char)v;
+            }
+            __syncthreads();
+            if (live) {
+                const double fs = (double)(si + 1) * 5.0;
+                const double cap = si + 1 < nsrc ? (double)(si + 2) * 5.0 : 1e30;
+#pragma unroll
+                for (int r = 0; r < 5; ++r) {
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) sum += s_tab[s_code[c + (r - 2) * PW + (q - 2)]];
+                    if (abort_test(sum, fmin((double)(r + 1) * fs, cap), A)) {
+                        aborted = true;
+                        live = false;
+                        break;
+                    }
+                }
+                if (si + 1 == nsrc) live = false;
+            }
+        }
+        if (mine) {
+            ++ne;
+            const float cst = aborted ? 1e9f : sum / (25.0f * (float)nsrc);
+            if (cst < best_c) {
+                best_c = cst;
+                best_d = d;
+            }
+        }
+    }
+    if (mine) {
+        const size_t i = (size_t)y * J.w + x;
+        J.plane[0][i] = make_float4(0.0f, 0.0f, 1.0f, best_d);
+        J.cost[0][i] = fminf(best_c, 1.0f);
+    }
+}
+
 // --------------------------------------------------------------------------
 // one pass (stereo.rs:279-369)
 // --------------------------------------------------------------------------
@@ -764,6 +849,7 @@ __global__ __launch_bounds__(TW* TH) void init_kernel(const JobDesc* __restrict_
                                                       u64* __restrict__ counter) {
     __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
+    __shared__ unsigned char s_code[EV >= 3 ? (TW + 4) * (TH + 4) : 1];
     __shared__ unsigned s_cnt;
     __shared__ float s_tab[64];
     const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
@@ -773,7 +859,10 @@ __global__ __launch_bounds__(TW* TH) void init_kernel(const JobDesc* __restrict_
     stage_tile<TW, TH>(J, tl.y, tl.z, s_cen, s_ray);
     __syncthreads();
     unsigned ne = 0;
-    init_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne, s_tab);
+    if (EV >= 3)
+        init_box_tile<TW, TH>(J, srcs, tl.y, tl.z, nh, true, s_cen, s_ray, s_code, ne, s_tab);
+    else
+        init_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne, s_tab);
     block_count<TW, TH>(ne, &s_cnt, counter);
 }
 
@@ -974,6 +1063,7 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
     constexpr int PW = TW + 4;
     __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
+    __shared__ unsigned char s_code[EV >= 3 ? (TW + 4) * (TH + 4) : 1];
     __shared__ unsigned s_cnt, s_acc;
     __shared__ float s_tab[64];
     const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
@@ -988,7 +1078,9 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
     unsigned ne = 0, na = 0;
     const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
     const int x = tl.y + lx, y = tl.z + ly;
-    if (x < J.w && y < J.h) {
+    const bool inside = x < J.w && y < J.h;
+    bool accept = false;
+    if (inside) {
         const int c = (ly + 2) * PW + (lx + 2);
         const size_t i = (size_t)y * J.w + x;
         const int cw = (J.w + stride - 1) / stride;
@@ -998,10 +1090,11 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
         const float a = (nx * ri.x + ny * ri.y) + nz;
         // the proposed plane passes through depth 1/inv along the ray of the
         // coarse pixel (stride*(x/stride), stride*(y/stride)), which lies in
-        // this tile because tiles start at multiples of 16
+        // this tile because tile origins are even
         const float2 rc = s_ray[c - (x % stride) - (y % stride) * PW];
         const float ac = (nx * rc.x + ny * rc.y) + nz;
-        if (conf >= conf_thr && a >= 0.1f && ac >= 0.1f && inv_n >= 0.0f && inv_n <= 1.0f) {
+        accept = conf >= conf_thr && a >= 0.1f && ac >= 0.1f && inv_n >= 0.0f && inv_n <= 1.0f;
+        if (accept) {
             const float inv = J.inv_min + (J.inv_max - J.inv_min) * inv_n;
             const float z = ((1.0f / inv) * ac) / a;
             const float cst = eval_plane_v4<PW, false, false>(s_cen, s_ray, c, srcs + J.src_off, J.nsrc, nx, ny, nz,
@@ -1010,11 +1103,17 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
             ++na;
             J.plane[0][i] = make_float4(nx, ny, nz, z);
             J.cost[0][i] = fminf(cst, 1.0f);
-        } else {
+        } else if (EV < 3) {
             unsigned ne2 = 0;
             init_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne2, s_tab);
             ne += ne2;
         }
+    }
+    if (EV >= 3) {
+        // the pixels without an accepted proposal sweep together
+        unsigned ne2 = 0;
+        init_box_tile<TW, TH>(J, srcs, tl.y, tl.z, nh, !accept, s_cen, s_ray, s_code, ne2, s_tab);
+        ne += ne2;
     }
     atomicAdd(&s_cnt, ne);
     atomicAdd(&s_acc, na);
