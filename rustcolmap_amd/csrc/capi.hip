@@ -1192,6 +1192,9 @@ static hipError_t launch_zero_grads(mvs_net* nt) { return net::launch_zero(nt->n
 static int plan_16(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_options_t* o, BatchPlan& P, int eval) {
     c->tuner.forced = true;
     c->tuner.force_cfg = TileCfg();
+    c->tuner.force_cfg.tw = 16;
+    c->tuner.force_cfg.th = 16;
+    c->tuner.force_cfg.persistent = 0;
     c->tuner.force_cfg.eval = eval;
     const int r = plan_batch(c, jobs, njobs, o, 4, -1, P);
     c->tuner.forced = false;

@@ -21,10 +21,13 @@ struct VolOut {
 };
 
 // Launch shape of the stereo stages (picked per shape class by the tuner).
+// Launch shape of the stereo kernels.  The default is the shape the tuner
+// picks at 1080p (8x8 tiles, persistent passes, evaluator 3).
 struct TileCfg {
-    int tw = 16, th = 16;
-    int persistent = 0;   // 1: persistent grid over an atomic tile queue
-    int eval = 1;         // plane_cost evaluator: 0 = reference loop order, 1 = hoisted geometry, 2 = + no-divide abort
+    int tw = 8, th = 8;
+    int persistent = 1;   // 1: persistent grid over an atomic tile queue
+    int eval = 3;         // plane_cost evaluator: 0 = reference loop order, 1 = hoisted geometry, 2 = + no-divide
+                          // abort, 3/4 = per-tap geometry, exact fast divide, strengthened abort, shared-cost init
 };
 
 int prepare_tiles(int w, int h, int& tiles_x);

@@ -51,7 +51,7 @@ struct Tuner {
         if (env && parse(env, c)) return c;
         auto it = table.find(shape_class(maxw, maxh, maxk));
         if (it != table.end()) return it->second;
-        return TileCfg();   // 16x16, tile grid
+        return TileCfg();   // 8x8, persistent, evaluator 3
     }
 };
 
