@@ -117,6 +117,91 @@ __global__ __launch_bounds__(256) void conv3x3_kernel(ConvArgs a) {
     }
 }
 
+// Forward convolution with LDS staging (mode 0): a workgroup computes a
+// 16x16-pixel x 32-channel output tile; per 32-channel input chunk it stages
+// the 18x18 input halo tile ([pixel][32 ch], 64 B per pixel) and the 9 taps'
+// weights ([tap][co][32 ci]) in LDS once, then each wave runs 9 taps x 4
+// pixel fragments (its 4 rows) x 2 channel fragments of MFMA from 16-byte
+// LDS reads -- 64 consecutive lanes read 1 KB of consecutive LDS, free of
+// bank conflicts.  Same values as conv3x3_kernel in mode 0: each output is
+// the fp32 MFMA sum over taps and channel chunks in the same order.
+constexpr int CT = 16;              // output tile side
+constexpr int CH = CT + 2;          // staged side with the halo
+
+__global__ __launch_bounds__(256) void conv3x3_lds_kernel(ConvArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[CH * CH * 32];
+    __shared__ __attribute__((aligned(16))) uint16_t s_w[9 * 32 * 32];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tiles_x = (a.W + CT - 1) / CT;
+    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int co0 = blockIdx.y * 32;
+    const int n = blockIdx.z;
+    const int cin = a.c0 + a.c1;
+    const int r16 = lane & 15, kq = lane >> 4;
+    const int x0 = tx * CT - 1, y0 = ty * CT - 1;   // staged origin (halo)
+    f32x4 acc[4][2];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int ci0 = 0; ci0 < cin; ci0 += 32) {
+        __syncthreads();   // previous chunk's LDS reads are done
+        // input halo tile: CH*CH pixels x 4 chunks of 8 channels
+        for (int j = threadIdx.x; j < CH * CH * 4; j += 256) {
+            const int pix = j >> 2, q = j & 3;
+            const int ix = x0 + pix % CH, iy = y0 + pix / CH;
+            us8 v = us8{0, 0, 0, 0, 0, 0, 0, 0};
+            if (ix >= 0 && iy >= 0 && ix < a.W && iy < a.H) {
+                const int c = ci0 + 8 * q;
+                const uint16_t* src;
+                if (ci0 < a.c0)
+                    src = a.x0 + (((size_t)n * a.H + iy) * a.W + ix) * a.c0 + c;
+                else
+                    src = a.x1 + (((size_t)n * a.H1 + (iy >> 1)) * a.W1 + (ix >> 1)) * a.c1 + (c - a.c0);
+                v = *(const us8*)src;
+            }
+            *(us8*)(s_in + pix * 32 + 8 * q) = v;
+        }
+        // weights of the 32 output channels: 9 taps x 32 co x 4 chunks
+        for (int j = threadIdx.x; j < 9 * 32 * 4; j += 256) {
+            const int q = j & 3, co = (j >> 2) & 31, t = j >> 7;
+            *(us8*)(s_w + (t * 32 + co) * 32 + 8 * q) =
+                *(const us8*)(a.wt + ((size_t)t * a.cout + co0 + co) * cin + ci0 + 8 * q);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int dy = t / 3, dx = t % 3;   // offsets into the halo tile
+            us8 bv[2];
+#pragma unroll
+            for (int nf = 0; nf < 2; ++nf) bv[nf] = *(const us8*)(s_w + (t * 32 + 16 * nf + r16) * 32 + 8 * kq);
+#pragma unroll
+            for (int mf = 0; mf < 4; ++mf) {
+                const int row = 4 * wave + mf + dy, col = r16 + dx;
+                const us8 av = *(const us8*)(s_in + (row * CH + col) * 32 + 8 * kq);
+#pragma unroll
+                for (int nf = 0; nf < 2; ++nf)
+                    acc[mf][nf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(av), as_bf(bv[nf]), acc[mf][nf], 0, 0,
+                                                                         0);
+            }
+        }
+    }
+    // D[m][n]: lane holds rows m = 4*kq + i (pixel within the fragment), column n = r16
+    for (int mf = 0; mf < 4; ++mf) {
+        const int oy = ty * CT + 4 * wave + mf;
+        if (oy >= a.H) continue;
+        for (int nf = 0; nf < 2; ++nf) {
+            const int co = co0 + 16 * nf + r16;
+            const float b = a.bias ? a.bias[co] : 0.0f;
+            for (int i = 0; i < 4; ++i) {
+                const int ox = tx * CT + 4 * kq + i;
+                if (ox >= a.W) continue;
+                float v = acc[mf][nf][i] + b;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                a.y[(((size_t)n * a.H + oy) * a.W + ox) * a.cout + co] = f2bf(v);
+            }
+        }
+    }
+}
+
 // The fragment index m of an MFMA row within a 16-pixel fragment maps to
 // pixel px = tx*16 + r16 on the A side; on the D side the row m is the
 // pixel index.  Both describe the same 16 pixels of one tile row.
@@ -383,8 +468,13 @@ __global__ void wprep_kernel(const float* __restrict__ w, int cin, int cout, uin
 static inline unsigned nb(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 hipError_t launch_conv(const ConvArgs& a, int N, hipStream_t st) {
-    const int tiles = ((a.W + 15) / 16) * ((a.H + 7) / 8);
-    hipLaunchKernelGGL(conv3x3_kernel, dim3(tiles, a.cout / 32, N), dim3(256), 0, st, a);
+    if (a.mode == 0) {
+        const int tiles = ((a.W + CT - 1) / CT) * ((a.H + CT - 1) / CT);
+        hipLaunchKernelGGL(conv3x3_lds_kernel, dim3(tiles, a.cout / 32, N), dim3(256), 0, st, a);
+    } else {
+        const int tiles = ((a.W + 15) / 16) * ((a.H + 7) / 8);
+        hipLaunchKernelGGL(conv3x3_kernel, dim3(tiles, a.cout / 32, N), dim3(256), 0, st, a);
+    }
     return hipGetLastError();
 }
 
