@@ -726,7 +726,9 @@ int mvs_autotune(mvs_ctx* c, const mvs_job_t* jobs, int32_t njobs, const mvs_opt
                 cand_cfg[4 * i + 2] = cf.persistent;
                 cand_cfg[4 * i + 3] = cf.eval;
             }
-            if (t < best_t) {
+            // a later candidate must win by 1 % to displace an earlier one, so
+            // near-ties (timing noise) resolve the same way run after run
+            if (t < best_t * 0.99) {
                 best_t = t;
                 best_cfg = cf;
             }
@@ -1153,6 +1155,7 @@ struct mvs_net {
     DBuf<uintptr_t> ptrs;     // device pointer tables (vol, gray, pred)
     DBuf<float> preds;
     DBuf<u64> cnt;
+    DBuf<u64> acc_list;       // accepted proposals (job, y, x) of the learned initialisation
 };
 
 static inline int round16(int v) { return (v + 15) & ~15; }
@@ -1173,6 +1176,7 @@ static void release_net(mvs_net* nt) {
     nt->ptrs.release();
     nt->preds.release();
     nt->cnt.release();
+    nt->acc_list.release();
 }
 
 static void orphan_net(mvs_net* nt) {
@@ -1610,9 +1614,10 @@ int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, i
         HCHK(hipMemcpyAsync(nt->ptrs.p, tab.data(), njobs * sizeof(uintptr_t), hipMemcpyHostToDevice, c->stream));
         HCHK(launch_fill_u64(c->counters.p, counter_words(P), 0ULL, c->stream));
         u64* acc_counter = c->counters.p + valid_base(P) - 4;
+        HCHK(nt->acc_list.need((size_t)P.n_pixels));
         HCHK(launch_proposal_init(P.cfg, c->jobs.p, c->srcs.p, c->tiles.p, (int)P.tiles.size(), P.nh,
                                   (const float* const*)nt->ptrs.p, MVS_NET_STRIDE, conf_thr, c->counters.p,
-                                  acc_counter, c->stream));
+                                  acc_counter, nt->acc_list.p, c->stream));
         HCHK(hipEventRecord(c->ev[3], c->stream));
         for (int p = 0; p < P.run_passes; p++) {
             const float t = (float)p / (float)P.n_passes;

@@ -1059,12 +1059,13 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
                                                                const int4* __restrict__ tiles, int ntiles, int nh,
                                                                const float* const* __restrict__ preds, int stride,
                                                                float conf_thr, u64* __restrict__ counter,
-                                                               u64* __restrict__ accepted) {
+                                                               u64* __restrict__ accepted, u64* __restrict__ list) {
     constexpr int PW = TW + 4;
     __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
     __shared__ unsigned char s_code[EV >= 3 ? (TW + 4) * (TH + 4) : 1];
     __shared__ unsigned s_cnt, s_acc;
+    __shared__ u64 s_base;
     __shared__ float s_tab[64];
     const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
     const JobDesc& J = jobs[tl.x];
@@ -1095,32 +1096,60 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
         const float ac = (nx * rc.x + ny * rc.y) + nz;
         accept = conf >= conf_thr && a >= 0.1f && ac >= 0.1f && inv_n >= 0.0f && inv_n <= 1.0f;
         if (accept) {
+            // the plane is stored now; its cost (a full evaluation over all
+            // sources, about 200 taps) is left to proposal_cost_kernel over
+            // the compacted list, so that a wave does not idle on it
             const float inv = J.inv_min + (J.inv_max - J.inv_min) * inv_n;
             const float z = ((1.0f / inv) * ac) / a;
-            const float cst = eval_plane_v4<PW, false, false>(s_cen, s_ray, c, srcs + J.src_off, J.nsrc, nx, ny, nz,
-                                                              z * a, __builtin_inff(), s_tab, J.pinhole_ok);
             ++ne;
             ++na;
             J.plane[0][i] = make_float4(nx, ny, nz, z);
-            J.cost[0][i] = fminf(cst, 1.0f);
         } else if (EV < 3) {
             unsigned ne2 = 0;
             init_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne2, s_tab);
             ne += ne2;
         }
     }
+    // slots of the accepted pixels in the list: one global atomic per block
+    unsigned slot = 0;
+    if (accept) slot = atomicAdd(&s_acc, 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = s_acc ? atomicAdd(accepted, (u64)s_acc) : 0;
+    __syncthreads();
+    if (accept) list[s_base + slot] = ((u64)tl.x << 42) | ((u64)y << 21) | (u64)x;
     if (EV >= 3) {
         // the pixels without an accepted proposal sweep together
         unsigned ne2 = 0;
         init_box_tile<TW, TH>(J, srcs, tl.y, tl.z, nh, !accept, s_cen, s_ray, s_code, ne2, s_tab);
         ne += ne2;
     }
+    (void)na;
     atomicAdd(&s_cnt, ne);
-    atomicAdd(&s_acc, na);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        if (counter) atomicAdd(counter, (u64)s_cnt);
-        if (accepted) atomicAdd(accepted, (u64)s_acc);
+    if (threadIdx.x == 0 && counter) atomicAdd(counter, (u64)s_cnt);
+}
+
+// Costs of the accepted proposals: one thread per entry of the compacted
+// list (job << 42 | y << 21 | x), plane_cost over all sources with no abort,
+// the reference loop order with clamped global window reads.  The list
+// length is read from the device; a fixed grid strides over it.
+__global__ __launch_bounds__(256) void proposal_cost_kernel(const JobDesc* __restrict__ jobs,
+                                                            const SrcDesc* __restrict__ srcs,
+                                                            const u64* __restrict__ list,
+                                                            const u64* __restrict__ count) {
+    const u64 n = *count;
+    for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (u64)gridDim.x * blockDim.x) {
+        const u64 e = list[k];
+        const int j = (int)(e >> 42), y = (int)((e >> 21) & 0x1FFFFF), x = (int)(e & 0x1FFFFF);
+        const JobDesc& J = jobs[j];
+        const size_t i = (size_t)y * J.w + x;
+        const float4 pl = J.plane[0][i];
+        const float2 ri = J.rays[i];
+        const float a = (pl.x * ri.x + pl.y * ri.y) + pl.z;
+        const GlobRef R{J.census, J.rays, x, y, J.w, J.h};
+        const float cst = eval_plane<false>(R, srcs + J.src_off, J.nsrc, pl.x, pl.y, pl.z, pl.w * a,
+                                            __builtin_inff(), nullptr);
+        J.cost[0][i] = fminf(cst, 1.0f);
     }
 }
 
@@ -1311,16 +1340,21 @@ bool proposal_supported(const TileCfg& cfg) {
 
 hipError_t launch_proposal_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles,
                                 int ntiles, int nh, const float* const* preds, int stride, float conf_thr,
-                                u64* counter, u64* accepted, hipStream_t st) {
+                                u64* counter, u64* accepted, u64* list, hipStream_t st) {
+    bool done = false;
 #define X(W, H, E)                                                                                              \
-    if (cfg.tw == W && cfg.th == H && cfg.eval == E) {                                                          \
+    if (!done && cfg.tw == W && cfg.th == H && cfg.eval == E) {                                                 \
         hipLaunchKernelGGL((proposal_init_kernel<W, H, E>), dim3(ntiles), dim3(W * H), 0, st, jobs, srcs, tiles, \
-                           ntiles, nh, preds, stride, conf_thr, counter, accepted);                             \
-        return hipGetLastError();                                                                               \
+                           ntiles, nh, preds, stride, conf_thr, counter, accepted, list);                       \
+        done = true;                                                                                            \
     }
     PROPOSAL_CASES(X)
 #undef X
-    return hipErrorInvalidValue;
+    if (!done) return hipErrorInvalidValue;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(proposal_cost_kernel, dim3(2048), dim3(256), 0, st, jobs, srcs, list, accepted);
+    return hipGetLastError();
 }
 
 hipError_t launch_assemble(const uint16_t* const* vols, const uint8_t* const* grays, int n, int h, int w, int Hp,

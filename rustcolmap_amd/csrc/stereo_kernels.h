@@ -49,7 +49,7 @@ hipError_t launch_volume(const JobDesc* jobs, const SrcDesc* srcs, const int4* t
 bool proposal_supported(const TileCfg& cfg);
 hipError_t launch_proposal_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles,
                                 int ntiles, int nh, const float* const* preds, int stride, float conf_thr,
-                                u64* counter, u64* accepted, hipStream_t st);
+                                u64* counter, u64* accepted, u64* list, hipStream_t st);
 hipError_t launch_assemble(const uint16_t* const* vols, const uint8_t* const* grays, int n, int h, int w, int Hp,
                            int Wp, int stride, int wfull, uint16_t* out, hipStream_t st);
 hipError_t launch_debug_cost(const JobDesc& J, const SrcDesc* S, const float* q, int n, float* cost, int* ham,
