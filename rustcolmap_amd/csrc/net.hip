@@ -137,7 +137,10 @@ void Net::release() {
 
 // Buffers for a batch of N padded images of H x W (multiples of 16).
 hipError_t Net::ensure(int N_, int H_, int W_, bool train) {
-    if (N_ == cap_N && H_ == cap_H && W_ == cap_W && (!train || grads_alloc)) {
+    // tensors are [N][H][W][C]: a smaller batch of the same size uses a
+    // prefix of the buffers, so only a larger batch, another image size or
+    // the first training call allocates
+    if (N_ <= cap_N && H_ == cap_H && W_ == cap_W && (!train || grads_alloc)) {
         N = N_;
         H = H_;
         W = W_;
