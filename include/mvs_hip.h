@@ -216,6 +216,15 @@ int mvs_debug_plane_cost(mvs_ctx *ctx, int32_t ref, const int32_t *srcs, int32_t
  * device (truncate toward zero, saturate, NaN -> 0), for n floats. */
 int mvs_debug_cvt(mvs_ctx *ctx, const float *x, int32_t n, int32_t *out);
 
+/* Input stage of load_views (mvs/src/lib.rs:185-197) on the device: a
+ * decoded u8 image (channels 1 = luma or 3 = RGB, ow x oh, row-major
+ * interleaved) resized to mw x mh as the image crate's
+ * resize_exact(..., Triangle) does (a copy at equal size), then its luma
+ * (gray_out, mw*mh) and RGB (rgb_out, mw*mh*3; gray replicated for a
+ * one-channel image), as to_luma8 / to_rgb8.  Either output may be NULL. */
+int mvs_resize_image(mvs_ctx *ctx, const uint8_t *src, int32_t channels, int32_t ow, int32_t oh, int32_t mw,
+                     int32_t mh, uint8_t *gray_out, uint8_t *rgb_out);
+
 /* Test probe: out[i] = a[i] / b[i] through the stereo kernels' scaling-free
  * division sequence, which is IEEE division when b and 1/b are normal, a / b
  * is normal, |a| >= 2^-104 and |a / b| < 2^96. */

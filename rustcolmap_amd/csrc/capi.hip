@@ -1087,6 +1087,54 @@ int mvs_debug_cvt(mvs_ctx* c, const float* x, int32_t n, int32_t* out) {
     return MVS_OK;
 }
 
+int mvs_resize_image(mvs_ctx* c, const uint8_t* src, int32_t channels, int32_t ow, int32_t oh, int32_t mw,
+                     int32_t mh, uint8_t* gray_out, uint8_t* rgb_out) {
+    if (!c || !src || (channels != 1 && channels != 3) || ow < 1 || oh < 1 || mw < 1 || mh < 1 ||
+        (!gray_out && !rgb_out))
+        return fail(MVS_ERR_ARG, "mvs_resize_image: bad arguments");
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        const size_t nin = (size_t)ow * oh * channels, nout = (size_t)mw * mh * channels, npx = (size_t)mw * mh;
+        DBuf<uint8_t> dsrc, dout, dgray;
+        DBuf<float> dtmp;
+        hipError_t e = dsrc.need(nin);
+        if (e == hipSuccess) e = dout.need(nout);
+        if (e == hipSuccess) e = dgray.need(npx);
+        if (e == hipSuccess && (mw != ow || mh != oh)) e = dtmp.need((size_t)ow * mh * channels);
+        if (e == hipSuccess) e = hipMemcpyAsync(dsrc.p, src, nin, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) {
+            if (mw == ow && mh == oh)   // resize_exact to the same size is a copy
+                e = hipMemcpyAsync(dout.p, dsrc.p, nin, hipMemcpyDeviceToDevice, c->stream);
+            else
+                e = launch_resize(dsrc.p, channels, ow, oh, mw, mh, dtmp.p, dout.p, c->stream);
+        }
+        if (e == hipSuccess && channels == 3 && gray_out) e = launch_luma(dout.p, (int64_t)npx, dgray.p, c->stream);
+        if (e == hipSuccess && gray_out)
+            e = hipMemcpyAsync(gray_out, channels == 3 ? dgray.p : dout.p, npx, hipMemcpyDeviceToHost, c->stream);
+        std::vector<uint8_t> host;
+        if (e == hipSuccess && rgb_out) {
+            if (channels == 3) {
+                e = hipMemcpyAsync(rgb_out, dout.p, nout, hipMemcpyDeviceToHost, c->stream);
+            } else {
+                host.resize(npx);
+                e = hipMemcpyAsync(host.data(), dout.p, npx, hipMemcpyDeviceToHost, c->stream);
+            }
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess && rgb_out && channels == 1)
+            for (size_t q = 0; q < npx; q++) rgb_out[3 * q] = rgb_out[3 * q + 1] = rgb_out[3 * q + 2] = host[q];
+        dsrc.release();
+        dout.release();
+        dgray.release();
+        dtmp.release();
+        if (e != hipSuccess) return fail(MVS_ERR_HIP, std::string("mvs_resize_image: ") + hipGetErrorString(e));
+        return MVS_OK;
+    } catch (const std::exception& ex) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_resize_image: ") + ex.what());
+    }
+}
+
 int mvs_debug_div(mvs_ctx* c, const float* a, const float* b, int32_t n, float* out) {
     if (!c || !a || !b || !out || n < 0) return fail(MVS_ERR_ARG, "mvs_debug_div: bad arguments");
     std::lock_guard<std::mutex> lk(c->mu);

@@ -54,6 +54,10 @@ hipError_t launch_assemble(const uint16_t* const* vols, const uint8_t* const* gr
                            int Wp, int stride, int wfull, uint16_t* out, hipStream_t st);
 hipError_t launch_debug_cost(const JobDesc& J, const SrcDesc* S, const float* q, int n, float* cost, int* ham,
                              int* ninv, float* peak, hipStream_t st);
+// input stage (image.hip)
+hipError_t launch_resize(const uint8_t* src, int channels, int w, int h, int mw, int mh, float* tmp, uint8_t* out,
+                         hipStream_t st);
+hipError_t launch_luma(const uint8_t* rgb, int64_t n, uint8_t* gray, hipStream_t st);
 hipError_t launch_debug_cvt(const float* x, int n, int* out, hipStream_t st);
 hipError_t launch_debug_div(const float* a, const float* b, int n, float* out, hipStream_t st);
 hipError_t launch_debug_rng(const int* xy, const u64* salt, int n, int ndraw, u64* seed_out, float* uni,

@@ -7,7 +7,8 @@
   digits, never an exponent).
 * `views_from_reconstruction`: the reference's `cam_from_rec` and the
   working-resolution arithmetic of `load_views` (mvs/src/lib.rs:103-117,
-  144-218) for images the caller has already decoded and resized.
+  144-218) for decoded images, resized on the device when they come at
+  their native size.
 * `dense_web.json` (mvs/src/lib.rs:437-485), the decimated display copy of the
   fused cloud, and `save_depth_vis` (mvs/src/fuse.rs:241-269), the grayscale
   depth visualisation as an 8-bit PNG.
@@ -448,14 +449,16 @@ def quat_to_rot(q) -> np.ndarray:
                      [2.0 * (x * z - w * y), 2.0 * (y * z + w * x), 1.0 - 2.0 * (x * x + y * y)]], np.float64)
 
 
-def views_from_reconstruction(rec: Reconstruction, images: Dict[str, Tuple[np.ndarray, Optional[np.ndarray]]],
-                              max_dim: int):
-    """ViewData for every image of `rec` (in `rec.images` order) from luma
-    (and optional RGB) the caller decoded and resized to the working size.
-    Returns (views, xyz (n, 3) float64, tracks as lists of view indices).
-    Raises FormatError for a missing camera or image, or an image whose size
-    is not the working size load_views would produce."""
-    from .mvs import Cam, Pose, ViewData
+def views_from_reconstruction(rec: Reconstruction, images: Dict[str, Tuple[Optional[np.ndarray], Optional[np.ndarray]]],
+                              max_dim: int, ctx=None):
+    """ViewData for every image of `rec` (in `rec.images` order).
+    images[name] = (gray, rgb): either at the working size load_views
+    produces (gray required, rgb optional), or one of them decoded at the
+    camera's native size, which is then resized and converted on the device
+    as load_views does (mvs.resize_image; needs `ctx`).  Returns (views,
+    xyz (n, 3) float64, tracks as lists of view indices).  Raises
+    FormatError for a missing camera or image or an unexpected size."""
+    from .mvs import Cam, Pose, ViewData, resize_image
     views = []
     for im in rec.images:
         c = rec.camera(im.camera_id)
@@ -466,6 +469,16 @@ def views_from_reconstruction(rec: Reconstruction, images: Dict[str, Tuple[np.nd
         if im.name not in images:
             raise FormatError("no pixels for image %s" % im.name)
         gray, rgb = images[im.name]
+        src = rgb if rgb is not None else gray
+        if src is None:
+            raise FormatError("no pixels for image %s" % im.name)
+        src = np.ascontiguousarray(src, np.uint8)
+        if src.shape[:2] == (c.height, c.width) and (c.height, c.width) != (mh, mw):
+            if ctx is None:
+                raise FormatError("image %s is at native size; resizing it needs a context" % im.name)
+            gray, rgb = resize_image(ctx, src, mw, mh)
+        if gray is None:
+            raise FormatError("image %s: luma missing at the working size" % im.name)
         gray = np.ascontiguousarray(gray, np.uint8)
         if gray.shape != (mh, mw):
             raise FormatError("image %s is %s, working size is %dx%d" % (im.name, gray.shape[::-1], mw, mh))

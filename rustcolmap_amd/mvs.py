@@ -528,6 +528,20 @@ class ProposalNet:
         return grads, pred, loss
 
 
+def resize_image(ctx: MvsContext, img: np.ndarray, mw: int, mh: int):
+    """load_views' input stage on the device: a decoded u8 image (h, w) or
+    (h, w, 3) resized to mw x mh with the Triangle filter, then
+    (gray (mh, mw), rgb (mh, mw, 3)) as to_luma8 / to_rgb8 make them."""
+    a = np.ascontiguousarray(img, np.uint8)
+    ch = 1 if a.ndim == 2 else a.shape[2]
+    oh, ow = a.shape[:2]
+    gray = np.empty((mh, mw), np.uint8)
+    rgb = np.empty((mh, mw, 3), np.uint8)
+    check(ctx._lib.mvs_resize_image(ctx._h, ptr(a), int(ch), int(ow), int(oh), int(mw), int(mh), ptr(gray), ptr(rgb)),
+          "mvs_resize_image")
+    return gray, rgb
+
+
 def sweep_volume(ctx: MvsContext, job, stride: int = 1) -> np.ndarray:
     """32-plane census sweep volume of a job on the grid of every stride-th
     pixel: (ceil(h/stride), ceil(w/stride), 32) plane costs."""
