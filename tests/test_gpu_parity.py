@@ -380,3 +380,15 @@ def test_run_mvs_matches_oracle_pipeline(ctx, tmp_path):
     oracle.write_ply(str(ref), exp)
     assert ply == ref.read_bytes()
     assert (tmp_path / "dense" / "mvs_stats.json").exists()
+    # dense_web.json written by the library = the restatement of
+    # mvs/src/lib.rs:437-485 on the same (oracle) points
+    from rustcolmap_amd import formats
+    from rustcolmap_amd.mvs import POINT_DTYPE
+    pts = np.zeros(len(exp), POINT_DTYPE)
+    for k in ("x", "y", "z"):
+        pts[k] = exp[k]
+    pts["rgb"] = exp["rgb"]
+    web = (tmp_path / "dense" / "dense_web.json").read_text()
+    assert web == formats.dense_web_json(pts)
+    import json
+    assert json.loads(web)["n"] <= len(exp)
