@@ -63,7 +63,7 @@ def profiled_traffic(kernel, path=TRAFFIC_SUMMARY):
     return fetch + write, os.path.relpath(path, ROOT)
 
 
-def rocprof_summary(path, copy_bytes=1 << 30, only="mvs::"):
+def rocprof_summary(path, copy_bytes=1 << 30, only=("mvs::", "mt::")):
     """Summarise rocprofv3 CSV output under `path`: per-kernel dispatch
     count / total / average duration (kernel_stats or kernel_trace CSV), and
     per-kernel averages of any collected counters (counter_collection CSV),
@@ -139,6 +139,12 @@ def main():
     ap.add_argument("--batched", action="store_true",
                     help="uneven batch: 64 views of mixed size, K uniform in [4, 12]; batched graph schedule vs "
                          "per-view launches (prints its own JSON line)")
+    ap.add_argument("--match", action="store_true",
+                    help="SfM descriptor matching: 20 frames x 2048 descriptors, all 190 pairs (prints its own "
+                         "JSON line)")
+    ap.add_argument("--match-frames", type=int, default=20)
+    ap.add_argument("--match-features", type=int, default=2048)
+    ap.add_argument("--match-exact", action="store_true", help="exact f32 tiles for every pair (no filter)")
     ap.add_argument("--profile", action="store_true",
                     help="one directly launched batch plus a 1 GiB device copy (for counter calibration), no JSON")
     args = ap.parse_args()
@@ -153,6 +159,8 @@ def main():
     from rustcolmap_amd.scene import config_spec, render_scene, select_sources
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.match:
+        return match_bench(args)
     if args.batched:
         return batched_bench(args)
     if args.learned or args.combined:
@@ -444,6 +452,118 @@ def batched_bench(args):
            "speedup_batched_vs_per_view": round(res["per_view_launches"]["ms_per_step"]
                                                 / res["batched_graph"]["ms_per_step"], 3)}
     print(json.dumps(out), flush=True)
+
+
+# f32 VALU operations/s without packed math: 256 CUs x 128 lanes/clock x 2.4 GHz (the published 157.3 TFLOP/s
+# vector f32 rate counts an FMA as two)
+VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
+MFMA_BF16_PEAK_TFLOPS = 2500.0             # dense bf16 MFMA
+MATCH_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "match_dist_tile_traffic.txt")
+
+
+def match_bench(args):
+    """Descriptor matching of run_sfm (sfm-lite/src/lib.rs:197-224):
+    `--match-frames` frames of `--match-features` SIFT-shaped descriptors
+    (synthetic sweep, 50 % overlap between neighbours), every pair i < j
+    (exhaustive), ratio 0.75.  A step = one mvs_match_frames call over all
+    pairs with the descriptors resident in HBM (match lists copied back to
+    the host inside the step).  value = image pairs matched per second.
+    With the filtered search (default) the dominant kernel is the MFMA tile
+    kernel, launched twice (pass 1 and pass 2); its algorithmic work is
+    3 bf16 products x 128 components x 2 flops per descriptor pair and
+    launch, against the dense bf16 MFMA peak.  With --match-exact it is the
+    exact distance-tile kernel: 3 f32 operations (difference, square, sum)
+    per component, 384 per descriptor pair, against the VALU rate without
+    FMA or packed math (the reference's rounding forbids both)."""
+    import torch
+    from rustcolmap_amd.matching import FrameMatcher, job_pairs, set_mode, synthetic_frames
+    from rustcolmap_amd.mvs import MvsContext
+    t0 = time.time()
+    frames = synthetic_frames(args.match_frames, args.match_features, seed=0, overlap=0.5)
+    pairs = job_pairs(len(frames), sequential=False)
+    log("generated %d frames in %.1fs" % (len(frames), time.time() - t0))
+    ctx = MvsContext(0)
+    set_mode(ctx, args.match_exact)
+    fm = FrameMatcher(ctx, frames)
+    if args.profile:
+        fm.match(pairs, 0.75)
+        log("profile run: %r" % (fm.stats(),))
+        from rustcolmap_amd import _lib as L
+        L.check(L.load().mvs_debug_copy(ctx._h, 1 << 30), "mvs_debug_copy")   # mvs::copy_kernel, 1 GiB
+        log("copy of 1073741824 bytes done")
+        return
+    for _ in range(max(args.warmup, 1)):
+        fm.match(pairs, 0.75)
+    torch.cuda.synchronize()
+    stages = []
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        res = fm.match(pairs, 0.75)
+        stages.append(fm.stats())
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t1) / args.steps
+    st = stages[-1]
+    n_match = sum(len(r) for r in res)
+    distances = st["distances"]
+    avg = {k: float(np.mean([x[k] for x in stages])) for k in ("exact_ms", "pass1_ms", "threshold_ms", "pass2_ms",
+                                                               "recheck_ms", "select_ms")}
+    filtered = st["filtered_pairs"] == st["pairs"]
+    if filtered:
+        # dominant kernel: the MFMA tile kernel, launched twice per step
+        # (pass 1 and pass 2); 3 bf16 products of 128 components per distance
+        kname = "mt::filt_tile_kernel"
+        launch_ms = (avg["pass1_ms"] + avg["pass2_ms"]) / 2.0
+        flops = distances * 3 * 2 * 128.0
+        peak, bound = MFMA_BF16_PEAK_TFLOPS, "mfma"
+    else:
+        kname = "mt::dist_tile_kernel"
+        launch_ms = avg["exact_ms"]
+        flops = distances * 384.0
+        peak, bound = VALU_PEAK_TOPS, "valu"
+    achieved = flops / (launch_ms / 1000.0) / 1e12
+    traffic = profiled_traffic(kname, MATCH_TRAFFIC_SUMMARY)
+    log("match: %.2f ms/step, stages %s, %d matches, %d overflow items" %
+        (ms, ", ".join("%s %.3f" % kv for kv in avg.items()), n_match, st["overflow_items"]))
+    out = {"metric": "SfM descriptor matching, image pairs/s (%d descriptors per frame, exhaustive pairs)"
+                     % args.match_features,
+           "value": round(len(pairs) / (ms / 1000.0), 2), "unit": "pairs/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "match%dx%d: %d frames x %d SIFT-shaped descriptors, %d pairs, ratio 0.75"
+                                  % (args.match_frames, args.match_features, len(frames), args.match_features,
+                                     len(pairs)),
+                      "search": "filtered" if filtered else "exact"},
+           "matches": n_match, "overflow_items": st["overflow_items"],
+           "stage_ms": {k: round(v, 4) for k, v in avg.items()},
+           "roofline": {"bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 2),
+                        "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                        "traffic": None if traffic is None else int(traffic[0]),
+                        "flops_per_launch": int(flops), "avg_launch_ms": round(launch_ms, 4), "kernel": kname}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = match_cpu_baseline(frames, pairs, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def match_cpu_baseline(frames, pairs, seconds):
+    """The oracle's match_pair (scalar C, both passes) over the same pairs
+    in turn, one pair per host thread, until about `seconds` of wall time."""
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    ncores = min(16, os.cpu_count() or 1)
+    oracle.load()
+    done = 0
+    t = time.perf_counter()
+    with ThreadPoolExecutor(ncores) as ex:
+        while done < len(pairs) and time.perf_counter() - t < seconds:
+            batch = pairs[done:done + ncores]
+            list(ex.map(lambda p: oracle.match_pair(frames[p[0]], frames[p[1]], 0.75), batch))
+            done += len(batch)
+    el = time.perf_counter() - t
+    val = done / el
+    log("cpu baseline: %d pairs in %.1fs on %d threads -> %.3f pairs/s" % (done, el, ncores, val))
+    return {"value": round(val, 4), "unit": "pairs/s", "cores": ncores, "kind": "port",
+            "sample": "oracle match_pair on the first %d of the %d pairs, one pair per thread" % (done, len(pairs)),
+            "seconds": round(el, 2)}
 
 
 def cpu_baseline(views, srcs, opts, seconds):

@@ -343,6 +343,51 @@ int mvs_net_debug_grads(mvs_net *net, const float *input, const float *target, i
 int mvs_estimate_depth_learned(mvs_ctx *ctx, mvs_net *net, const mvs_job_t *jobs, int32_t njobs,
                                const mvs_options_t *opts, int32_t vol_k, float conf_thr, int64_t *n_accepted);
 
+/* ---- SfM descriptor matching (sfm-lite/src/matching.rs:22-98) ----------
+ * Descriptors are 128 f32 each, row-major (SiftFrame.desc, sift.rs).  A
+ * match (i, j) of sets a and b: j is i's nearest neighbour in b and i is j's
+ * nearest in a (squared L2 summed in f32 in component order, first index
+ * on ties), both second-nearest distances are finite, d1(i) <= 0.9,
+ * d1(i) <= ratio * d2(i) and d1(j) <= ratio * d2(j) (ratio rounded to f32).
+ * Matches are listed in increasing i.  Output slots: pair p owns
+ * min(n_a, n_b) consecutive entries of out_a / out_b starting at the sum of
+ * min(n_a, n_b) over the pairs before it; out_n[p] receives its count. */
+typedef struct mvs_match_stats {   /* the last batch */
+    int64_t pairs, filtered_pairs, distances, matches;
+    int64_t overflow_items;            /* rows / columns scanned in full after the filter */
+    int64_t reserved_i[3];
+    double exact_ms;                   /* exact tiles + merge (pairs outside the filter) */
+    double pass1_ms, threshold_ms, pass2_ms, recheck_ms;   /* filtered search stages */
+    double select_ms;                  /* mutual + ratio tests, compaction */
+    double reserved[2];
+} mvs_match_stats_t;
+
+/* match_pair(fa, fb, ratio) (matching.rs:22): host sets of na and nb
+ * descriptors; out_a / out_b need min(na, nb) entries (may be NULL to get
+ * the count only); *out_n = number of matches. */
+int mvs_match_pair(mvs_ctx *ctx, const float *desc_a, int32_t na, const float *desc_b, int32_t nb, double ratio,
+                   uint32_t *out_a, uint32_t *out_b, int64_t *out_n);
+/* Frames of run_sfm (lib.rs:140-161) made resident: counts[f] descriptors
+ * of frame f, concatenated in frame order.  Replaces earlier frames. */
+int mvs_upload_descriptors(mvs_ctx *ctx, const float *desc, const int32_t *counts, int32_t nframes);
+/* The job-pair loop of run_sfm (lib.rs:197-224) for the matching step:
+ * pairs[2p], pairs[2p+1] = frame indices (i, j) of pair p, matched as
+ * match_pair(frames[i], frames[j], ratio), all pairs in one batch. */
+int mvs_match_frames(mvs_ctx *ctx, const int32_t *pairs, int32_t npairs, double ratio, uint32_t *out_a,
+                     uint32_t *out_b, int64_t *out_n);
+int mvs_get_match_stats(mvs_ctx *ctx, mvs_match_stats_t *out);
+/* Search used for pairs of two sets whose nonzero components all lie in
+ * [2^-40, 2^40]: 0 (default) = filtered (bf16x3 MFMA estimate with a proven
+ * error bound, exact f32 recheck of every possible nearest / second-nearest
+ * descriptor); 1 = exact f32 tiles for every pair.  Other sets always take
+ * the exact tiles.  Both give the reference's result bit for bit. */
+int mvs_match_set_mode(mvs_ctx *ctx, int32_t mode);
+/* Check of the filter's error bound on two safe sets: *max_rel = max over
+ * (i, j) of |exact - estimate| / (|a_i|^2 + |b_j|^2) (the filter assumes
+ * <= 2^-11). */
+int mvs_debug_match_bound(mvs_ctx *ctx, const float *desc_a, int32_t na, const float *desc_b, int32_t nb,
+                          float *max_rel);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,7 @@ def load():
         "orc_normalize": (None, [C.c_double] * 6 + [P]),
         "orc_project": (C.c_int, [P, P, C.c_double, C.c_double, C.c_double, C.c_double, P, P]),
         "orc_quat_to_rot": (None, [P, P]),
+        "orc_match_pair": (C.c_int64, [P, C.c_int, P, C.c_int, C.c_double, P, P]),
         "orc_ws_new": (P, [P, C.c_int, C.c_int, P, C.c_int, C.c_double, C.c_double]),
         "orc_ws_free": (None, [P]),
         "orc_ws_export": (None, [P, P, P, P]),
@@ -319,3 +320,16 @@ def quat_to_rot(q) -> np.ndarray:
     R = np.empty(9, np.float64)
     load().orc_quat_to_rot(_p(qa), _p(R))
     return R.reshape(3, 3)
+
+
+def match_pair(da: np.ndarray, db: np.ndarray, ratio: float = 0.8) -> np.ndarray:
+    """(n, 2) int32 index pairs (a, b): mutual nearest neighbours of two
+    128-D f32 descriptor sets that pass the symmetric ratio test
+    (sfm-lite/src/matching.rs:22-98)."""
+    a = np.ascontiguousarray(da, np.float32).reshape(-1, 128)
+    b = np.ascontiguousarray(db, np.float32).reshape(-1, 128)
+    m = max(min(len(a), len(b)), 1)
+    oa = np.empty(m, np.int32)
+    ob = np.empty(m, np.int32)
+    n = load().orc_match_pair(_p(a), len(a), _p(b), len(b), float(ratio), _p(oa), _p(ob))
+    return np.stack([oa[:n], ob[:n]], axis=1)

@@ -20,6 +20,7 @@
  *   sfm-lite/src/geom.rs:27-42 Cam::normalize             -> orc_normalize
  *   sfm-lite/src/geom.rs:63-76 Pose::project              -> orc_project
  *   sfm-lite/src/geom.rs:82-96 quat_to_rot                -> orc_quat_to_rot
+ *   sfm-lite/src/matching.rs:22-98 match_pair             -> orc_match_pair
  *
  * Float semantics follow Rust: no FMA contraction (build with
  * -ffp-contract=off), correctly rounded / and sqrt, f32 literals where the
@@ -847,4 +848,63 @@ ORC_API int orc_write_ply(const char *path, const orc_point *pts, int64_t n) {
         fwrite(c, 1, 3, fp);
     }
     return fclose(fp) == 0 ? 0 : -1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* match_pair (sfm-lite/src/matching.rs:22-98)                               */
+/* ------------------------------------------------------------------------ */
+
+/* Best and second-best squared distance of descriptor `d` against `set`:
+ * s = sum over k of (d[k] - e[k])^2 accumulated in f32 in k order; a strictly
+ * smaller s replaces the best (so the first of equal minima is kept) and
+ * demotes it to second, otherwise a strictly smaller s than second replaces
+ * second.  idx stays UINT32_MAX for an empty set. */
+static void best_two(const float *d, const float *set, int n, uint32_t *idx, float *best, float *second) {
+    float b = INFINITY, s2 = INFINITY;
+    uint32_t bi = UINT32_MAX;
+    for (int j = 0; j < n; j++) {
+        const float *e = set + (size_t)j * 128;
+        float s = 0.0f;
+        for (int k = 0; k < 128; k++) {
+            const float t = d[k] - e[k];
+            s += t * t;
+        }
+        if (s < b) {
+            s2 = b;
+            b = s;
+            bi = (uint32_t)j;
+        } else if (s < s2) {
+            s2 = s;
+        }
+    }
+    *idx = bi;
+    *best = b;
+    *second = s2;
+}
+
+/* Mutual-best matches with the symmetric ratio test, in increasing index of
+ * `a`; returns their number (out_a / out_b need min(na, nb) entries). */
+ORC_API int64_t orc_match_pair(const float *da, int na, const float *db, int nb, double ratio_f64, int32_t *out_a,
+                               int32_t *out_b) {
+    if (na <= 0 || nb <= 0) return 0;
+    uint32_t *fi = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)na);
+    float *f1 = (float *)malloc(sizeof(float) * (size_t)na), *f2 = (float *)malloc(sizeof(float) * (size_t)na);
+    uint32_t *bi = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)nb);
+    float *b1 = (float *)malloc(sizeof(float) * (size_t)nb), *b2 = (float *)malloc(sizeof(float) * (size_t)nb);
+    for (int i = 0; i < na; i++) best_two(da + (size_t)i * 128, db, nb, &fi[i], &f1[i], &f2[i]);
+    for (int j = 0; j < nb; j++) best_two(db + (size_t)j * 128, da, na, &bi[j], &b1[j], &b2[j]);
+    const float ratio = (float)ratio_f64;
+    int64_t n = 0;
+    for (int i = 0; i < na; i++) {
+        const uint32_t j = fi[i];
+        if (j == UINT32_MAX || !isfinite(f2[i])) continue;
+        if (f1[i] > 0.9f || f1[i] > ratio * f2[i]) continue;
+        if (bi[j] != (uint32_t)i) continue;
+        if (!isfinite(b2[j]) || b1[j] > ratio * b2[j]) continue;
+        out_a[n] = i;
+        out_b[n] = (int32_t)j;
+        n++;
+    }
+    free(fi); free(f1); free(f2); free(bi); free(b1); free(b2);
+    return n;
 }

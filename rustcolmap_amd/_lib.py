@@ -61,6 +61,14 @@ class MvsRunStatsC(C.Structure):
                 ("fuse_ms", C.c_double), ("per_view_valid", C.c_int64 * 4096)]
 
 
+class MvsMatchStatsC(C.Structure):
+    _fields_ = [("pairs", C.c_int64), ("filtered_pairs", C.c_int64), ("distances", C.c_int64),
+                ("matches", C.c_int64), ("overflow_items", C.c_int64), ("reserved_i", C.c_int64 * 3),
+                ("exact_ms", C.c_double), ("pass1_ms", C.c_double), ("threshold_ms", C.c_double),
+                ("pass2_ms", C.c_double), ("recheck_ms", C.c_double), ("select_ms", C.c_double),
+                ("reserved", C.c_double * 2)]
+
+
 # every symbol the header declares, with its ctypes signature
 _P = C.c_void_p
 _SIGS = {
@@ -112,6 +120,12 @@ _SIGS = {
                                       _P]),
     "mvs_estimate_depth_learned": (C.c_int, [_P, _P, C.POINTER(MvsJobC), C.c_int32, C.POINTER(MvsOptionsC),
                                              C.c_int32, C.c_float, C.POINTER(C.c_int64)]),
+    "mvs_match_pair": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, C.c_double, _P, _P, C.POINTER(C.c_int64)]),
+    "mvs_upload_descriptors": (C.c_int, [_P, _P, _P, C.c_int32]),
+    "mvs_match_frames": (C.c_int, [_P, _P, C.c_int32, C.c_double, _P, _P, _P]),
+    "mvs_get_match_stats": (C.c_int, [_P, C.POINTER(MvsMatchStatsC)]),
+    "mvs_match_set_mode": (C.c_int, [_P, C.c_int32]),
+    "mvs_debug_match_bound": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, _P]),
 }
 
 _lib = None

@@ -20,7 +20,7 @@ PKG = os.path.join(ROOT, "rustcolmap_amd")
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libmvs_hip.so")
 OBJDIR = os.path.join(PKG, "_build")
-HIP_SOURCES = ["stereo.hip", "fuse.hip", "image.hip", "net_kernels.hip", "net.hip", "capi.hip", "plan.cpp"]
+HIP_SOURCES = ["stereo.hip", "fuse.hip", "image.hip", "match.hip", "net_kernels.hip", "net.hip", "capi.hip", "plan.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

@@ -20,6 +20,7 @@
 
 #include "../../include/mvs_hip.h"
 #include "net_internal.h"
+#include "match_kernels.h"
 #include "net_kernels.h"
 #include "stereo_kernels.h"
 #include "tuner.h"
@@ -102,6 +103,43 @@ struct GraphCache {
     }
 };
 
+// descriptor matching: resident frames and the batch buffers
+struct MatchSets {
+    DBuf<float> desc;                  // [total][128]
+    DBuf<float> norm;                  // squared norms
+    DBuf<uint16_t> hi, lo;             // bf16 split
+    std::vector<int64_t> off;          // first row of each set
+    std::vector<int32_t> n;            // rows per set
+    std::vector<int> safe;             // set may use the filtered search
+    void release() {
+        desc.release(); norm.release(); hi.release(); lo.release();
+        off.clear(); n.clear(); safe.clear();
+    }
+};
+
+struct MatchState {
+    MatchSets frames;                  // mvs_upload_descriptors
+    MatchSets tmp;                     // the two sets of one mvs_match_pair call
+    int mode = 0;                      // 0: filtered where safe, 1: exact tiles only
+    DBuf<int64_t> set_off;
+    DBuf<int> bad;
+    DBuf<mt::PairDesc> pairs;
+    DBuf<mt::Best2> rpart, cpart, fwd, bwd;
+    DBuf<float2> frpart, fcpart;
+    DBuf<float> trow, tcol;
+    DBuf<uint32_t> rcnt, ccnt, rcand, ccand, ovf, ovf_n;
+    DBuf<uint32_t> oa, ob, da, db;
+    DBuf<int64_t> counts, prefix;
+    mvs_match_stats_t stats;
+    void release() {
+        frames.release(); tmp.release(); set_off.release(); bad.release();
+        pairs.release(); rpart.release(); cpart.release(); fwd.release(); bwd.release();
+        frpart.release(); fcpart.release(); trow.release(); tcol.release();
+        rcnt.release(); ccnt.release(); rcand.release(); ccand.release(); ovf.release(); ovf_n.release();
+        oa.release(); ob.release(); da.release(); db.release(); counts.release(); prefix.release();
+    }
+};
+
 struct mvs_net;
 static void orphan_net(mvs_net* nt);
 
@@ -130,6 +168,7 @@ struct mvs_ctx {
     mvs_stats_t stats;
     Tuner tuner;
     std::vector<mvs_net*> nets;   // proposal networks bound to this context
+    MatchState match;
 };
 
 static int set_device(mvs_ctx* c) {
@@ -192,6 +231,7 @@ void mvs_ctx_destroy(mvs_ctx* c) {
     c->prep.release(); c->jobs.release(); c->srcs.release(); c->tiles.release();
     c->counters.release(); c->queues.release(); c->fviews.release(); c->forder.release();
     c->tmp_depth.release(); c->dbg.release();
+    c->match.release();
     free_fuse_scratch(c->fuse);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1810,3 +1850,289 @@ extern "C" int mvs_run_mvs(mvs_ctx* c, const mvs_view_t* views, int32_t nviews, 
         return fail(MVS_ERR_NOMEM, std::string("mvs_run_mvs: ") + e.what());
     }
 }
+
+// --------------------------------------------------------------------------
+// descriptor matching (sfm-lite/src/matching.rs:22-98, lib.rs:197-224)
+// --------------------------------------------------------------------------
+
+// Upload sets to S (host descriptors, concatenated) and prepare norms, the
+// bf16 split and the per-set safety flags.
+static int match_prepare(mvs_ctx* c, MatchSets& S, const float* desc, const int32_t* counts, int32_t nsets,
+                         const char* who) {
+    MatchState& M = c->match;
+    int64_t total = 0;
+    for (int32_t f = 0; f < nsets; f++) {
+        if (counts[f] < 0) return fail(MVS_ERR_ARG, std::string(who) + ": negative descriptor count");
+        total += counts[f];
+    }
+    if (total > 0 && !desc) return fail(MVS_ERR_ARG, std::string(who) + ": NULL descriptors");
+    S.off.assign((size_t)nsets, 0);
+    S.n.assign(counts, counts + nsets);
+    S.safe.assign((size_t)nsets, 0);
+    int64_t o = 0;
+    for (int32_t f = 0; f < nsets; f++) {
+        S.off[(size_t)f] = o;
+        o += counts[f];
+    }
+    const size_t rows = (size_t)std::max<int64_t>(total, 1);
+    HCHK(S.desc.need(rows * mt::DIM));
+    HCHK(S.norm.need(rows));
+    HCHK(S.hi.need(rows * mt::DIM));
+    HCHK(S.lo.need(rows * mt::DIM));
+    HCHK(M.set_off.need((size_t)std::max(nsets, 1)));
+    HCHK(M.bad.need((size_t)std::max(nsets, 1)));
+    hipStream_t st = c->stream;
+    if (total > 0)
+        HCHK(hipMemcpyAsync(S.desc.p, desc, sizeof(float) * mt::DIM * (size_t)total, hipMemcpyHostToDevice, st));
+    if (nsets > 0) {
+        HCHK(hipMemcpyAsync(M.set_off.p, S.off.data(), sizeof(int64_t) * (size_t)nsets, hipMemcpyHostToDevice, st));
+        HCHK(hipMemsetAsync(M.bad.p, 0, sizeof(int) * (size_t)nsets, st));
+    }
+    HCHK(mt::launch_prep(S.desc.p, total, M.set_off.p, nsets, S.norm.p, S.hi.p, S.lo.p, M.bad.p, st));
+    std::vector<int> bad((size_t)nsets);
+    if (nsets > 0) HCHK(hipMemcpyAsync(bad.data(), M.bad.p, sizeof(int) * (size_t)nsets, hipMemcpyDeviceToHost, st));
+    HCHK(hipStreamSynchronize(st));
+    for (int32_t f = 0; f < nsets; f++) S.safe[(size_t)f] = bad[(size_t)f] == 0;
+    return MVS_OK;
+}
+
+// One batch over the sets of S; pairs index the sets.  Pairs of two safe
+// sets take the filtered search (unless mode == 1), the rest the exact
+// tiles; both write fwd / bwd, then one selection pass lists the matches.
+static int match_batch(mvs_ctx* c, MatchSets& S, const int32_t* pairs, int32_t npairs, double ratio,
+                       uint32_t* out_a, uint32_t* out_b, int64_t* out_n, const char* who) {
+    MatchState& M = c->match;
+    std::vector<mt::PairDesc> all((size_t)npairs), ex, fl;
+    int64_t fr = 0, fc = 0, out = 0;
+    for (int32_t p = 0; p < npairs; p++) {
+        const int32_t ia = pairs[2 * p], ib = pairs[2 * p + 1];
+        if (ia < 0 || ib < 0 || ia >= (int32_t)S.n.size() || ib >= (int32_t)S.n.size())
+            return fail(MVS_ERR_ARG, std::string(who) + ": pair index out of range");
+        mt::PairDesc d = {};
+        d.a = S.off[(size_t)ia];
+        d.b = S.off[(size_t)ib];
+        d.na = S.n[(size_t)ia];
+        d.nb = S.n[(size_t)ib];
+        d.frow = fr;
+        d.fcol = fc;
+        d.out = out;
+        fr += d.na;
+        fc += d.nb;
+        out += std::min(d.na, d.nb);
+        all[(size_t)p] = d;
+        if (d.na == 0 || d.nb == 0) continue;   // no matches, nothing to search
+        const bool filt = M.mode == 0 && S.safe[(size_t)ia] && S.safe[(size_t)ib];
+        (filt ? fl : ex).push_back(d);
+    }
+    // launch-local numbering: tiles, partials, rows and columns
+    auto number = [](std::vector<mt::PairDesc>& v, int tile, int64_t& tiles, int64_t& rp, int64_t& cp, int64_t& lr,
+                     int64_t& lc) {
+        tiles = rp = cp = lr = lc = 0;
+        for (auto& d : v) {
+            d.ta = (d.na + tile - 1) / tile;
+            d.tb = (d.nb + tile - 1) / tile;
+            d.tile0 = tiles;
+            d.rpart = rp;
+            d.cpart = cp;
+            d.lrow = lr;
+            d.lcol = lc;
+            tiles += (int64_t)d.ta * d.tb;
+            rp += (int64_t)d.tb * d.na;
+            cp += (int64_t)d.ta * d.nb;
+            lr += d.na;
+            lc += d.nb;
+        }
+    };
+    int64_t xt, xrp, xcp, xr, xc, ft, frp, fcp, frr, fcc;
+    number(ex, mt::TILE, xt, xrp, xcp, xr, xc);
+    number(fl, mt::FTILE, ft, frp, fcp, frr, fcc);
+    const size_t nex = ex.size(), nfl = fl.size();
+    HCHK(M.pairs.need((size_t)npairs + nex + nfl + 1));
+    HCHK(M.rpart.need((size_t)std::max<int64_t>(xrp, 1)));
+    HCHK(M.cpart.need((size_t)std::max<int64_t>(xcp, 1)));
+    HCHK(M.fwd.need((size_t)std::max<int64_t>(fr, 1)));
+    HCHK(M.bwd.need((size_t)std::max<int64_t>(fc, 1)));
+    HCHK(M.frpart.need((size_t)std::max<int64_t>(frp, 1)));
+    HCHK(M.fcpart.need((size_t)std::max<int64_t>(fcp, 1)));
+    HCHK(M.trow.need((size_t)std::max<int64_t>(frr, 1)));
+    HCHK(M.tcol.need((size_t)std::max<int64_t>(fcc, 1)));
+    HCHK(M.rcnt.need((size_t)std::max<int64_t>(frr, 1)));
+    HCHK(M.ccnt.need((size_t)std::max<int64_t>(fcc, 1)));
+    HCHK(M.rcand.need((size_t)std::max<int64_t>(frr, 1) * mt::CAP));
+    HCHK(M.ccand.need((size_t)std::max<int64_t>(fcc, 1) * mt::CAP));
+    HCHK(M.ovf.need((size_t)std::max<int64_t>(frr + fcc, 1)));
+    HCHK(M.ovf_n.need(1));
+    HCHK(M.oa.need((size_t)std::max<int64_t>(out, 1)));
+    HCHK(M.ob.need((size_t)std::max<int64_t>(out, 1)));
+    HCHK(M.da.need((size_t)std::max<int64_t>(out, 1)));
+    HCHK(M.db.need((size_t)std::max<int64_t>(out, 1)));
+    HCHK(M.counts.need((size_t)std::max(npairs, 1)));
+    HCHK(M.prefix.need((size_t)std::max(npairs, 1)));
+    if (frr + fcc >= (int64_t)0x7fffffff) return fail(MVS_ERR_ARG, std::string(who) + ": batch too large");
+    mt::PairDesc* d_all = M.pairs.p;
+    mt::PairDesc* d_ex = M.pairs.p + npairs;
+    mt::PairDesc* d_fl = d_ex + nex;
+    hipStream_t st = c->stream;
+    if (npairs > 0)
+        HCHK(hipMemcpyAsync(d_all, all.data(), sizeof(mt::PairDesc) * (size_t)npairs, hipMemcpyHostToDevice, st));
+    if (nex > 0) HCHK(hipMemcpyAsync(d_ex, ex.data(), sizeof(mt::PairDesc) * nex, hipMemcpyHostToDevice, st));
+    if (nfl > 0) HCHK(hipMemcpyAsync(d_fl, fl.data(), sizeof(mt::PairDesc) * nfl, hipMemcpyHostToDevice, st));
+    mt::FiltBufs fb = {M.frpart.p, M.fcpart.p, M.trow.p, M.tcol.p, M.rcnt.p,
+                       M.ccnt.p,   M.rcand.p,  M.ccand.p, M.ovf.p, M.ovf_n.p};
+    HCHK(hipEventRecord(c->ev[0], st));
+    HCHK(mt::launch_dist_tiles(S.desc.p, d_ex, (int)nex, xt, M.rpart.p, M.cpart.p, st));
+    HCHK(mt::launch_merge_partials(d_ex, (int)nex, xr, xc, M.rpart.p, M.cpart.p, M.fwd.p, M.bwd.p, st));
+    HCHK(hipEventRecord(c->ev[1], st));
+    HCHK(mt::launch_filtered(S.desc.p, S.norm.p, S.hi.p, S.lo.p, d_fl, (int)nfl, ft, frr, fcc, fb, M.fwd.p, M.bwd.p,
+                             st, c->ev + 2));
+    if (ft <= 0)
+        for (int e = 2; e < 5; e++) HCHK(hipEventRecord(c->ev[e], st));
+    HCHK(hipEventRecord(c->ev[5], st));
+    HCHK(mt::launch_select(d_all, npairs, M.fwd.p, M.bwd.p, (float)ratio, M.oa.p, M.ob.p, M.counts.p, st));
+    HCHK(hipEventRecord(c->ev[6], st));
+    HCHK(mt::launch_pack(d_all, npairs, M.counts.p, M.prefix.p, M.oa.p, M.ob.p, M.da.p, M.db.p, st));
+    std::vector<int64_t> cnt((size_t)npairs);
+    uint32_t novf = 0;
+    if (npairs > 0)
+        HCHK(hipMemcpyAsync(cnt.data(), M.counts.p, sizeof(int64_t) * (size_t)npairs, hipMemcpyDeviceToHost, st));
+    if (ft > 0) HCHK(hipMemcpyAsync(&novf, M.ovf_n.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HCHK(hipStreamSynchronize(st));
+    // the dense lists come back in one copy per side and are spread into
+    // the callers' slots (pair p at all[p].out)
+    int64_t nm = 0;
+    for (int32_t p = 0; p < npairs; p++) {
+        out_n[p] = cnt[(size_t)p];
+        nm += cnt[(size_t)p];
+    }
+    if (nm > 0 && (out_a || out_b)) {
+        std::vector<uint32_t> ha(out_a ? (size_t)nm : 0), hb(out_b ? (size_t)nm : 0);
+        if (out_a) HCHK(hipMemcpyAsync(ha.data(), M.da.p, sizeof(uint32_t) * (size_t)nm, hipMemcpyDeviceToHost, st));
+        if (out_b) HCHK(hipMemcpyAsync(hb.data(), M.db.p, sizeof(uint32_t) * (size_t)nm, hipMemcpyDeviceToHost, st));
+        HCHK(hipStreamSynchronize(st));
+        int64_t q = 0;
+        for (int32_t p = 0; p < npairs; p++) {
+            const int64_t n = cnt[(size_t)p], o = all[(size_t)p].out;
+            if (n > 0 && out_a) memcpy(out_a + o, ha.data() + q, sizeof(uint32_t) * (size_t)n);
+            if (n > 0 && out_b) memcpy(out_b + o, hb.data() + q, sizeof(uint32_t) * (size_t)n);
+            q += n;
+        }
+    }
+    float t[6] = {0, 0, 0, 0, 0, 0};
+    for (int e = 0; e < 6; e++) (void)hipEventElapsedTime(&t[e], c->ev[e], c->ev[e + 1]);
+    mvs_match_stats_t& R = M.stats;
+    memset(&R, 0, sizeof(R));
+    R.pairs = npairs;
+    R.filtered_pairs = (int64_t)nfl;
+    for (const auto& q : all) R.distances += (int64_t)q.na * q.nb;
+    for (int32_t p = 0; p < npairs; p++) R.matches += cnt[(size_t)p];
+    R.overflow_items = novf;
+    R.exact_ms = t[0];
+    R.pass1_ms = t[1];
+    R.threshold_ms = t[2];
+    R.pass2_ms = t[3];
+    R.recheck_ms = t[4];
+    R.select_ms = t[5];
+    return MVS_OK;
+}
+
+extern "C" {
+
+int mvs_upload_descriptors(mvs_ctx* c, const float* desc, const int32_t* counts, int32_t nframes) {
+    if (!c || nframes < 0 || (nframes > 0 && !counts)) return fail(MVS_ERR_ARG, "mvs_upload_descriptors: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int r = set_device(c)) return r;
+    try {
+        return match_prepare(c, c->match.frames, desc, counts, nframes, "mvs_upload_descriptors");
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_upload_descriptors: ") + e.what());
+    }
+}
+
+int mvs_match_frames(mvs_ctx* c, const int32_t* pairs, int32_t npairs, double ratio, uint32_t* out_a,
+                     uint32_t* out_b, int64_t* out_n) {
+    if (!c || npairs < 0 || (npairs > 0 && (!pairs || !out_n)))
+        return fail(MVS_ERR_ARG, "mvs_match_frames: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int r = set_device(c)) return r;
+    try {
+        return match_batch(c, c->match.frames, pairs, npairs, ratio, out_a, out_b, out_n, "mvs_match_frames");
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_match_frames: ") + e.what());
+    }
+}
+
+int mvs_match_pair(mvs_ctx* c, const float* desc_a, int32_t na, const float* desc_b, int32_t nb, double ratio,
+                   uint32_t* out_a, uint32_t* out_b, int64_t* out_n) {
+    if (!c || na < 0 || nb < 0 || (na > 0 && !desc_a) || (nb > 0 && !desc_b) || !out_n)
+        return fail(MVS_ERR_ARG, "mvs_match_pair: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int r = set_device(c)) return r;
+    try {
+        // the two sets side by side in one host buffer
+        std::vector<float> both((size_t)(na + nb) * mt::DIM);
+        if (na > 0) memcpy(both.data(), desc_a, sizeof(float) * mt::DIM * (size_t)na);
+        if (nb > 0) memcpy(both.data() + (size_t)na * mt::DIM, desc_b, sizeof(float) * mt::DIM * (size_t)nb);
+        const int32_t counts[2] = {na, nb};
+        if (int r = match_prepare(c, c->match.tmp, both.data(), counts, 2, "mvs_match_pair")) return r;
+        const int32_t pr[2] = {0, 1};
+        return match_batch(c, c->match.tmp, pr, 1, ratio, out_a, out_b, out_n, "mvs_match_pair");
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_match_pair: ") + e.what());
+    }
+}
+
+int mvs_match_set_mode(mvs_ctx* c, int32_t mode) {
+    if (!c || mode < 0 || mode > 1) return fail(MVS_ERR_ARG, "mvs_match_set_mode: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->match.mode = mode;
+    return MVS_OK;
+}
+
+int mvs_get_match_stats(mvs_ctx* c, mvs_match_stats_t* out) {
+    if (!c || !out) return fail(MVS_ERR_ARG, "mvs_get_match_stats: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    *out = c->match.stats;
+    return MVS_OK;
+}
+
+int mvs_debug_match_bound(mvs_ctx* c, const float* desc_a, int32_t na, const float* desc_b, int32_t nb,
+                          float* max_rel) {
+    if (!c || na <= 0 || nb <= 0 || !desc_a || !desc_b || !max_rel)
+        return fail(MVS_ERR_ARG, "mvs_debug_match_bound: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (int r = set_device(c)) return r;
+    MatchState& M = c->match;
+    try {
+        std::vector<float> both((size_t)(na + nb) * mt::DIM);
+        memcpy(both.data(), desc_a, sizeof(float) * mt::DIM * (size_t)na);
+        memcpy(both.data() + (size_t)na * mt::DIM, desc_b, sizeof(float) * mt::DIM * (size_t)nb);
+        const int32_t counts[2] = {na, nb};
+        if (int r = match_prepare(c, M.tmp, both.data(), counts, 2, "mvs_debug_match_bound")) return r;
+        if (!M.tmp.safe[0] || !M.tmp.safe[1])
+            return fail(MVS_ERR_ARG, "mvs_debug_match_bound: components outside {0} U [2^-40, 2^40]");
+        mt::PairDesc d = {};
+        d.a = 0;
+        d.b = na;
+        d.na = na;
+        d.nb = nb;
+        d.ta = (na + mt::FTILE - 1) / mt::FTILE;
+        d.tb = (nb + mt::FTILE - 1) / mt::FTILE;
+        HCHK(M.pairs.need(1));
+        HCHK(M.ovf_n.need(1));
+        HCHK(hipMemcpyAsync(M.pairs.p, &d, sizeof(d), hipMemcpyHostToDevice, c->stream));
+        HCHK(hipMemsetAsync(M.ovf_n.p, 0, sizeof(uint32_t), c->stream));
+        mt::FiltBufs fb = {};
+        fb.ovf_n = M.ovf_n.p;
+        HCHK(mt::launch_filter_check(M.tmp.desc.p, M.tmp.norm.p, M.tmp.hi.p, M.tmp.lo.p, M.pairs.p, 1,
+                                     (int64_t)d.ta * d.tb, fb, c->stream));
+        uint32_t bitsv = 0;
+        HCHK(hipMemcpyAsync(&bitsv, M.ovf_n.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HCHK(hipStreamSynchronize(c->stream));
+        memcpy(max_rel, &bitsv, sizeof(float));
+        return MVS_OK;
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_debug_match_bound: ") + e.what());
+    }
+}
+
+}  // extern "C"

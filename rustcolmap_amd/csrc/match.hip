@@ -1,0 +1,861 @@
+// match.hip -- brute-force descriptor matching of image pairs on the device
+// (sfm-lite/src/matching.rs:22-98 match_pair, called per job pair at
+// sfm-lite/src/lib.rs:221-224).
+//
+// Reference semantics kept bit for bit:
+//   * distance s(i, j) = sum over k = 0..127 of (a_k - b_k)^2, each difference,
+//     square and running sum rounded to f32, in k order (no FMA);
+//   * per descriptor the best (smallest s, first index on ties) and the second
+//     best (the second order statistic of the multiset of s, +inf if absent);
+//     NaN distances are never taken;
+//   * a match (i, j) needs j = best(i), i = best(j), finite second(i) and
+//     second(j), best(i) <= 0.9, best(i) <= ratio * second(i) and
+//     best(j) <= ratio * second(j) (ratio in f32); matches are listed in
+//     increasing i.
+//
+// The reference runs two passes (a against b, b against a) that compute the
+// same distance twice (the square hides the sign of a_k - b_k).  Here each
+// distance is computed once in a 64 x 64 tile and folded into both the row
+// (a-side) and the column (b-side) statistics.  The per-tile statistics are
+// merged with an order-free rule: best = lexicographic min of (s, index),
+// second = min(max(best1, best2), second1, second2) -- the second order
+// statistic of the union -- so tiles and lanes merge in any order and still
+// give the sequential scan's answer.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "match_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace mt {
+
+constexpr int KC = 64;   // descriptor components staged per chunk
+
+__device__ __forceinline__ void push(float v, uint32_t j, float& b, uint32_t& bi, float& s) {
+    if (v < b) {
+        s = b;
+        b = v;
+        bi = j;
+    } else if (v < s) {
+        s = v;
+    }
+}
+
+__device__ __forceinline__ void merge(float& b, uint32_t& bi, float& s, float b2, uint32_t i2, float s2) {
+    const float sec = fminf(fmaxf(b, b2), fminf(s, s2));
+    if (b2 < b || (b2 == b && i2 < bi)) {
+        b = b2;
+        bi = i2;
+    }
+    s = sec;
+}
+
+__device__ __forceinline__ void merge_lanes(float& b, uint32_t& bi, float& s, int m) {
+    const float b2 = __shfl_xor(b, m);
+    const uint32_t i2 = (uint32_t)__shfl_xor((int)bi, m);
+    const float s2 = __shfl_xor(s, m);
+    merge(b, bi, s, b2, i2, s2);
+}
+
+// Flattened tile t -> pair: the last pair whose tile0 <= t.
+__device__ __forceinline__ int find_pair(const PairDesc* pairs, int npairs, int64_t t) {
+    int lo = 0, hi = npairs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pairs[mid].tile0 <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Stage KC components of 64 descriptors as [k][d] (d fastest): lane d reads
+// 16 bytes of descriptor d, so the LDS writes of a wave hit 64 consecutive
+// words.  Rows past the set are zero (their distances are masked later).
+__device__ __forceinline__ void stage(float (*s)[TILE], const float* __restrict__ desc, int64_t row0, int n, int kc) {
+    const int d = threadIdx.x & 63;
+    const bool ok = d < n;
+    const float4* src = reinterpret_cast<const float4*>(desc + (row0 + (ok ? d : 0)) * DIM + kc);
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+        const int q = (threadIdx.x >> 6) + 4 * it;   // float4 index within the chunk
+        float4 v = ok ? src[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        s[q * 4 + 0][d] = v.x;
+        s[q * 4 + 1][d] = v.y;
+        s[q * 4 + 2][d] = v.z;
+        s[q * 4 + 3][d] = v.w;
+    }
+}
+
+// 256 threads; thread (ty, tx) = (tid / 16, tid % 16) owns rows 4ty..4ty+3 and
+// columns 4tx..4tx+3 of the tile.  The logical tile index is spread so that
+// consecutive tiles of one pair run on the same XCD (blocks go round-robin
+// over the 8 XCDs, each with its own L2).
+__global__ __launch_bounds__(256) void dist_tile_kernel(const float* __restrict__ desc,
+                                                        const PairDesc* __restrict__ pairs, int npairs,
+                                                        int64_t ntiles, int64_t per_xcd, Best2* __restrict__ rpart,
+                                                        Best2* __restrict__ cpart) {
+    const int64_t t = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (t >= ntiles) return;
+    __shared__ int s_pair;
+    if (threadIdx.x == 0) s_pair = find_pair(pairs, npairs, t);
+    __syncthreads();
+    const PairDesc P = pairs[s_pair];
+    const int64_t lt = t - P.tile0;
+    const int rt = (int)(lt / P.tb), ct = (int)(lt - (int64_t)rt * P.tb);
+    const int i0 = rt * TILE, j0 = ct * TILE;
+    const int nra = min(TILE, P.na - i0), nrb = min(TILE, P.nb - j0);
+
+    __shared__ float sa[KC][TILE];
+    __shared__ float sb[KC][TILE];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    float acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = 0.0f;
+
+    for (int kc = 0; kc < DIM; kc += KC) {
+        if (kc) __syncthreads();
+        stage(sa, desc, P.a + i0, nra, kc);
+        stage(sb, desc, P.b + j0, nrb, kc);
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < KC; ++k) {
+            const float4 av = *reinterpret_cast<const float4*>(&sa[k][ty * 4]);
+            const float4 bv = *reinterpret_cast<const float4*>(&sb[k][tx * 4]);
+            const float a_[4] = {av.x, av.y, av.z, av.w};
+            const float b_[4] = {bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float d = a_[r] - b_[c];
+                    const float sq = d * d;
+                    acc[r][c] = acc[r][c] + sq;
+                }
+        }
+    }
+
+    // rows (a side): scan own 4 columns in j order, merge over the 16 lanes
+    // of the row group, lane tx == 0 writes.
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        float b = INFINITY, s = INFINITY;
+        uint32_t bi = NONE;
+        const int i = ty * 4 + r;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int j = tx * 4 + c;
+            if (j < nrb && i < nra) push(acc[r][c], (uint32_t)(j0 + j), b, bi, s);
+        }
+        merge_lanes(b, bi, s, 1);
+        merge_lanes(b, bi, s, 2);
+        merge_lanes(b, bi, s, 4);
+        merge_lanes(b, bi, s, 8);
+        if (tx == 0 && i < nra) {
+            Best2 o;
+            o.b = b;
+            o.i = bi;
+            o.s = s;
+            o.pad = 0;
+            rpart[P.rpart + (int64_t)ct * P.na + i0 + i] = o;
+        }
+    }
+
+    // columns (b side): scan own 4 rows in i order, merge over the 4 row
+    // groups of the wave, then over the 4 waves through LDS.
+    __syncthreads();   // sa / sb are reused below
+    float* sred = &sa[0][0];   // [4 waves][64 columns][3]
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float b = INFINITY, s = INFINITY;
+        uint32_t bi = NONE;
+        const int j = tx * 4 + c;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = ty * 4 + r;
+            if (j < nrb && i < nra) push(acc[r][c], (uint32_t)(i0 + i), b, bi, s);
+        }
+        merge_lanes(b, bi, s, 16);
+        merge_lanes(b, bi, s, 32);
+        if ((threadIdx.x & 63) < 16) {
+            float* o = sred + (w * TILE + j) * 3;
+            o[0] = b;
+            o[1] = __uint_as_float(bi);
+            o[2] = s;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < TILE && (int)threadIdx.x < nrb) {
+        const int j = threadIdx.x;
+        float b = sred[j * 3 + 0], s = sred[j * 3 + 2];
+        uint32_t bi = __float_as_uint(sred[j * 3 + 1]);
+#pragma unroll
+        for (int ww = 1; ww < 4; ++ww) {
+            const float* o = sred + (ww * TILE + j) * 3;
+            merge(b, bi, s, o[0], __float_as_uint(o[1]), o[2]);
+        }
+        Best2 o;
+        o.b = b;
+        o.i = bi;
+        o.s = s;
+        o.pad = 0;
+        cpart[P.cpart + (int64_t)rt * P.nb + j0 + j] = o;
+    }
+}
+
+// Launch-local row item q in [0, rows_total): pair by lrow; columns by lcol.
+__device__ __forceinline__ int find_pair_by(const PairDesc* pairs, int npairs, int64_t q, bool cols) {
+    int lo = 0, hi = npairs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int64_t base = cols ? pairs[mid].lcol : pairs[mid].lrow;
+        if (base <= q)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void merge_partials_kernel(const PairDesc* __restrict__ pairs, int npairs,
+                                                             int64_t rows_total, int64_t cols_total,
+                                                             const Best2* __restrict__ rpart,
+                                                             const Best2* __restrict__ cpart, Best2* __restrict__ fwd,
+                                                             Best2* __restrict__ bwd) {
+    const bool cols = blockIdx.y == 1;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (cols ? cols_total : rows_total)) return;
+    const PairDesc P = pairs[find_pair_by(pairs, npairs, q, cols)];
+    const int64_t local = q - (cols ? P.lcol : P.lrow);
+    const int nt = cols ? P.ta : P.tb;
+    const int n = cols ? P.nb : P.na;
+    const Best2* src = (cols ? cpart + P.cpart : rpart + P.rpart) + local;
+    float b = INFINITY, s = INFINITY;
+    uint32_t bi = NONE;
+    for (int t = 0; t < nt; ++t) {
+        const Best2 p = src[(int64_t)t * n];
+        merge(b, bi, s, p.b, p.i, p.s);
+    }
+    Best2 o;
+    o.b = b;
+    o.i = bi;
+    o.s = s;
+    o.pad = 0;
+    (cols ? bwd + P.fcol : fwd + P.frow)[local] = o;
+}
+
+// One block per pair: the reference's final loop over a's descriptors,
+// compacted in order with a block-wide prefix sum.
+__global__ __launch_bounds__(1024) void select_kernel(const PairDesc* __restrict__ pairs,
+                                                      const Best2* __restrict__ fwd, const Best2* __restrict__ bwd,
+                                                      float ratio, uint32_t* __restrict__ out_a,
+                                                      uint32_t* __restrict__ out_b, int64_t* __restrict__ counts) {
+    const PairDesc P = pairs[blockIdx.x];
+    __shared__ int s_wave[16];
+    __shared__ int s_base;
+    if (threadIdx.x == 0) s_base = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const bool any = P.na > 0 && P.nb > 0;
+    for (int c0 = 0; any && c0 < P.na; c0 += blockDim.x) {
+        const int i = c0 + threadIdx.x;
+        bool keep = false;
+        uint32_t jb = 0;
+        if (i < P.na) {
+            const Best2 f = fwd[P.frow + i];
+            jb = f.i;
+            if (f.i != NONE && isfinite(f.s) && !(f.b > 0.9f) && !(f.b > ratio * f.s)) {
+                const Best2 g = bwd[P.fcol + f.i];
+                keep = g.i == (uint32_t)i && isfinite(g.s) && !(g.b > ratio * g.s);
+            }
+        }
+        const uint64_t m = __ballot(keep);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) s_wave[wv] = __popcll(m);
+        __syncthreads();
+        int off = s_base;
+        for (int q = 0; q < wv; ++q) off += s_wave[q];
+        if (keep) {
+            out_a[P.out + off + before] = (uint32_t)i;
+            out_b[P.out + off + before] = jb;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int q = 0; q < nw; ++q) tot += s_wave[q];
+            s_base += tot;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counts[blockIdx.x] = any ? s_base : 0;
+}
+
+// --------------------------------------------------------------------------
+// Filtered search.
+//
+// For safe sets (every nonzero component in [2^-40, 2^40]) the distance
+// s(i, j) is first estimated from the MFMA product of the bf16 split
+// x = hi + lo (+ r, |r| <= 2^-18 |x|):
+//   A(i, j) = |a_i|^2 + |b_j|^2 - 2 (hi_a.hi_b + hi_a.lo_b + lo_a.hi_b),
+// with |s(i, j) - A(i, j)| <= EPS_REL (|a_i|^2 + |b_j|^2) =: E(i, j).  The
+// bound covers the neglected lo.lo and r terms (<= 3.1 * 2^-18 sum |a_k b_k|),
+// f32 accumulation of the 384 exact bf16 products and of the norms, the
+// final combination, and the rounding of the reference's own sum
+// (<= 130 * 2^-24 |a - b|^2); all together below 6e-5 (|a|^2 + |b|^2),
+// an eighth of EPS_REL.  The component range keeps every product and
+// partial sum a normal f32, so the relative bounds hold.
+//
+// Pass 1: per descriptor, T = the second smallest upper bound U = A + E.
+//   At least two exact distances are <= T, so the exact second-nearest
+//   distance s2 <= T.
+// Pass 2: candidates = the descriptors with lower bound L = A - E <= T.
+//   Every descriptor whose exact distance is <= s2 -- the nearest (all of
+//   them on ties) and the second nearest -- is a candidate.
+// Recheck: the exact distances of the candidates, merged with the
+//   order-free rule, give the reference's (best index, best, second).
+//   Descriptors with more than CAP candidates are scanned in full.
+// --------------------------------------------------------------------------
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint16_t bf16_rn(float f) {   // round to nearest even (finite input)
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 as_bf(us8 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// 16 threads per descriptor, 8 components each.
+__global__ __launch_bounds__(256) void prep_kernel(const float* __restrict__ desc, int64_t n,
+                                                   const int64_t* __restrict__ set_off, int nsets,
+                                                   float* __restrict__ norm, uint16_t* __restrict__ hi,
+                                                   uint16_t* __restrict__ lo, int* __restrict__ bad) {
+    const int64_t d = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int seg = threadIdx.x & 15;
+    float ss = 0.0f;
+    bool badv = false;
+    if (d < n) {
+        const float4* src = reinterpret_cast<const float4*>(desc + d * DIM + seg * 8);
+        const float4 v0 = src[0], v1 = src[1];
+        const float x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+        us8 h, l;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const float ax = fabsf(x[q]);
+            badv |= !(ax == 0.0f || (ax >= 0x1p-40f && ax <= 0x1p40f));
+            const uint16_t hb = bf16_rn(x[q]);
+            const float hf = __uint_as_float((uint32_t)hb << 16);
+            h[q] = hb;
+            l[q] = bf16_rn(x[q] - hf);   // x - hf is exact
+            ss = ss + x[q] * x[q];
+        }
+        *reinterpret_cast<us8*>(hi + d * DIM + seg * 8) = h;
+        *reinterpret_cast<us8*>(lo + d * DIM + seg * 8) = l;
+    }
+    ss += __shfl_xor(ss, 1);
+    ss += __shfl_xor(ss, 2);
+    ss += __shfl_xor(ss, 4);
+    ss += __shfl_xor(ss, 8);
+    if (d < n && seg == 0) norm[d] = ss;
+    if (d < n && badv) {
+        int lo_s = 0, hi_s = nsets - 1;   // the last set starting at or before d
+        while (lo_s < hi_s) {
+            const int mid = (lo_s + hi_s + 1) >> 1;
+            if (set_off[mid] <= d)
+                lo_s = mid;
+            else
+                hi_s = mid - 1;
+        }
+        atomicOr(&bad[lo_s], 1);
+    }
+}
+
+__device__ __forceinline__ void two_min(float& u1, float& u2, float v) {
+    u2 = fminf(u2, fmaxf(u1, v));
+    u1 = fminf(u1, v);
+}
+
+__device__ __forceinline__ void two_min_merge(float& u1, float& u2, float v1, float v2) {
+    u2 = fminf(fmaxf(u1, v1), fminf(u2, v2));
+    u1 = fminf(u1, v1);
+}
+
+constexpr int LROW = 40;   // LDS row stride (u16): 32 components + 8 pad
+
+// PASS 1: two smallest upper bounds per row / column of a 128 x 128 tile.
+// PASS 2: candidate lists against the thresholds.
+// PASS 3: (check) max over the tile of |exact - estimate| / (|a|^2 + |b|^2)
+//         into *fb.ovf_n as float bits.
+// 4 waves in a 2 x 2 grid, each 64 x 64 = 4 x 4 MFMA 16x16x32 fragments.
+template <int PASS>
+__global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict__ desc,
+                                                        const float* __restrict__ norm,
+                                                        const uint16_t* __restrict__ hi,
+                                                        const uint16_t* __restrict__ lo,
+                                                        const PairDesc* __restrict__ pairs, int npairs,
+                                                        int64_t ntiles, int64_t per_xcd, FiltBufs fb) {
+    const int64_t t = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (t >= ntiles) return;
+    __shared__ int s_pair;
+    if (threadIdx.x == 0) s_pair = find_pair(pairs, npairs, t);
+    __syncthreads();
+    const PairDesc P = pairs[s_pair];
+    const int64_t lt = t - P.tile0;
+    const int rt = (int)(lt / P.tb), ct = (int)(lt - (int64_t)rt * P.tb);
+    const int i0 = rt * FTILE, j0 = ct * FTILE;
+    const int nra = min(FTILE, P.na - i0), nrb = min(FTILE, P.nb - j0);
+
+    __shared__ __align__(16) uint16_t sAh[FTILE * LROW], sAl[FTILE * LROW], sBh[FTILE * LROW], sBl[FTILE * LROW];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
+    const int r16 = lane & 15, kq = lane >> 4;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int mf = 0; mf < 4; ++mf)
+#pragma unroll
+        for (int nf = 0; nf < 4; ++nf) acc[mf][nf] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int kc = 0; kc < DIM; kc += 32) {
+        if (kc) __syncthreads();
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            const int item = threadIdx.x + 256 * it;   // 128 rows x 4 segments of 8 components
+            const int row = item >> 2, seg = item & 3;
+            us8 vh = us8{0, 0, 0, 0, 0, 0, 0, 0}, vl = vh, wh = vh, wl = vh;
+            if (row < nra) {
+                const size_t g = (size_t)(P.a + i0 + row) * DIM + kc + seg * 8;
+                vh = *reinterpret_cast<const us8*>(hi + g);
+                vl = *reinterpret_cast<const us8*>(lo + g);
+            }
+            if (row < nrb) {
+                const size_t g = (size_t)(P.b + j0 + row) * DIM + kc + seg * 8;
+                wh = *reinterpret_cast<const us8*>(hi + g);
+                wl = *reinterpret_cast<const us8*>(lo + g);
+            }
+            *reinterpret_cast<us8*>(sAh + row * LROW + seg * 8) = vh;
+            *reinterpret_cast<us8*>(sAl + row * LROW + seg * 8) = vl;
+            *reinterpret_cast<us8*>(sBh + row * LROW + seg * 8) = wh;
+            *reinterpret_cast<us8*>(sBl + row * LROW + seg * 8) = wl;
+        }
+        __syncthreads();
+        us8 bh[4], bl[4];
+#pragma unroll
+        for (int nf = 0; nf < 4; ++nf) {
+            const int col = wc * 64 + nf * 16 + r16;
+            bh[nf] = *reinterpret_cast<const us8*>(sBh + col * LROW + kq * 8);
+            bl[nf] = *reinterpret_cast<const us8*>(sBl + col * LROW + kq * 8);
+        }
+#pragma unroll
+        for (int mf = 0; mf < 4; ++mf) {
+            const int row = wr * 64 + mf * 16 + r16;
+            const us8 ah = *reinterpret_cast<const us8*>(sAh + row * LROW + kq * 8);
+            const us8 al = *reinterpret_cast<const us8*>(sAl + row * LROW + kq * 8);
+#pragma unroll
+            for (int nf = 0; nf < 4; ++nf) {
+                acc[mf][nf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah), as_bf(bh[nf]), acc[mf][nf], 0, 0, 0);
+                acc[mf][nf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah), as_bf(bl[nf]), acc[mf][nf], 0, 0, 0);
+                acc[mf][nf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(al), as_bf(bh[nf]), acc[mf][nf], 0, 0, 0);
+            }
+        }
+    }
+
+    // element (mf, nf, i) of this lane: tile row R = wr*64 + mf*16 + kq*4 + i,
+    // tile column C = wc*64 + nf*16 + r16
+    float nrow[4][4], ncol[4];
+#pragma unroll
+    for (int mf = 0; mf < 4; ++mf)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int R = wr * 64 + mf * 16 + kq * 4 + i;
+            nrow[mf][i] = R < nra ? norm[P.a + i0 + R] : 0.0f;
+        }
+#pragma unroll
+    for (int nf = 0; nf < 4; ++nf) {
+        const int C = wc * 64 + nf * 16 + r16;
+        ncol[nf] = C < nrb ? norm[P.b + j0 + C] : 0.0f;
+    }
+
+    if constexpr (PASS == 1) {
+        const float cp = 1.0f + EPS_REL;
+        float ru1[4][4], ru2[4][4], cu1[4], cu2[4];
+#pragma unroll
+        for (int mf = 0; mf < 4; ++mf)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) ru1[mf][i] = ru2[mf][i] = INFINITY;
+#pragma unroll
+        for (int nf = 0; nf < 4; ++nf) cu1[nf] = cu2[nf] = INFINITY;
+#pragma unroll
+        for (int mf = 0; mf < 4; ++mf)
+#pragma unroll
+            for (int nf = 0; nf < 4; ++nf)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int R = wr * 64 + mf * 16 + kq * 4 + i, C = wc * 64 + nf * 16 + r16;
+                    float U = __builtin_fmaf(-2.0f, acc[mf][nf][i], cp * (nrow[mf][i] + ncol[nf]));
+                    if (R >= nra || C >= nrb) U = INFINITY;
+                    two_min(ru1[mf][i], ru2[mf][i], U);
+                    two_min(cu1[nf], cu2[nf], U);
+                }
+        // rows: over the 16 lanes of a kq group, then the two column waves
+#pragma unroll
+        for (int mf = 0; mf < 4; ++mf)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int m = 1; m < 16; m <<= 1)
+                    two_min_merge(ru1[mf][i], ru2[mf][i], __shfl_xor(ru1[mf][i], m), __shfl_xor(ru2[mf][i], m));
+        // columns: over the 4 kq groups, then the two row waves
+#pragma unroll
+        for (int nf = 0; nf < 4; ++nf)
+#pragma unroll
+            for (int m = 16; m < 64; m <<= 1)
+                two_min_merge(cu1[nf], cu2[nf], __shfl_xor(cu1[nf], m), __shfl_xor(cu2[nf], m));
+        __syncthreads();   // the staging buffers are reused
+        float2* srow = reinterpret_cast<float2*>(sAh);   // [2 column waves][128 rows]
+        float2* scol = reinterpret_cast<float2*>(sBh);   // [2 row waves][128 columns]
+        if (r16 == 0) {
+#pragma unroll
+            for (int mf = 0; mf < 4; ++mf)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int R = wr * 64 + mf * 16 + kq * 4 + i;
+                    srow[wc * FTILE + R] = make_float2(ru1[mf][i], ru2[mf][i]);
+                }
+        }
+        if (kq == 0) {
+#pragma unroll
+            for (int nf = 0; nf < 4; ++nf) {
+                const int C = wc * 64 + nf * 16 + r16;
+                scol[wr * FTILE + C] = make_float2(cu1[nf], cu2[nf]);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < FTILE) {
+            const int R = threadIdx.x;
+            if (R < nra) {
+                float2 a = srow[R], b = srow[FTILE + R];
+                two_min_merge(a.x, a.y, b.x, b.y);
+                fb.rpart[P.rpart + (int64_t)ct * P.na + i0 + R] = a;
+            }
+        } else {
+            const int C = threadIdx.x - FTILE;
+            if (C < nrb) {
+                float2 a = scol[C], b = scol[FTILE + C];
+                two_min_merge(a.x, a.y, b.x, b.y);
+                fb.cpart[P.cpart + (int64_t)rt * P.nb + j0 + C] = a;
+            }
+        }
+    } else if constexpr (PASS == 2) {
+        const float cm = 1.0f - EPS_REL;
+        float trow[4][4], tcol[4];
+#pragma unroll
+        for (int mf = 0; mf < 4; ++mf)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int R = wr * 64 + mf * 16 + kq * 4 + i;
+                trow[mf][i] = R < nra ? fb.trow[P.lrow + i0 + R] : -INFINITY;
+            }
+#pragma unroll
+        for (int nf = 0; nf < 4; ++nf) {
+            const int C = wc * 64 + nf * 16 + r16;
+            tcol[nf] = C < nrb ? fb.tcol[P.lcol + j0 + C] : -INFINITY;
+        }
+#pragma unroll
+        for (int mf = 0; mf < 4; ++mf)
+#pragma unroll
+            for (int nf = 0; nf < 4; ++nf)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int R = wr * 64 + mf * 16 + kq * 4 + i, C = wc * 64 + nf * 16 + r16;
+                    const float L = __builtin_fmaf(-2.0f, acc[mf][nf][i], cm * (nrow[mf][i] + ncol[nf]));
+                    if (C < nrb && L <= trow[mf][i]) {
+                        const uint32_t q = (uint32_t)(P.lrow + i0 + R);
+                        const uint32_t k = atomicAdd(&fb.rcnt[q], 1u);
+                        if (k < (uint32_t)CAP)
+                            fb.rcand[(size_t)q * CAP + k] = (uint32_t)(j0 + C);
+                        else if (k == (uint32_t)CAP)
+                            fb.ovf[atomicAdd(fb.ovf_n, 1u)] = q;
+                    }
+                    if (R < nra && L <= tcol[nf]) {
+                        const uint32_t q = (uint32_t)(P.lcol + j0 + C);
+                        const uint32_t k = atomicAdd(&fb.ccnt[q], 1u);
+                        if (k < (uint32_t)CAP)
+                            fb.ccand[(size_t)q * CAP + k] = (uint32_t)(i0 + R);
+                        else if (k == (uint32_t)CAP)
+                            fb.ovf[atomicAdd(fb.ovf_n, 1u)] = 0x80000000u | q;
+                    }
+                }
+    } else {
+        float worst = 0.0f;
+#pragma unroll
+        for (int mf = 0; mf < 4; ++mf)
+#pragma unroll
+            for (int nf = 0; nf < 4; ++nf)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int R = wr * 64 + mf * 16 + kq * 4 + i, C = wc * 64 + nf * 16 + r16;
+                    if (R >= nra || C >= nrb) continue;
+                    const float* x = desc + (P.a + i0 + R) * DIM;
+                    const float* y = desc + (P.b + j0 + C) * DIM;
+                    float s = 0.0f;
+                    for (int k = 0; k < DIM; ++k) {
+                        const float d = x[k] - y[k];
+                        s = s + d * d;
+                    }
+                    const float t2 = nrow[mf][i] + ncol[nf];
+                    const float A = t2 - 2.0f * acc[mf][nf][i];
+                    const float e = t2 > 0.0f ? fabsf(s - A) / t2 : fabsf(s - A);
+                    worst = fmaxf(worst, e);
+                }
+        atomicMax(fb.ovf_n, __float_as_uint(worst));   // non-negative floats order as their bits
+    }
+}
+
+// Launch-local row / column q: T = second smallest upper bound over its
+// tiles; clears its candidate count.
+__global__ __launch_bounds__(256) void filt_threshold_kernel(const PairDesc* __restrict__ pairs, int npairs,
+                                                             int64_t rows_total, int64_t cols_total, FiltBufs fb) {
+    const bool cols = blockIdx.y == 1;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0 && !cols) *fb.ovf_n = 0u;
+    if (q >= (cols ? cols_total : rows_total)) return;
+    const PairDesc P = pairs[find_pair_by(pairs, npairs, q, cols)];
+    const int64_t local = q - (cols ? P.lcol : P.lrow);
+    const int nt = cols ? P.ta : P.tb;
+    const int n = cols ? P.nb : P.na;
+    const float2* src = (cols ? fb.cpart + P.cpart : fb.rpart + P.rpart) + local;
+    float u1 = INFINITY, u2 = INFINITY;
+    for (int t = 0; t < nt; ++t) {
+        const float2 p = src[(int64_t)t * n];
+        two_min_merge(u1, u2, p.x, p.y);
+    }
+    (cols ? fb.tcol : fb.trow)[q] = u2;
+    (cols ? fb.ccnt : fb.rcnt)[q] = 0u;
+}
+
+// Exact distance of a-side descriptor x and b-side descriptor y, as the
+// reference sums it.
+__device__ __forceinline__ float exact_dist(const float* __restrict__ x, const float* __restrict__ y) {
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const float4* y4 = reinterpret_cast<const float4*>(y);
+    float s = 0.0f;
+#pragma unroll 4
+    for (int k = 0; k < DIM / 4; ++k) {
+        const float4 a = x4[k], b = y4[k];
+        float d = a.x - b.x;
+        s = s + d * d;
+        d = a.y - b.y;
+        s = s + d * d;
+        d = a.z - b.z;
+        s = s + d * d;
+        d = a.w - b.w;
+        s = s + d * d;
+    }
+    return s;
+}
+
+// 8 lanes per launch-local row or column: lane s rechecks candidate s.
+__global__ __launch_bounds__(256) void filt_recheck_kernel(const float* __restrict__ desc,
+                                                           const PairDesc* __restrict__ pairs, int npairs,
+                                                           int64_t rows_total, int64_t cols_total, FiltBufs fb,
+                                                           Best2* __restrict__ fwd, Best2* __restrict__ bwd) {
+    static_assert(CAP == 8, "one lane per candidate slot");
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t item = g >> 3;
+    const int s = (int)(g & 7);
+    const bool valid = item < rows_total + cols_total;
+    const bool row = item < rows_total;
+    const int64_t q = row ? item : item - rows_total;
+    float b = INFINITY, sec = INFINITY;
+    uint32_t bi = NONE;
+    uint32_t cnt = 0;
+    PairDesc P;
+    int64_t local = 0;
+    if (valid) {
+        P = pairs[find_pair_by(pairs, npairs, q, !row)];
+        local = q - (row ? P.lrow : P.lcol);
+        cnt = row ? fb.rcnt[q] : fb.ccnt[q];
+        if (cnt <= (uint32_t)CAP && (uint32_t)s < cnt) {
+            const uint32_t other = row ? fb.rcand[(size_t)q * CAP + s] : fb.ccand[(size_t)q * CAP + s];
+            const float* x = desc + (row ? P.a + local : P.a + other) * DIM;
+            const float* y = desc + (row ? P.b + other : P.b + local) * DIM;
+            b = exact_dist(x, y);
+            bi = other;
+        }
+    }
+    merge_lanes(b, bi, sec, 1);
+    merge_lanes(b, bi, sec, 2);
+    merge_lanes(b, bi, sec, 4);
+    if (valid && s == 0 && cnt <= (uint32_t)CAP) {
+        Best2 o;
+        o.b = b;
+        o.i = bi;
+        o.s = sec;
+        o.pad = 0;
+        (row ? fwd + P.frow : bwd + P.fcol)[local] = o;
+    }
+}
+
+// One wave per overflowed row / column: full exact scan.
+__global__ __launch_bounds__(256) void filt_overflow_kernel(const float* __restrict__ desc,
+                                                            const PairDesc* __restrict__ pairs, int npairs,
+                                                            FiltBufs fb, Best2* __restrict__ fwd,
+                                                            Best2* __restrict__ bwd) {
+    const uint32_t n = *fb.ovf_n;
+    const int lane = threadIdx.x & 63;
+    const uint32_t nw = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t o = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); o < n; o += nw) {
+        const uint32_t it = fb.ovf[o];
+        const bool row = (it >> 31) == 0;
+        const int64_t q = it & 0x7fffffffu;
+        const PairDesc P = pairs[find_pair_by(pairs, npairs, q, !row)];
+        const int64_t local = q - (row ? P.lrow : P.lcol);
+        const int n_other = row ? P.nb : P.na;
+        float b = INFINITY, sec = INFINITY;
+        uint32_t bi = NONE;
+        for (int j = lane; j < n_other; j += 64) {
+            const float* x = desc + (row ? P.a + local : P.a + j) * DIM;
+            const float* y = desc + (row ? P.b + j : P.b + local) * DIM;
+            push(exact_dist(x, y), (uint32_t)j, b, bi, sec);
+        }
+        for (int m = 1; m < 64; m <<= 1) merge_lanes(b, bi, sec, m);
+        if (lane == 0) {
+            Best2 ob;
+            ob.b = b;
+            ob.i = bi;
+            ob.s = sec;
+            ob.pad = 0;
+            (row ? fwd + P.frow : bwd + P.fcol)[local] = ob;
+        }
+    }
+}
+
+// Exclusive prefix of the per-pair match counts (one block), then every
+// pair's matches gathered from its slots into one dense list per side.
+__global__ __launch_bounds__(1024) void count_scan_kernel(const int64_t* __restrict__ counts, int npairs,
+                                                          int64_t* __restrict__ prefix) {
+    __shared__ int64_t s_part[1024];
+    __shared__ int64_t s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < npairs; c0 += 1024) {
+        const int p = c0 + threadIdx.x;
+        const int64_t v = p < npairs ? counts[p] : 0;
+        s_part[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {   // Hillis-Steele inclusive scan
+            const int64_t add = threadIdx.x >= (unsigned)off ? s_part[threadIdx.x - off] : 0;
+            __syncthreads();
+            s_part[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (p < npairs) prefix[p] = s_carry + s_part[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) s_carry += s_part[1023];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void pack_kernel(const PairDesc* __restrict__ pairs,
+                                                   const int64_t* __restrict__ counts,
+                                                   const int64_t* __restrict__ prefix,
+                                                   const uint32_t* __restrict__ oa, const uint32_t* __restrict__ ob,
+                                                   uint32_t* __restrict__ da, uint32_t* __restrict__ db) {
+    const int p = blockIdx.x;
+    const int64_t n = counts[p], src = pairs[p].out, dst = prefix[p];
+    for (int64_t k = threadIdx.x; k < n; k += blockDim.x) {
+        da[dst + k] = oa[src + k];
+        db[dst + k] = ob[src + k];
+    }
+}
+
+hipError_t launch_pack(const PairDesc* pairs, int npairs, const int64_t* counts, int64_t* prefix,
+                       const uint32_t* oa, const uint32_t* ob, uint32_t* da, uint32_t* db, hipStream_t st) {
+    if (npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(count_scan_kernel, dim3(1), dim3(1024), 0, st, counts, npairs, prefix);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(pack_kernel, dim3(npairs), dim3(256), 0, st, pairs, counts, prefix, oa, ob, da, db);
+    return hipGetLastError();
+}
+
+hipError_t launch_dist_tiles(const float* desc, const PairDesc* pairs, int npairs, int64_t ntiles, Best2* rpart,
+                             Best2* cpart, hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    const int64_t per_xcd = (ntiles + 7) / 8;
+    hipLaunchKernelGGL(dist_tile_kernel, dim3((unsigned)(per_xcd * 8)), dim3(256), 0, st, desc, pairs, npairs, ntiles,
+                       per_xcd, rpart, cpart);
+    return hipGetLastError();
+}
+
+hipError_t launch_merge_partials(const PairDesc* pairs, int npairs, int64_t rows_total, int64_t cols_total,
+                                 const Best2* rpart, const Best2* cpart, Best2* fwd, Best2* bwd, hipStream_t st) {
+    const int64_t n = rows_total > cols_total ? rows_total : cols_total;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(merge_partials_kernel, dim3((unsigned)((n + 255) / 256), 2), dim3(256), 0, st, pairs, npairs,
+                       rows_total, cols_total, rpart, cpart, fwd, bwd);
+    return hipGetLastError();
+}
+
+hipError_t launch_select(const PairDesc* pairs, int npairs, const Best2* fwd, const Best2* bwd, float ratio,
+                         uint32_t* out_a, uint32_t* out_b, int64_t* counts, hipStream_t st) {
+    if (npairs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(select_kernel, dim3(npairs), dim3(1024), 0, st, pairs, fwd, bwd, ratio, out_a, out_b, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_prep(const float* desc, int64_t n, const int64_t* set_off, int nsets, float* norm, uint16_t* hi,
+                       uint16_t* lo, int* bad, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(prep_kernel, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, st, desc, n, set_off, nsets, norm,
+                       hi, lo, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_filtered(const float* desc, const float* norm, const uint16_t* hi, const uint16_t* lo,
+                           const PairDesc* pairs, int npairs, int64_t ntiles, int64_t rows_total, int64_t cols_total,
+                           const FiltBufs& fb, Best2* fwd, Best2* bwd, hipStream_t st, hipEvent_t* ev) {
+    if (ntiles <= 0) return hipSuccess;
+    const int64_t per_xcd = (ntiles + 7) / 8;
+    const dim3 tgrid((unsigned)(per_xcd * 8));
+    hipError_t e;
+    hipLaunchKernelGGL(filt_tile_kernel<1>, tgrid, dim3(256), 0, st, desc, norm, hi, lo, pairs, npairs, ntiles,
+                       per_xcd, fb);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[0], st);
+    const int64_t n = rows_total > cols_total ? rows_total : cols_total;
+    hipLaunchKernelGGL(filt_threshold_kernel, dim3((unsigned)((n + 255) / 256), 2), dim3(256), 0, st, pairs, npairs,
+                       rows_total, cols_total, fb);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[1], st);
+    hipLaunchKernelGGL(filt_tile_kernel<2>, tgrid, dim3(256), 0, st, desc, norm, hi, lo, pairs, npairs, ntiles,
+                       per_xcd, fb);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[2], st);
+    const int64_t items = (rows_total + cols_total) * 8;
+    hipLaunchKernelGGL(filt_recheck_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, st, desc, pairs,
+                       npairs, rows_total, cols_total, fb, fwd, bwd);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(filt_overflow_kernel, dim3(256), dim3(256), 0, st, desc, pairs, npairs, fb, fwd, bwd);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_check(const float* desc, const float* norm, const uint16_t* hi, const uint16_t* lo,
+                               const PairDesc* pairs, int npairs, int64_t ntiles, const FiltBufs& fb,
+                               hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    const int64_t per_xcd = (ntiles + 7) / 8;
+    hipLaunchKernelGGL(filt_tile_kernel<3>, dim3((unsigned)(per_xcd * 8)), dim3(256), 0, st, desc, norm, hi, lo,
+                       pairs, npairs, ntiles, per_xcd, fb);
+    return hipGetLastError();
+}
+
+}  // namespace mt
