@@ -458,7 +458,7 @@ def batched_bench(args):
 # vector f32 rate counts an FMA as two)
 VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
 MFMA_BF16_PEAK_TFLOPS = 2500.0             # dense bf16 MFMA
-MATCH_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "match_dist_tile_traffic.txt")
+MATCH_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "match_filt_tile_match20x2048_traffic.txt")
 
 
 def match_bench(args):
@@ -497,16 +497,29 @@ def match_bench(args):
     torch.cuda.synchronize()
     stages = []
     t1 = time.perf_counter()
+    call_ms = []
+    step_ms = []
+    import gc
+    gc.collect()
+    gc.disable()   # no collector pause of the interpreter inside the timed steps
     for _ in range(args.steps):
+        t2 = time.perf_counter()
         res = fm.match(pairs, 0.75)
+        step_ms.append(1000.0 * (time.perf_counter() - t2))
+        call_ms.append(fm.last_call_ms)
         stages.append(fm.stats())
     torch.cuda.synchronize()
     ms = 1000.0 * (time.perf_counter() - t1) / args.steps
+    gc.enable()
     st = stages[-1]
     n_match = sum(len(r) for r in res)
     distances = st["distances"]
     avg = {k: float(np.mean([x[k] for x in stages])) for k in ("exact_ms", "pass1_ms", "threshold_ms", "pass2_ms",
                                                                "recheck_ms", "select_ms")}
+    host = {k: round(float(np.mean([x[k] for x in stages])), 4) for k in ("host_launch_ms", "host_wait_ms",
+                                                                        "host_out_ms")}
+    host["c_call_ms"] = round(float(np.mean(call_ms)), 4)
+    host["python_step_ms"] = [round(x, 3) for x in step_ms]
     filtered = st["filtered_pairs"] == st["pairs"]
     if filtered:
         # dominant kernel: the MFMA tile kernel, launched twice per step
@@ -521,7 +534,13 @@ def match_bench(args):
         flops = distances * 384.0
         peak, bound = VALU_PEAK_TOPS, "valu"
     achieved = flops / (launch_ms / 1000.0) / 1e12
-    traffic = profiled_traffic(kname, MATCH_TRAFFIC_SUMMARY)
+    if filtered:   # mean HBM bytes of the pass-1 and pass-2 launches
+        tr = [profiled_traffic(kname + "<%d>" % k, MATCH_TRAFFIC_SUMMARY) for k in (1, 2)]
+        traffic = None if None in tr or args.match_frames != 20 or args.match_features != 2048 else \
+            ((tr[0][0] + tr[1][0]) / 2.0, tr[0][1])
+    else:
+        traffic = None   # no committed counter summary for the exact tiles
+
     log("match: %.2f ms/step, stages %s, %d matches, %d overflow items" %
         (ms, ", ".join("%s %.3f" % kv for kv in avg.items()), n_match, st["overflow_items"]))
     out = {"metric": "SfM descriptor matching, image pairs/s (%d descriptors per frame, exhaustive pairs)"
@@ -534,7 +553,7 @@ def match_bench(args):
                                      len(pairs)),
                       "search": "filtered" if filtered else "exact"},
            "matches": n_match, "overflow_items": st["overflow_items"],
-           "stage_ms": {k: round(v, 4) for k, v in avg.items()},
+           "stage_ms": {k: round(v, 4) for k, v in avg.items()}, "host_ms": host,
            "roofline": {"bound": bound, "achieved": round(achieved, 3), "peak": round(peak, 2),
                         "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
                         "traffic": None if traffic is None else int(traffic[0]),

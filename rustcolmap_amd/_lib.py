@@ -66,7 +66,8 @@ class MvsMatchStatsC(C.Structure):
                 ("matches", C.c_int64), ("overflow_items", C.c_int64), ("reserved_i", C.c_int64 * 3),
                 ("exact_ms", C.c_double), ("pass1_ms", C.c_double), ("threshold_ms", C.c_double),
                 ("pass2_ms", C.c_double), ("recheck_ms", C.c_double), ("select_ms", C.c_double),
-                ("reserved", C.c_double * 2)]
+                ("host_launch_ms", C.c_double), ("host_wait_ms", C.c_double), ("host_out_ms", C.c_double),
+                ("reserved", C.c_double * 3)]
 
 
 # every symbol the header declares, with its ctypes signature

@@ -14,6 +14,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <chrono>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -64,6 +65,27 @@ struct DBuf {
     }
     void release() {
         if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Pinned host staging (hipHostMalloc), grown on demand.
+template <class T>
+struct HBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    hipError_t need(size_t n) {
+        if (p && n <= cap) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipHostMalloc((void**)&p, (n ? n : 1) * sizeof(T), hipHostMallocDefault);
+        if (e == hipSuccess) cap = n;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
         p = nullptr;
         cap = 0;
     }
@@ -130,6 +152,9 @@ struct MatchState {
     DBuf<uint32_t> rcnt, ccnt, rcand, ccand, ovf, ovf_n;
     DBuf<uint32_t> oa, ob, da, db;
     DBuf<int64_t> counts, prefix;
+    HBuf<mt::PairDesc> h_pairs;        // pinned: pair lists up
+    HBuf<int64_t> h_counts;            // pinned: counts (+ overflow count) down
+    HBuf<uint32_t> h_out;              // pinned: dense match lists down
     mvs_match_stats_t stats;
     void release() {
         frames.release(); tmp.release(); set_off.release(); bad.release();
@@ -137,6 +162,7 @@ struct MatchState {
         frpart.release(); fcpart.release(); trow.release(); tcol.release();
         rcnt.release(); ccnt.release(); rcand.release(); ccand.release(); ovf.release(); ovf_n.release();
         oa.release(); ob.release(); da.release(); db.release(); counts.release(); prefix.release();
+        h_pairs.release(); h_counts.release(); h_out.release();
     }
 };
 
@@ -1901,6 +1927,7 @@ static int match_prepare(mvs_ctx* c, MatchSets& S, const float* desc, const int3
 // tiles; both write fwd / bwd, then one selection pass lists the matches.
 static int match_batch(mvs_ctx* c, MatchSets& S, const int32_t* pairs, int32_t npairs, double ratio,
                        uint32_t* out_a, uint32_t* out_b, int64_t* out_n, const char* who) {
+    const auto t_start = std::chrono::steady_clock::now();
     MatchState& M = c->match;
     std::vector<mt::PairDesc> all((size_t)npairs), ex, fl;
     int64_t fr = 0, fc = 0, out = 0;
@@ -1973,10 +2000,13 @@ static int match_batch(mvs_ctx* c, MatchSets& S, const int32_t* pairs, int32_t n
     mt::PairDesc* d_ex = M.pairs.p + npairs;
     mt::PairDesc* d_fl = d_ex + nex;
     hipStream_t st = c->stream;
-    if (npairs > 0)
-        HCHK(hipMemcpyAsync(d_all, all.data(), sizeof(mt::PairDesc) * (size_t)npairs, hipMemcpyHostToDevice, st));
-    if (nex > 0) HCHK(hipMemcpyAsync(d_ex, ex.data(), sizeof(mt::PairDesc) * nex, hipMemcpyHostToDevice, st));
-    if (nfl > 0) HCHK(hipMemcpyAsync(d_fl, fl.data(), sizeof(mt::PairDesc) * nfl, hipMemcpyHostToDevice, st));
+    const size_t nlist = (size_t)npairs + nex + nfl;
+    HCHK(M.h_pairs.need(nlist + 1));
+    HCHK(M.h_counts.need((size_t)npairs + 1));
+    std::copy(all.begin(), all.end(), M.h_pairs.p);
+    std::copy(ex.begin(), ex.end(), M.h_pairs.p + npairs);
+    std::copy(fl.begin(), fl.end(), M.h_pairs.p + npairs + nex);
+    if (nlist > 0) HCHK(hipMemcpyAsync(d_all, M.h_pairs.p, sizeof(mt::PairDesc) * nlist, hipMemcpyHostToDevice, st));
     mt::FiltBufs fb = {M.frpart.p, M.fcpart.p, M.trow.p, M.tcol.p, M.rcnt.p,
                        M.ccnt.p,   M.rcand.p,  M.ccand.p, M.ovf.p, M.ovf_n.p};
     HCHK(hipEventRecord(c->ev[0], st));
@@ -1991,32 +2021,38 @@ static int match_batch(mvs_ctx* c, MatchSets& S, const int32_t* pairs, int32_t n
     HCHK(mt::launch_select(d_all, npairs, M.fwd.p, M.bwd.p, (float)ratio, M.oa.p, M.ob.p, M.counts.p, st));
     HCHK(hipEventRecord(c->ev[6], st));
     HCHK(mt::launch_pack(d_all, npairs, M.counts.p, M.prefix.p, M.oa.p, M.ob.p, M.da.p, M.db.p, st));
-    std::vector<int64_t> cnt((size_t)npairs);
+    const auto t_launched = std::chrono::steady_clock::now();
+    int64_t* cnt = M.h_counts.p;
     uint32_t novf = 0;
     if (npairs > 0)
-        HCHK(hipMemcpyAsync(cnt.data(), M.counts.p, sizeof(int64_t) * (size_t)npairs, hipMemcpyDeviceToHost, st));
-    if (ft > 0) HCHK(hipMemcpyAsync(&novf, M.ovf_n.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HCHK(hipMemcpyAsync(cnt, M.counts.p, sizeof(int64_t) * (size_t)npairs, hipMemcpyDeviceToHost, st));
+    if (ft > 0) HCHK(hipMemcpyAsync(&cnt[npairs], M.ovf_n.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HCHK(hipStreamSynchronize(st));
+    if (ft > 0) memcpy(&novf, &cnt[npairs], sizeof(uint32_t));
+    const auto t_synced = std::chrono::steady_clock::now();
     // the dense lists come back in one copy per side and are spread into
     // the callers' slots (pair p at all[p].out)
     int64_t nm = 0;
     for (int32_t p = 0; p < npairs; p++) {
-        out_n[p] = cnt[(size_t)p];
-        nm += cnt[(size_t)p];
+        out_n[p] = cnt[p];
+        nm += cnt[p];
     }
     if (nm > 0 && (out_a || out_b)) {
-        std::vector<uint32_t> ha(out_a ? (size_t)nm : 0), hb(out_b ? (size_t)nm : 0);
-        if (out_a) HCHK(hipMemcpyAsync(ha.data(), M.da.p, sizeof(uint32_t) * (size_t)nm, hipMemcpyDeviceToHost, st));
-        if (out_b) HCHK(hipMemcpyAsync(hb.data(), M.db.p, sizeof(uint32_t) * (size_t)nm, hipMemcpyDeviceToHost, st));
+        HCHK(M.h_out.need(2 * (size_t)nm));
+        uint32_t* ha = M.h_out.p;
+        uint32_t* hb = M.h_out.p + nm;
+        if (out_a) HCHK(hipMemcpyAsync(ha, M.da.p, sizeof(uint32_t) * (size_t)nm, hipMemcpyDeviceToHost, st));
+        if (out_b) HCHK(hipMemcpyAsync(hb, M.db.p, sizeof(uint32_t) * (size_t)nm, hipMemcpyDeviceToHost, st));
         HCHK(hipStreamSynchronize(st));
         int64_t q = 0;
         for (int32_t p = 0; p < npairs; p++) {
-            const int64_t n = cnt[(size_t)p], o = all[(size_t)p].out;
-            if (n > 0 && out_a) memcpy(out_a + o, ha.data() + q, sizeof(uint32_t) * (size_t)n);
-            if (n > 0 && out_b) memcpy(out_b + o, hb.data() + q, sizeof(uint32_t) * (size_t)n);
+            const int64_t n = cnt[p], o = all[(size_t)p].out;
+            if (n > 0 && out_a) memcpy(out_a + o, ha + q, sizeof(uint32_t) * (size_t)n);
+            if (n > 0 && out_b) memcpy(out_b + o, hb + q, sizeof(uint32_t) * (size_t)n);
             q += n;
         }
     }
+    const auto t_done = std::chrono::steady_clock::now();
     float t[6] = {0, 0, 0, 0, 0, 0};
     for (int e = 0; e < 6; e++) (void)hipEventElapsedTime(&t[e], c->ev[e], c->ev[e + 1]);
     mvs_match_stats_t& R = M.stats;
@@ -2024,7 +2060,10 @@ static int match_batch(mvs_ctx* c, MatchSets& S, const int32_t* pairs, int32_t n
     R.pairs = npairs;
     R.filtered_pairs = (int64_t)nfl;
     for (const auto& q : all) R.distances += (int64_t)q.na * q.nb;
-    for (int32_t p = 0; p < npairs; p++) R.matches += cnt[(size_t)p];
+    R.matches = nm;
+    R.host_launch_ms = std::chrono::duration<double, std::milli>(t_launched - t_start).count();
+    R.host_wait_ms = std::chrono::duration<double, std::milli>(t_synced - t_launched).count();
+    R.host_out_ms = std::chrono::duration<double, std::milli>(t_done - t_synced).count();
     R.overflow_items = novf;
     R.exact_ms = t[0];
     R.pass1_ms = t[1];

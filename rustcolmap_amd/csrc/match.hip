@@ -389,6 +389,22 @@ __device__ __forceinline__ void two_min_merge(float& u1, float& u2, float v1, fl
     u1 = fminf(u1, v1);
 }
 
+// Lane exchange inside a 16-lane row through DPP (a VALU move, no LDS):
+// 0xB1 = quad_perm [1,0,3,2], 0x4E = quad_perm [2,3,0,1], 0x141 =
+// row_half_mirror, 0x140 = row_mirror.  The four steps pair every lane with
+// each of the other 15 through the merge tree.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ void two_min_row16(float& u1, float& u2) {
+    two_min_merge(u1, u2, dpp_f<0xB1>(u1), dpp_f<0xB1>(u2));
+    two_min_merge(u1, u2, dpp_f<0x4E>(u1), dpp_f<0x4E>(u2));
+    two_min_merge(u1, u2, dpp_f<0x141>(u1), dpp_f<0x141>(u2));
+    two_min_merge(u1, u2, dpp_f<0x140>(u1), dpp_f<0x140>(u2));
+}
+
 constexpr int LROW = 40;   // LDS row stride (u16): 32 components + 8 pad
 
 // PASS 1: two smallest upper bounds per row / column of a 128 x 128 tile.
@@ -423,29 +439,50 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
 #pragma unroll
         for (int nf = 0; nf < 4; ++nf) acc[mf][nf] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+    // Staging: thread item = 128 rows x 4 segments of 8 components, two per
+    // thread.  Rows past a set re-read its last row (in bounds; masked in the
+    // epilogue).  The next chunk's global loads are issued before this
+    // chunk's MFMAs and land in LDS after them.
+    const uint16_t* ga[2];
+    const uint16_t* gb[2];
+    int soff[2];
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const int item = threadIdx.x + 256 * it;
+        const int row = item >> 2, seg = item & 3;
+        ga[it] = hi + (size_t)(P.a + i0 + min(row, nra - 1)) * DIM + seg * 8;
+        gb[it] = hi + (size_t)(P.b + j0 + min(row, nrb - 1)) * DIM + seg * 8;
+        soff[it] = row * LROW + seg * 8;
+    }
+    const ptrdiff_t lo_off = lo - hi;
+    us8 pf[2][4];
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        pf[it][0] = *reinterpret_cast<const us8*>(ga[it]);
+        pf[it][1] = *reinterpret_cast<const us8*>(ga[it] + lo_off);
+        pf[it][2] = *reinterpret_cast<const us8*>(gb[it]);
+        pf[it][3] = *reinterpret_cast<const us8*>(gb[it] + lo_off);
+    }
+#pragma unroll
     for (int kc = 0; kc < DIM; kc += 32) {
-        if (kc) __syncthreads();
+        if (kc) __syncthreads();   // every wave is done with the previous chunk
 #pragma unroll
         for (int it = 0; it < 2; ++it) {
-            const int item = threadIdx.x + 256 * it;   // 128 rows x 4 segments of 8 components
-            const int row = item >> 2, seg = item & 3;
-            us8 vh = us8{0, 0, 0, 0, 0, 0, 0, 0}, vl = vh, wh = vh, wl = vh;
-            if (row < nra) {
-                const size_t g = (size_t)(P.a + i0 + row) * DIM + kc + seg * 8;
-                vh = *reinterpret_cast<const us8*>(hi + g);
-                vl = *reinterpret_cast<const us8*>(lo + g);
-            }
-            if (row < nrb) {
-                const size_t g = (size_t)(P.b + j0 + row) * DIM + kc + seg * 8;
-                wh = *reinterpret_cast<const us8*>(hi + g);
-                wl = *reinterpret_cast<const us8*>(lo + g);
-            }
-            *reinterpret_cast<us8*>(sAh + row * LROW + seg * 8) = vh;
-            *reinterpret_cast<us8*>(sAl + row * LROW + seg * 8) = vl;
-            *reinterpret_cast<us8*>(sBh + row * LROW + seg * 8) = wh;
-            *reinterpret_cast<us8*>(sBl + row * LROW + seg * 8) = wl;
+            *reinterpret_cast<us8*>(sAh + soff[it]) = pf[it][0];
+            *reinterpret_cast<us8*>(sAl + soff[it]) = pf[it][1];
+            *reinterpret_cast<us8*>(sBh + soff[it]) = pf[it][2];
+            *reinterpret_cast<us8*>(sBl + soff[it]) = pf[it][3];
         }
         __syncthreads();
+        if (kc + 32 < DIM) {
+#pragma unroll
+            for (int it = 0; it < 2; ++it) {
+                pf[it][0] = *reinterpret_cast<const us8*>(ga[it] + kc + 32);
+                pf[it][1] = *reinterpret_cast<const us8*>(ga[it] + lo_off + kc + 32);
+                pf[it][2] = *reinterpret_cast<const us8*>(gb[it] + kc + 32);
+                pf[it][3] = *reinterpret_cast<const us8*>(gb[it] + lo_off + kc + 32);
+            }
+        }
         us8 bh[4], bl[4];
 #pragma unroll
         for (int nf = 0; nf < 4; ++nf) {
@@ -469,22 +506,27 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
 
     // element (mf, nf, i) of this lane: tile row R = wr*64 + mf*16 + kq*4 + i,
     // tile column C = wc*64 + nf*16 + r16
+    // Norms, pre-scaled by the pass's bound factor (1 + EPS_REL for U,
+    // 1 - EPS_REL for L).  Rows / columns past a set get +inf in pass 1 (U =
+    // +inf never enters the two smallest) and NaN in pass 2 (every compare
+    // with L fails).
+    const float scale = PASS == 1 ? 1.0f + EPS_REL : PASS == 2 ? 1.0f - EPS_REL : 1.0f;
+    const float pad = PASS == 1 ? INFINITY : PASS == 2 ? __builtin_nanf("") : 0.0f;
     float nrow[4][4], ncol[4];
 #pragma unroll
     for (int mf = 0; mf < 4; ++mf)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int R = wr * 64 + mf * 16 + kq * 4 + i;
-            nrow[mf][i] = R < nra ? norm[P.a + i0 + R] : 0.0f;
+            nrow[mf][i] = R < nra ? scale * norm[P.a + i0 + R] : pad;
         }
 #pragma unroll
     for (int nf = 0; nf < 4; ++nf) {
         const int C = wc * 64 + nf * 16 + r16;
-        ncol[nf] = C < nrb ? norm[P.b + j0 + C] : 0.0f;
+        ncol[nf] = C < nrb ? scale * norm[P.b + j0 + C] : pad;
     }
 
     if constexpr (PASS == 1) {
-        const float cp = 1.0f + EPS_REL;
         float ru1[4][4], ru2[4][4], cu1[4], cu2[4];
 #pragma unroll
         for (int mf = 0; mf < 4; ++mf)
@@ -498,9 +540,7 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
             for (int nf = 0; nf < 4; ++nf)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const int R = wr * 64 + mf * 16 + kq * 4 + i, C = wc * 64 + nf * 16 + r16;
-                    float U = __builtin_fmaf(-2.0f, acc[mf][nf][i], cp * (nrow[mf][i] + ncol[nf]));
-                    if (R >= nra || C >= nrb) U = INFINITY;
+                    const float U = __builtin_fmaf(-2.0f, acc[mf][nf][i], nrow[mf][i] + ncol[nf]);
                     two_min(ru1[mf][i], ru2[mf][i], U);
                     two_min(cu1[nf], cu2[nf], U);
                 }
@@ -508,10 +548,7 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
 #pragma unroll
         for (int mf = 0; mf < 4; ++mf)
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int m = 1; m < 16; m <<= 1)
-                    two_min_merge(ru1[mf][i], ru2[mf][i], __shfl_xor(ru1[mf][i], m), __shfl_xor(ru2[mf][i], m));
+            for (int i = 0; i < 4; ++i) two_min_row16(ru1[mf][i], ru2[mf][i]);
         // columns: over the 4 kq groups, then the two row waves
 #pragma unroll
         for (int nf = 0; nf < 4; ++nf)
@@ -554,19 +591,18 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
             }
         }
     } else if constexpr (PASS == 2) {
-        const float cm = 1.0f - EPS_REL;
         float trow[4][4], tcol[4];
 #pragma unroll
         for (int mf = 0; mf < 4; ++mf)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int R = wr * 64 + mf * 16 + kq * 4 + i;
-                trow[mf][i] = R < nra ? fb.trow[P.lrow + i0 + R] : -INFINITY;
+                trow[mf][i] = R < nra ? fb.trow[P.lrow + i0 + R] : 0.0f;
             }
 #pragma unroll
         for (int nf = 0; nf < 4; ++nf) {
             const int C = wc * 64 + nf * 16 + r16;
-            tcol[nf] = C < nrb ? fb.tcol[P.lcol + j0 + C] : -INFINITY;
+            tcol[nf] = C < nrb ? fb.tcol[P.lcol + j0 + C] : 0.0f;
         }
 #pragma unroll
         for (int mf = 0; mf < 4; ++mf)
@@ -575,8 +611,8 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int R = wr * 64 + mf * 16 + kq * 4 + i, C = wc * 64 + nf * 16 + r16;
-                    const float L = __builtin_fmaf(-2.0f, acc[mf][nf][i], cm * (nrow[mf][i] + ncol[nf]));
-                    if (C < nrb && L <= trow[mf][i]) {
+                    const float L = __builtin_fmaf(-2.0f, acc[mf][nf][i], nrow[mf][i] + ncol[nf]);
+                    if (L <= trow[mf][i]) {
                         const uint32_t q = (uint32_t)(P.lrow + i0 + R);
                         const uint32_t k = atomicAdd(&fb.rcnt[q], 1u);
                         if (k < (uint32_t)CAP)
@@ -584,7 +620,7 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
                         else if (k == (uint32_t)CAP)
                             fb.ovf[atomicAdd(fb.ovf_n, 1u)] = q;
                     }
-                    if (R < nra && L <= tcol[nf]) {
+                    if (L <= tcol[nf]) {
                         const uint32_t q = (uint32_t)(P.lcol + j0 + C);
                         const uint32_t k = atomicAdd(&fb.ccnt[q], 1u);
                         if (k < (uint32_t)CAP)

@@ -11,6 +11,7 @@ libmvs_hip.so; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import time
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence, Tuple
 
@@ -92,8 +93,10 @@ class FrameMatcher:
         oa = np.empty(tot, np.uint32) if download else None
         ob = np.empty(tot, np.uint32) if download else None
         cnt = np.empty(max(len(pr), 1), np.int64)
+        t0 = time.perf_counter()
         check(self.ctx._lib.mvs_match_frames(self.ctx._h, ptr(pr), len(pr), float(ratio), ptr(oa), ptr(ob),
                                              ptr(cnt)), "mvs_match_frames")
+        self.last_call_ms = 1000.0 * (time.perf_counter() - t0)
         if not download:
             return [PairMatches(np.zeros(int(cnt[p]), np.uint32), np.zeros(int(cnt[p]), np.uint32))
                     for p in range(len(pr))]
@@ -105,7 +108,8 @@ class FrameMatcher:
 
 
 STAT_FIELDS = ("pairs", "filtered_pairs", "distances", "matches", "overflow_items", "exact_ms", "pass1_ms",
-               "threshold_ms", "pass2_ms", "recheck_ms", "select_ms")
+               "threshold_ms", "pass2_ms", "recheck_ms", "select_ms", "host_launch_ms", "host_wait_ms",
+               "host_out_ms")
 
 
 def match_stats(ctx: MvsContext) -> dict:
