@@ -469,9 +469,9 @@ def match_bench(args):
     pairs with the descriptors resident in HBM (match lists copied back to
     the host inside the step).  value = image pairs matched per second.
     With the filtered search (default) the dominant kernel is the MFMA tile
-    kernel, launched twice (pass 1 and pass 2); its algorithmic work is
-    3 bf16 products x 128 components x 2 flops per descriptor pair and
-    launch, against the dense bf16 MFMA peak.  With --match-exact it is the
+    kernel (one launch per step); its algorithmic work is 3 bf16 products x
+    128 components x 2 flops per descriptor pair, against the dense bf16
+    MFMA peak.  With --match-exact it is the
     exact distance-tile kernel: 3 f32 operations (difference, square, sum)
     per component, 384 per descriptor pair, against the VALU rate without
     FMA or packed math (the reference's rounding forbids both)."""
@@ -496,12 +496,12 @@ def match_bench(args):
         fm.match(pairs, 0.75)
     torch.cuda.synchronize()
     stages = []
-    t1 = time.perf_counter()
     call_ms = []
     step_ms = []
     import gc
     gc.collect()
     gc.disable()   # no collector pause of the interpreter inside the timed steps
+    t1 = time.perf_counter()
     for _ in range(args.steps):
         t2 = time.perf_counter()
         res = fm.match(pairs, 0.75)
@@ -514,18 +514,18 @@ def match_bench(args):
     st = stages[-1]
     n_match = sum(len(r) for r in res)
     distances = st["distances"]
-    avg = {k: float(np.mean([x[k] for x in stages])) for k in ("exact_ms", "pass1_ms", "threshold_ms", "pass2_ms",
-                                                               "recheck_ms", "select_ms")}
+    avg = {k: float(np.mean([x[k] for x in stages])) for k in ("exact_ms", "tile_ms", "collect_ms", "recheck_ms",
+                                                               "select_ms")}
     host = {k: round(float(np.mean([x[k] for x in stages])), 4) for k in ("host_launch_ms", "host_wait_ms",
                                                                         "host_out_ms")}
     host["c_call_ms"] = round(float(np.mean(call_ms)), 4)
     host["python_step_ms"] = [round(x, 3) for x in step_ms]
     filtered = st["filtered_pairs"] == st["pairs"]
     if filtered:
-        # dominant kernel: the MFMA tile kernel, launched twice per step
-        # (pass 1 and pass 2); 3 bf16 products of 128 components per distance
-        kname = "mt::filt_tile_kernel"
-        launch_ms = (avg["pass1_ms"] + avg["pass2_ms"]) / 2.0
+        # dominant kernel: the MFMA tile kernel, one launch per step; 3 bf16
+        # products of 128 components per distance
+        kname = "mt::filt_tile_kernel<1>"
+        launch_ms = avg["tile_ms"]
         flops = distances * 3 * 2 * 128.0
         peak, bound = MFMA_BF16_PEAK_TFLOPS, "mfma"
     else:
@@ -534,10 +534,10 @@ def match_bench(args):
         flops = distances * 384.0
         peak, bound = VALU_PEAK_TOPS, "valu"
     achieved = flops / (launch_ms / 1000.0) / 1e12
-    if filtered:   # mean HBM bytes of the pass-1 and pass-2 launches
-        tr = [profiled_traffic(kname + "<%d>" % k, MATCH_TRAFFIC_SUMMARY) for k in (1, 2)]
-        traffic = None if None in tr or args.match_frames != 20 or args.match_features != 2048 else \
-            ((tr[0][0] + tr[1][0]) / 2.0, tr[0][1])
+    if filtered:
+        traffic = profiled_traffic(kname, MATCH_TRAFFIC_SUMMARY)
+        if args.match_frames != 20 or args.match_features != 2048:
+            traffic = None   # the committed summary is for the default workload
     else:
         traffic = None   # no committed counter summary for the exact tiles
 

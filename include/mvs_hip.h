@@ -357,12 +357,12 @@ typedef struct mvs_match_stats {   /* the last batch */
     int64_t overflow_items;            /* rows / columns scanned in full after the filter */
     int64_t reserved_i[3];
     double exact_ms;                   /* exact tiles + merge (pairs outside the filter) */
-    double pass1_ms, threshold_ms, pass2_ms, recheck_ms;   /* filtered search stages */
+    double tile_ms, collect_ms, recheck_ms;   /* filtered search: MFMA tile pass, candidates, exact recheck */
     double select_ms;                  /* mutual + ratio tests, compaction */
     double host_launch_ms;             /* host: argument checks, pair lists, launches */
     double host_wait_ms;               /* host: waiting for the device */
     double host_out_ms;                /* host: match lists copied into the caller's slots */
-    double reserved[3];
+    double reserved[4];
 } mvs_match_stats_t;
 
 /* match_pair(fa, fb, ratio) (matching.rs:22): host sets of na and nb

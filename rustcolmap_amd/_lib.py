@@ -64,10 +64,10 @@ class MvsRunStatsC(C.Structure):
 class MvsMatchStatsC(C.Structure):
     _fields_ = [("pairs", C.c_int64), ("filtered_pairs", C.c_int64), ("distances", C.c_int64),
                 ("matches", C.c_int64), ("overflow_items", C.c_int64), ("reserved_i", C.c_int64 * 3),
-                ("exact_ms", C.c_double), ("pass1_ms", C.c_double), ("threshold_ms", C.c_double),
-                ("pass2_ms", C.c_double), ("recheck_ms", C.c_double), ("select_ms", C.c_double),
+                ("exact_ms", C.c_double), ("tile_ms", C.c_double), ("collect_ms", C.c_double),
+                ("recheck_ms", C.c_double), ("select_ms", C.c_double),
                 ("host_launch_ms", C.c_double), ("host_wait_ms", C.c_double), ("host_out_ms", C.c_double),
-                ("reserved", C.c_double * 3)]
+                ("reserved", C.c_double * 4)]
 
 
 # every symbol the header declares, with its ctypes signature

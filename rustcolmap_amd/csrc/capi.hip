@@ -147,8 +147,8 @@ struct MatchState {
     DBuf<int> bad;
     DBuf<mt::PairDesc> pairs;
     DBuf<mt::Best2> rpart, cpart, fwd, bwd;
-    DBuf<float2> frpart, fcpart;
-    DBuf<float> trow, tcol;
+    DBuf<mt::TileMeta> rmeta, cmeta;
+    DBuf<uint4> rmask, cmask;
     DBuf<uint32_t> rcnt, ccnt, rcand, ccand, ovf, ovf_n;
     DBuf<uint32_t> oa, ob, da, db;
     DBuf<int64_t> counts, prefix;
@@ -159,7 +159,7 @@ struct MatchState {
     void release() {
         frames.release(); tmp.release(); set_off.release(); bad.release();
         pairs.release(); rpart.release(); cpart.release(); fwd.release(); bwd.release();
-        frpart.release(); fcpart.release(); trow.release(); tcol.release();
+        rmeta.release(); cmeta.release(); rmask.release(); cmask.release();
         rcnt.release(); ccnt.release(); rcand.release(); ccand.release(); ovf.release(); ovf_n.release();
         oa.release(); ob.release(); da.release(); db.release(); counts.release(); prefix.release();
         h_pairs.release(); h_counts.release(); h_out.release();
@@ -1979,10 +1979,10 @@ static int match_batch(mvs_ctx* c, MatchSets& S, const int32_t* pairs, int32_t n
     HCHK(M.cpart.need((size_t)std::max<int64_t>(xcp, 1)));
     HCHK(M.fwd.need((size_t)std::max<int64_t>(fr, 1)));
     HCHK(M.bwd.need((size_t)std::max<int64_t>(fc, 1)));
-    HCHK(M.frpart.need((size_t)std::max<int64_t>(frp, 1)));
-    HCHK(M.fcpart.need((size_t)std::max<int64_t>(fcp, 1)));
-    HCHK(M.trow.need((size_t)std::max<int64_t>(frr, 1)));
-    HCHK(M.tcol.need((size_t)std::max<int64_t>(fcc, 1)));
+    HCHK(M.rmeta.need((size_t)std::max<int64_t>(frp, 1)));
+    HCHK(M.cmeta.need((size_t)std::max<int64_t>(fcp, 1)));
+    HCHK(M.rmask.need((size_t)std::max<int64_t>(frp, 1)));
+    HCHK(M.cmask.need((size_t)std::max<int64_t>(fcp, 1)));
     HCHK(M.rcnt.need((size_t)std::max<int64_t>(frr, 1)));
     HCHK(M.ccnt.need((size_t)std::max<int64_t>(fcc, 1)));
     HCHK(M.rcand.need((size_t)std::max<int64_t>(frr, 1) * mt::CAP));
@@ -2007,8 +2007,8 @@ static int match_batch(mvs_ctx* c, MatchSets& S, const int32_t* pairs, int32_t n
     std::copy(ex.begin(), ex.end(), M.h_pairs.p + npairs);
     std::copy(fl.begin(), fl.end(), M.h_pairs.p + npairs + nex);
     if (nlist > 0) HCHK(hipMemcpyAsync(d_all, M.h_pairs.p, sizeof(mt::PairDesc) * nlist, hipMemcpyHostToDevice, st));
-    mt::FiltBufs fb = {M.frpart.p, M.fcpart.p, M.trow.p, M.tcol.p, M.rcnt.p,
-                       M.ccnt.p,   M.rcand.p,  M.ccand.p, M.ovf.p, M.ovf_n.p};
+    mt::FiltBufs fb = {M.rmeta.p, M.rmask.p, M.cmeta.p, M.cmask.p, M.rcnt.p,
+                       M.ccnt.p,  M.rcand.p, M.ccand.p, M.ovf.p,   M.ovf_n.p};
     HCHK(hipEventRecord(c->ev[0], st));
     HCHK(mt::launch_dist_tiles(S.desc.p, d_ex, (int)nex, xt, M.rpart.p, M.cpart.p, st));
     HCHK(mt::launch_merge_partials(d_ex, (int)nex, xr, xc, M.rpart.p, M.cpart.p, M.fwd.p, M.bwd.p, st));
@@ -2016,10 +2016,10 @@ static int match_batch(mvs_ctx* c, MatchSets& S, const int32_t* pairs, int32_t n
     HCHK(mt::launch_filtered(S.desc.p, S.norm.p, S.hi.p, S.lo.p, d_fl, (int)nfl, ft, frr, fcc, fb, M.fwd.p, M.bwd.p,
                              st, c->ev + 2));
     if (ft <= 0)
-        for (int e = 2; e < 5; e++) HCHK(hipEventRecord(c->ev[e], st));
-    HCHK(hipEventRecord(c->ev[5], st));
+        for (int e = 2; e < 4; e++) HCHK(hipEventRecord(c->ev[e], st));
+    HCHK(hipEventRecord(c->ev[4], st));
     HCHK(mt::launch_select(d_all, npairs, M.fwd.p, M.bwd.p, (float)ratio, M.oa.p, M.ob.p, M.counts.p, st));
-    HCHK(hipEventRecord(c->ev[6], st));
+    HCHK(hipEventRecord(c->ev[5], st));
     HCHK(mt::launch_pack(d_all, npairs, M.counts.p, M.prefix.p, M.oa.p, M.ob.p, M.da.p, M.db.p, st));
     const auto t_launched = std::chrono::steady_clock::now();
     int64_t* cnt = M.h_counts.p;
@@ -2053,8 +2053,8 @@ static int match_batch(mvs_ctx* c, MatchSets& S, const int32_t* pairs, int32_t n
         }
     }
     const auto t_done = std::chrono::steady_clock::now();
-    float t[6] = {0, 0, 0, 0, 0, 0};
-    for (int e = 0; e < 6; e++) (void)hipEventElapsedTime(&t[e], c->ev[e], c->ev[e + 1]);
+    float t[5] = {0, 0, 0, 0, 0};
+    for (int e = 0; e < 5; e++) (void)hipEventElapsedTime(&t[e], c->ev[e], c->ev[e + 1]);
     mvs_match_stats_t& R = M.stats;
     memset(&R, 0, sizeof(R));
     R.pairs = npairs;
@@ -2066,11 +2066,10 @@ static int match_batch(mvs_ctx* c, MatchSets& S, const int32_t* pairs, int32_t n
     R.host_out_ms = std::chrono::duration<double, std::milli>(t_done - t_synced).count();
     R.overflow_items = novf;
     R.exact_ms = t[0];
-    R.pass1_ms = t[1];
-    R.threshold_ms = t[2];
-    R.pass2_ms = t[3];
-    R.recheck_ms = t[4];
-    R.select_ms = t[5];
+    R.tile_ms = t[1];
+    R.collect_ms = t[2];
+    R.recheck_ms = t[3];
+    R.select_ms = t[4];
     return MVS_OK;
 }
 

@@ -379,8 +379,9 @@ __global__ __launch_bounds__(256) void prep_kernel(const float* __restrict__ des
     }
 }
 
+// (u1 <= u2) <- the two smallest of {u1, u2, v}; med3 is the middle value.
 __device__ __forceinline__ void two_min(float& u1, float& u2, float v) {
-    u2 = fminf(u2, fmaxf(u1, v));
+    u2 = __builtin_amdgcn_fmed3f(u1, u2, v);
     u1 = fminf(u1, v);
 }
 
@@ -407,8 +408,12 @@ __device__ __forceinline__ void two_min_row16(float& u1, float& u2) {
 
 constexpr int LROW = 40;   // LDS row stride (u16): 32 components + 8 pad
 
-// PASS 1: two smallest upper bounds per row / column of a 128 x 128 tile.
-// PASS 2: candidate lists against the thresholds.
+// PASS 1: one record per row and per column of a 128 x 128 tile: the two
+//         smallest upper bounds U (u1 <= u2) and the elements whose lower
+//         bound L is <= u2 (count, and the first FSLOTS as tile-local index
+//         and L).  The row's global threshold T (its second smallest U over
+//         all tiles) is <= u2 of every tile, so every element with L <= T is
+//         among them.
 // PASS 3: (check) max over the tile of |exact - estimate| / (|a|^2 + |b|^2)
 //         into *fb.ovf_n as float bits.
 // 4 waves in a 2 x 2 grid, each 64 x 64 = 4 x 4 MFMA 16x16x32 fragments.
@@ -423,6 +428,7 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
     if (t >= ntiles) return;
     __shared__ int s_pair;
     if (threadIdx.x == 0) s_pair = find_pair(pairs, npairs, t);
+    if (PASS == 1 && t == 0 && threadIdx.x == 0) *fb.ovf_n = 0u;   // read by the collection after this launch
     __syncthreads();
     const PairDesc P = pairs[s_pair];
     const int64_t lt = t - P.tile0;
@@ -506,12 +512,12 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
 
     // element (mf, nf, i) of this lane: tile row R = wr*64 + mf*16 + kq*4 + i,
     // tile column C = wc*64 + nf*16 + r16
-    // Norms, pre-scaled by the pass's bound factor (1 + EPS_REL for U,
-    // 1 - EPS_REL for L).  Rows / columns past a set get +inf in pass 1 (U =
-    // +inf never enters the two smallest) and NaN in pass 2 (every compare
-    // with L fails).
-    const float scale = PASS == 1 ? 1.0f + EPS_REL : PASS == 2 ? 1.0f - EPS_REL : 1.0f;
-    const float pad = PASS == 1 ? INFINITY : PASS == 2 ? __builtin_nanf("") : 0.0f;
+    // Norms, pre-scaled by 1 + EPS_REL in pass 1: tU = (1 + EPS)(|a|^2 +
+    // |b|^2), U = tU - 2G, L = U - k tU with k = 2 EPS / (1 + EPS).  Rows /
+    // columns past a set get +inf: U = +inf never enters the two smallest and
+    // L = NaN fails every compare.
+    const float scale = PASS == 1 ? 1.0f + EPS_REL : 1.0f;
+    const float pad = PASS == 1 ? INFINITY : 0.0f;
     float nrow[4][4], ncol[4];
 #pragma unroll
     for (int mf = 0; mf < 4; ++mf)
@@ -555,9 +561,14 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
 #pragma unroll
             for (int m = 16; m < 64; m <<= 1)
                 two_min_merge(cu1[nf], cu2[nf], __shfl_xor(cu1[nf], m), __shfl_xor(cu2[nf], m));
-        __syncthreads();   // the staging buffers are reused
-        float2* srow = reinterpret_cast<float2*>(sAh);   // [2 column waves][128 rows]
-        float2* scol = reinterpret_cast<float2*>(sBh);   // [2 row waves][128 columns]
+        __syncthreads();   // the staging buffers are reused below
+        float2* srow = reinterpret_cast<float2*>(sAh);               // [2 column waves][128 rows]
+        float2* scol = reinterpret_cast<float2*>(sAh) + 2 * FTILE;   // [2 row waves][128 columns]
+        float2* sur = reinterpret_cast<float2*>(sAl);                // merged (u1, u2) per row
+        float2* suc = reinterpret_cast<float2*>(sAl) + FTILE;        // per column
+        uint32_t* srm = reinterpret_cast<uint32_t*>(sBh);            // [128 rows][4 words]: columns with L <= u2
+        uint32_t* scm = srm + 4 * FTILE;                             // [128 columns][4 words]: rows
+        uint32_t* snmax = reinterpret_cast<uint32_t*>(sBl);          // [2]: max scaled norm of rows, columns
         if (r16 == 0) {
 #pragma unroll
             for (int mf = 0; mf < 4; ++mf)
@@ -574,61 +585,118 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
                 scol[wr * FTILE + C] = make_float2(cu1[nf], cu2[nf]);
             }
         }
+        if (threadIdx.x < 2) snmax[threadIdx.x] = 0u;
         __syncthreads();
-        if (threadIdx.x < FTILE) {
-            const int R = threadIdx.x;
-            if (R < nra) {
-                float2 a = srow[R], b = srow[FTILE + R];
-                two_min_merge(a.x, a.y, b.x, b.y);
-                fb.rpart[P.rpart + (int64_t)ct * P.na + i0 + R] = a;
-            }
-        } else {
-            const int C = threadIdx.x - FTILE;
-            if (C < nrb) {
-                float2 a = scol[C], b = scol[FTILE + C];
-                two_min_merge(a.x, a.y, b.x, b.y);
-                fb.cpart[P.cpart + (int64_t)rt * P.nb + j0 + C] = a;
-            }
+        {
+            const int q = threadIdx.x, e = q & (FTILE - 1);
+            const bool is_row = q < FTILE;
+            float2 a = is_row ? srow[e] : scol[e];
+            const float2 b = is_row ? srow[FTILE + e] : scol[FTILE + e];
+            two_min_merge(a.x, a.y, b.x, b.y);
+            (is_row ? sur : suc)[e] = a;
+            // scaled norms are >= 0, so their bit patterns order as the values
+            if (e < (is_row ? nra : nrb))
+                atomicMax(&snmax[is_row ? 0 : 1],
+                          __float_as_uint((1.0f + EPS_REL) * norm[is_row ? P.a + i0 + e : P.b + j0 + e]));
         }
-    } else if constexpr (PASS == 2) {
-        float trow[4][4], tcol[4];
+        __syncthreads();
+        // Second sweep over the accumulators: bit masks of the elements with
+        // L <= the tile's u2, per row (columns) and per column (rows).  The
+        // empty asm makes the norms and accumulators opaque, so tU and U are
+        // recomputed from the accumulator registers instead of being kept
+        // live (64 more registers) across the reductions.
+#pragma unroll
+        for (int mf = 0; mf < 4; ++mf)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) asm volatile("" : "+v"(nrow[mf][i]));
+#pragma unroll
+        for (int nf = 0; nf < 4; ++nf) asm volatile("" : "+v"(ncol[nf]));
+#pragma unroll
+        for (int mf = 0; mf < 4; ++mf)
+#pragma unroll
+            for (int nf = 0; nf < 4; ++nf) asm volatile("" : "+a"(acc[mf][nf]));
+        const float kb = 2.0f * EPS_REL / (1.0f + EPS_REL);
+        float tc[4];
+        uint32_t cw[4][2];   // per column block: rows wr*64 + (0..31), wr*64 + (32..63)
+#pragma unroll
+        for (int nf = 0; nf < 4; ++nf) {
+            tc[nf] = suc[wc * 64 + nf * 16 + r16].y;
+            cw[nf][0] = cw[nf][1] = 0u;
+        }
 #pragma unroll
         for (int mf = 0; mf < 4; ++mf)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int R = wr * 64 + mf * 16 + kq * 4 + i;
-                trow[mf][i] = R < nra ? fb.trow[P.lrow + i0 + R] : 0.0f;
+                const float tr = sur[R].y;
+                uint32_t rw0 = 0u, rw1 = 0u;   // columns wc*64 + (0..31), wc*64 + (32..63)
+#pragma unroll
+                for (int nf = 0; nf < 4; ++nf) {
+                    const float tU = nrow[mf][i] + ncol[nf];
+                    const float U = __builtin_fmaf(-2.0f, acc[mf][nf][i], tU);
+                    const float L = __builtin_fmaf(-kb, tU, U);
+                    const uint32_t rbit = L <= tr ? 1u << ((nf & 1) * 16 + r16) : 0u;
+                    if (nf < 2)
+                        rw0 |= rbit;
+                    else
+                        rw1 |= rbit;
+                    cw[nf][mf >> 1] |= L <= tc[nf] ? 1u << ((mf & 1) * 16 + kq * 4 + i) : 0u;
+                }
+                // OR over the 16 lanes of the row group; lane r16 == 0 owns the
+                // row's two words of this column half (no other wave writes them)
+                rw0 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw0, 0xB1, 0xF, 0xF, false);
+                rw1 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw1, 0xB1, 0xF, 0xF, false);
+                rw0 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw0, 0x4E, 0xF, 0xF, false);
+                rw1 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw1, 0x4E, 0xF, 0xF, false);
+                rw0 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw0, 0x141, 0xF, 0xF, false);
+                rw1 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw1, 0x141, 0xF, 0xF, false);
+                rw0 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw0, 0x140, 0xF, 0xF, false);
+                rw1 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw1, 0x140, 0xF, 0xF, false);
+                if (r16 == 0) {
+                    srm[R * 4 + wc * 2 + 0] = rw0;
+                    srm[R * 4 + wc * 2 + 1] = rw1;
+                }
             }
+        // columns: OR over the 4 kq groups; lane kq == 0 owns the column's two
+        // words of this row half
 #pragma unroll
         for (int nf = 0; nf < 4; ++nf) {
-            const int C = wc * 64 + nf * 16 + r16;
-            tcol[nf] = C < nrb ? fb.tcol[P.lcol + j0 + C] : 0.0f;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                cw[nf][h] |= (uint32_t)__shfl_xor((int)cw[nf][h], 16);
+                cw[nf][h] |= (uint32_t)__shfl_xor((int)cw[nf][h], 32);
+            }
+            if (kq == 0) {
+                const int C = wc * 64 + nf * 16 + r16;
+                scm[C * 4 + wr * 2 + 0] = cw[nf][0];
+                scm[C * 4 + wr * 2 + 1] = cw[nf][1];
+            }
         }
-#pragma unroll
-        for (int mf = 0; mf < 4; ++mf)
-#pragma unroll
-            for (int nf = 0; nf < 4; ++nf)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int R = wr * 64 + mf * 16 + kq * 4 + i, C = wc * 64 + nf * 16 + r16;
-                    const float L = __builtin_fmaf(-2.0f, acc[mf][nf][i], nrow[mf][i] + ncol[nf]);
-                    if (L <= trow[mf][i]) {
-                        const uint32_t q = (uint32_t)(P.lrow + i0 + R);
-                        const uint32_t k = atomicAdd(&fb.rcnt[q], 1u);
-                        if (k < (uint32_t)CAP)
-                            fb.rcand[(size_t)q * CAP + k] = (uint32_t)(j0 + C);
-                        else if (k == (uint32_t)CAP)
-                            fb.ovf[atomicAdd(fb.ovf_n, 1u)] = q;
-                    }
-                    if (L <= tcol[nf]) {
-                        const uint32_t q = (uint32_t)(P.lcol + j0 + C);
-                        const uint32_t k = atomicAdd(&fb.ccnt[q], 1u);
-                        if (k < (uint32_t)CAP)
-                            fb.ccand[(size_t)q * CAP + k] = (uint32_t)(i0 + R);
-                        else if (k == (uint32_t)CAP)
-                            fb.ovf[atomicAdd(fb.ovf_n, 1u)] = 0x80000000u | q;
-                    }
-                }
+        __syncthreads();
+        // records: threads 0..127 rows, 128..255 columns
+        const int q = threadIdx.x, e = q & (FTILE - 1);
+        const bool is_row = q < FTILE;
+        if (e < (is_row ? nra : nrb)) {
+            const float2 u = is_row ? sur[e] : suc[e];
+            const float own = (1.0f + EPS_REL) * norm[is_row ? P.a + i0 + e : P.b + j0 + e];
+            const float other = __uint_as_float(snmax[is_row ? 1 : 0]);
+            // every element's L >= u1 - kb * max tU; the factor and the
+            // constant cover the rounding of L
+            TileMeta m;
+            m.u1 = u.x;
+            m.u2 = u.y;
+            m.lb = u.x - kb * (own + other) * (1.0f + 0x1p-20f) - 0x1p-100f;
+            m.pad = 0u;
+            const uint32_t* w = (is_row ? srm : scm) + e * 4;
+            const uint4 mask = make_uint4(w[0], w[1], w[2], w[3]);
+            if (is_row) {
+                fb.rmeta[P.rpart + (int64_t)ct * P.na + i0 + e] = m;
+                fb.rmask[P.rpart + (int64_t)ct * P.na + i0 + e] = mask;
+            } else {
+                fb.cmeta[P.cpart + (int64_t)rt * P.nb + j0 + e] = m;
+                fb.cmask[P.cpart + (int64_t)rt * P.nb + j0 + e] = mask;
+            }
+        }
     } else {
         float worst = 0.0f;
 #pragma unroll
@@ -655,26 +723,51 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
     }
 }
 
-// Launch-local row / column q: T = second smallest upper bound over its
-// tiles; clears its candidate count.
-__global__ __launch_bounds__(256) void filt_threshold_kernel(const PairDesc* __restrict__ pairs, int npairs,
-                                                             int64_t rows_total, int64_t cols_total, FiltBufs fb) {
+// Launch-local row / column q: T = the second smallest upper bound over its
+// tiles; every element with L <= T lies in a tile whose lower bound is <= T
+// and is set in that tile's mask (L <= T <= the tile's u2).  The masked
+// elements of those tiles are the candidates -- a superset, which the exact
+// recheck cannot be misled by: an extra element has exact distance >= L > T,
+// above the second nearest.  More than CAP: the row is scanned in full.
+__global__ __launch_bounds__(256) void filt_collect_kernel(const PairDesc* __restrict__ pairs, int npairs,
+                                                           int64_t rows_total, int64_t cols_total, FiltBufs fb) {
     const bool cols = blockIdx.y == 1;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q == 0 && !cols) *fb.ovf_n = 0u;
     if (q >= (cols ? cols_total : rows_total)) return;
     const PairDesc P = pairs[find_pair_by(pairs, npairs, q, cols)];
     const int64_t local = q - (cols ? P.lcol : P.lrow);
     const int nt = cols ? P.ta : P.tb;
     const int n = cols ? P.nb : P.na;
-    const float2* src = (cols ? fb.cpart + P.cpart : fb.rpart + P.rpart) + local;
+    const int64_t base = (cols ? P.cpart : P.rpart) + local;
+    const TileMeta* meta = (cols ? fb.cmeta : fb.rmeta) + base;
+    const uint4* mask = (cols ? fb.cmask : fb.rmask) + base;
     float u1 = INFINITY, u2 = INFINITY;
     for (int t = 0; t < nt; ++t) {
-        const float2 p = src[(int64_t)t * n];
-        two_min_merge(u1, u2, p.x, p.y);
+        const TileMeta m = meta[(int64_t)t * n];
+        two_min_merge(u1, u2, m.u1, m.u2);
     }
-    (cols ? fb.tcol : fb.trow)[q] = u2;
-    (cols ? fb.ccnt : fb.rcnt)[q] = 0u;
+    uint32_t c = 0;
+    uint32_t* cand = (cols ? fb.ccand : fb.rcand) + (size_t)q * CAP;
+    for (int t = 0; t < nt; ++t) {
+        if (!(meta[(int64_t)t * n].lb <= u2)) continue;
+        const uint4 m4 = mask[(int64_t)t * n];
+        const uint32_t w[4] = {m4.x, m4.y, m4.z, m4.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t bits = w[k];
+            while (bits) {
+                const int b = __builtin_ctz(bits);
+                bits &= bits - 1u;
+                if (c < (uint32_t)CAP) cand[c] = (uint32_t)(t * FTILE + k * 32 + b);
+                ++c;
+            }
+        }
+    }
+    if (c > (uint32_t)CAP) {
+        c = CAP + 1;
+        fb.ovf[atomicAdd(fb.ovf_n, 1u)] = (cols ? 0x80000000u : 0u) | (uint32_t)q;
+    }
+    (cols ? fb.ccnt : fb.rcnt)[q] = c;
 }
 
 // Exact distance of a-side descriptor x and b-side descriptor y, as the
@@ -698,12 +791,13 @@ __device__ __forceinline__ float exact_dist(const float* __restrict__ x, const f
     return s;
 }
 
-// 8 lanes per launch-local row or column: lane s rechecks candidate s.
+// 8 lanes per launch-local row or column: lane s rechecks candidates s and
+// s + 8; candidates may come in any order (the merge is order-free).
 __global__ __launch_bounds__(256) void filt_recheck_kernel(const float* __restrict__ desc,
                                                            const PairDesc* __restrict__ pairs, int npairs,
                                                            int64_t rows_total, int64_t cols_total, FiltBufs fb,
                                                            Best2* __restrict__ fwd, Best2* __restrict__ bwd) {
-    static_assert(CAP == 8, "one lane per candidate slot");
+    static_assert(CAP == 16, "8 lanes, two candidate slots each");
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t item = g >> 3;
     const int s = (int)(g & 7);
@@ -719,12 +813,11 @@ __global__ __launch_bounds__(256) void filt_recheck_kernel(const float* __restri
         P = pairs[find_pair_by(pairs, npairs, q, !row)];
         local = q - (row ? P.lrow : P.lcol);
         cnt = row ? fb.rcnt[q] : fb.ccnt[q];
-        if (cnt <= (uint32_t)CAP && (uint32_t)s < cnt) {
-            const uint32_t other = row ? fb.rcand[(size_t)q * CAP + s] : fb.ccand[(size_t)q * CAP + s];
+        for (uint32_t k = (uint32_t)s; cnt <= (uint32_t)CAP && k < cnt; k += 8) {
+            const uint32_t other = row ? fb.rcand[(size_t)q * CAP + k] : fb.ccand[(size_t)q * CAP + k];
             const float* x = desc + (row ? P.a + local : P.a + other) * DIM;
             const float* y = desc + (row ? P.b + other : P.b + local) * DIM;
-            b = exact_dist(x, y);
-            bi = other;
+            merge(b, bi, sec, exact_dist(x, y), other, INFINITY);
         }
     }
     merge_lanes(b, bi, sec, 1);
@@ -861,21 +954,16 @@ hipError_t launch_filtered(const float* desc, const float* norm, const uint16_t*
                            const FiltBufs& fb, Best2* fwd, Best2* bwd, hipStream_t st, hipEvent_t* ev) {
     if (ntiles <= 0) return hipSuccess;
     const int64_t per_xcd = (ntiles + 7) / 8;
-    const dim3 tgrid((unsigned)(per_xcd * 8));
     hipError_t e;
-    hipLaunchKernelGGL(filt_tile_kernel<1>, tgrid, dim3(256), 0, st, desc, norm, hi, lo, pairs, npairs, ntiles,
-                       per_xcd, fb);
+    hipLaunchKernelGGL(filt_tile_kernel<1>, dim3((unsigned)(per_xcd * 8)), dim3(256), 0, st, desc, norm, hi, lo,
+                       pairs, npairs, ntiles, per_xcd, fb);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[0], st);
     const int64_t n = rows_total > cols_total ? rows_total : cols_total;
-    hipLaunchKernelGGL(filt_threshold_kernel, dim3((unsigned)((n + 255) / 256), 2), dim3(256), 0, st, pairs, npairs,
+    hipLaunchKernelGGL(filt_collect_kernel, dim3((unsigned)((n + 255) / 256), 2), dim3(256), 0, st, pairs, npairs,
                        rows_total, cols_total, fb);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev) (void)hipEventRecord(ev[1], st);
-    hipLaunchKernelGGL(filt_tile_kernel<2>, tgrid, dim3(256), 0, st, desc, norm, hi, lo, pairs, npairs, ntiles,
-                       per_xcd, fb);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (ev) (void)hipEventRecord(ev[2], st);
     const int64_t items = (rows_total + cols_total) * 8;
     hipLaunchKernelGGL(filt_recheck_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, st, desc, pairs,
                        npairs, rows_total, cols_total, fb, fwd, bwd);

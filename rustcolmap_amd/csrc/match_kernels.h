@@ -42,7 +42,7 @@ struct PairDesc {
 // Filtered search (bf16x3 MFMA products, exact re-evaluation of the few
 // descriptors that can be nearest or second nearest).
 constexpr int FTILE = 128;   // descriptors per filtered tile side
-constexpr int CAP = 8;       // candidate slots per descriptor
+constexpr int CAP = 16;      // candidate slots per descriptor
 // |exact f32 distance - MFMA estimate| <= EPS_REL * (|a|^2 + |b|^2) on
 // descriptors whose nonzero components lie in [2^-40, 2^40] (safe sets)
 constexpr float EPS_REL = 1.0f / 2048.0f;
@@ -53,13 +53,21 @@ constexpr float EPS_REL = 1.0f / 2048.0f;
 hipError_t launch_prep(const float* desc, int64_t n, const int64_t* set_off, int nsets, float* norm, uint16_t* hi,
                        uint16_t* lo, int* bad, hipStream_t st);
 
+// Per row (or column) and tile of the filtered search: the two smallest
+// upper bounds and a lower bound of every element's L in the tile; beside
+// it a 128-bit mask of the tile's elements whose L is <= u2.
+struct TileMeta {
+    float u1, u2, lb;
+    uint32_t pad;
+};
+
 // Buffers of one filtered launch (indices are launch-local rows / columns).
 struct FiltBufs {
-    float2* rpart;     // [tb][na] two smallest upper bounds per row and column tile
-    float2* cpart;     // [ta][nb]
-    float* trow;       // second smallest upper bound over the row
-    float* tcol;
-    uint32_t* rcnt;    // candidates per row (may exceed CAP: overflow)
+    TileMeta* rmeta;   // [tb][na] per row and column tile
+    uint4* rmask;
+    TileMeta* cmeta;   // [ta][nb] per column and row tile
+    uint4* cmask;
+    uint32_t* rcnt;    // candidates per row (CAP + 1: overflow, scanned in full)
     uint32_t* ccnt;
     uint32_t* rcand;   // [rows][CAP]
     uint32_t* ccand;
@@ -67,9 +75,9 @@ struct FiltBufs {
     uint32_t* ovf_n;   // [1]
 };
 
-// Pass 1, thresholds, pass 2, recheck and overflow scans of the pairs in
-// `pairs` (all sets safe; tiles of FTILE); writes fwd / bwd.  ev (optional,
-// 3 events) are recorded after pass 1, the thresholds and pass 2.
+// MFMA tile pass, candidate collection, recheck and overflow scans of the
+// pairs in `pairs` (all sets safe; tiles of FTILE); writes fwd / bwd.  ev
+// (optional, 2 events) are recorded after the tile pass and the collection.
 hipError_t launch_filtered(const float* desc, const float* norm, const uint16_t* hi, const uint16_t* lo,
                            const PairDesc* pairs, int npairs, int64_t ntiles, int64_t rows_total, int64_t cols_total,
                            const FiltBufs& fb, Best2* fwd, Best2* bwd, hipStream_t st, hipEvent_t* ev);

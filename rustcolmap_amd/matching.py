@@ -107,9 +107,8 @@ class FrameMatcher:
         return match_stats(self.ctx)
 
 
-STAT_FIELDS = ("pairs", "filtered_pairs", "distances", "matches", "overflow_items", "exact_ms", "pass1_ms",
-               "threshold_ms", "pass2_ms", "recheck_ms", "select_ms", "host_launch_ms", "host_wait_ms",
-               "host_out_ms")
+STAT_FIELDS = ("pairs", "filtered_pairs", "distances", "matches", "overflow_items", "exact_ms", "tile_ms",
+               "collect_ms", "recheck_ms", "select_ms", "host_launch_ms", "host_wait_ms", "host_out_ms")
 
 
 def match_stats(ctx: MvsContext) -> dict:

@@ -146,11 +146,11 @@ def test_unsafe_sets_take_exact_tiles(ctx):
 
 
 def test_filter_overflow_rows(ctx):
-    """Clusters of near-identical descriptors give rows with more than 8
+    """Clusters of 20 near-identical descriptors give rows with more than 16
     candidates: those are scanned in full."""
     rng = np.random.default_rng(4)
-    base = sift_like(rng, 40)
-    a = np.repeat(base, 12, axis=0) + (rng.standard_normal((480, 128)) * 1e-4).astype(np.float32)
+    base = sift_like(rng, 24)
+    a = np.repeat(base, 20, axis=0) + (rng.standard_normal((480, 128)) * 1e-4).astype(np.float32)
     a = np.abs(a).astype(np.float32)
     b = np.concatenate([a[rng.permutation(480)[:200]], sift_like(rng, 100)])
     b = np.ascontiguousarray(b + (rng.standard_normal(b.shape) * 1e-5).astype(np.float32))

@@ -1,6 +1,8 @@
 """CPU model of the filtered descriptor search (no GPU): the bf16 hi/lo
-split, the product estimate, the per-row threshold T = second smallest upper
-bound and the candidate sets L <= T.  Checks on SIFT-shaped and wide-range
+split, the product estimate, the per-tile records (two smallest upper bounds,
+a lower bound of the tile's L, the mask of elements with L <= the tile's
+second smallest U), the row threshold T = second smallest upper bound and the
+candidate sets.  Checks on SIFT-shaped and wide-range
 sets that (1) the estimate stays within the assumed bound 2^-11 (|a|^2+|b|^2)
 of the reference's f32 sum with a wide margin, and (2) the exact nearest /
 second-nearest over the candidates equal the full scan's, for every row and
@@ -53,14 +55,35 @@ def stats_full(S):
     return bi, b1, b2
 
 
-def stats_filtered(S, t, A):
-    U = (np.float32(1 + EPS_REL) * t - np.float32(2) * ((t - A) / np.float32(2))).astype(np.float32)
-    L = (np.float32(1 - EPS_REL) * t - np.float32(2) * ((t - A) / np.float32(2))).astype(np.float32)
-    T = np.partition(U, 1, axis=1)[:, 1] if U.shape[1] > 1 else np.full(len(U), np.inf, np.float32)
+def stats_filtered(S, t, A, tile=128, cap=16):
+    """The device search, row by row: per tile of `tile` columns the two
+    smallest upper bounds U, a lower bound lb of the tile's L values and the
+    mask of elements with L <= the tile's u2; T = the second smallest U of the
+    row; candidates = the masked elements of the tiles with lb <= T (more
+    than `cap`: the row is scanned in full)."""
+    eps = np.float32(EPS_REL)
+    tU = (np.float32(1) + eps) * t
+    U = (tU - np.float32(2) * ((t - A) / np.float32(2))).astype(np.float32)
+    kb = np.float32(2) * eps / (np.float32(1) + eps)
+    L = (U - kb * tU).astype(np.float32)
     out = []
     ncand = []
+    nfull = 0
+    n = S.shape[1]
     for i in range(len(S)):
-        c = np.nonzero(L[i] <= T[i])[0]
+        tiles = []
+        for c0 in range(0, n, tile):
+            u = np.sort(U[i, c0:c0 + tile])
+            u1, u2 = u[0], (u[1] if len(u) > 1 else np.float32(np.inf))
+            lb = u1 - kb * tU[i, c0:c0 + tile].max() * np.float32(1 + 2.0 ** -20)
+            assert lb <= L[i, c0:c0 + tile].min()
+            tiles.append((u1, u2, lb, c0 + np.nonzero(L[i, c0:c0 + tile] <= u2)[0]))
+        T = np.sort(np.array([x for tt in tiles for x in tt[:2]], np.float32))[1]
+        c = np.concatenate([tt[3] for tt in tiles if tt[2] <= T]).astype(np.int64)
+        assert set(np.nonzero(L[i] <= T)[0]) <= set(c)
+        if len(c) > cap:
+            nfull += 1
+            c = np.arange(n)
         ncand.append(len(c))
         v = S[i, c]
         k = np.lexsort((c, v))[0]
@@ -68,7 +91,7 @@ def stats_filtered(S, t, A):
         b2 = np.partition(v, 1)[1] if len(c) > 1 else np.inf
         out.append((bi, b1, b2))
     bi, b1, b2 = map(np.array, zip(*out))
-    return bi, b1, b2, np.array(ncand)
+    return bi, b1, b2, np.array(ncand), nfull
 
 
 def _sets():
@@ -78,8 +101,8 @@ def _sets():
     a = (rng.standard_normal((200, 128)) * 2.0 ** rng.integers(-12, 12, (200, 1))).astype(np.float32)
     b = np.concatenate([a[:100] * np.float32(1 + 2.0 ** -10), (rng.standard_normal((120, 128)) * 8).astype(np.float32)])
     yield "wide", a, np.ascontiguousarray(b, np.float32)
-    base = sift_like(rng, 20)
-    c = np.abs(np.repeat(base, 10, axis=0) + (rng.standard_normal((200, 128)) * 1e-4)).astype(np.float32)
+    base = sift_like(rng, 10)
+    c = np.abs(np.repeat(base, 20, axis=0) + (rng.standard_normal((200, 128)) * 1e-4)).astype(np.float32)
     yield "clusters", c, np.ascontiguousarray(c[rng.permutation(200)] + np.float32(1e-5), np.float32)
 
 
@@ -91,9 +114,11 @@ def test_filter_model(name, a, b):
     assert rel.max() < EPS_REL / 8, rel.max()
     for M, tt, AA in ((S, t, A), (S.T.copy(), t.T.copy(), A.T.copy())):
         fi, f1, f2 = stats_full(M)
-        gi, g1, g2, nc = stats_filtered(M, tt, AA)
+        gi, g1, g2, nc, nfull = stats_filtered(M, tt, AA)
         np.testing.assert_array_equal(fi, gi)
         np.testing.assert_array_equal(f1, g1)
         np.testing.assert_array_equal(f2, g2)
         if name == "sift":
-            assert np.median(nc) <= 4
+            assert np.median(nc) <= 4 and nfull <= len(M) // 50
+        if name == "clusters":
+            assert nfull > 0   # the full-scan fallback is exercised
