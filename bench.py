@@ -136,6 +136,9 @@ def main():
     ap.add_argument("--conf", type=float, default=0.5, help="confidence threshold of the learned initialisation")
     ap.add_argument("--conf-sweep", type=lambda v: [float(x) for x in v.split(",") if x], default=[0.3, 0.1],
                     help="further confidence thresholds reported beside --conf")
+    ap.add_argument("--replace-sweep", type=lambda v: [float(x) for x in v.split(",") if x], default=[],
+                    help="confidence thresholds at which the learned run is repeated with an accepted proposal "
+                         "replacing the plain init sweep (instead of joining it as a 49th hypothesis)")
     ap.add_argument("--batched", action="store_true",
                     help="uneven batch: 64 views of mixed size, K uniform in [4, 12]; batched graph schedule vs "
                          "per-view launches (prints its own JSON line)")
@@ -367,9 +370,12 @@ def learned_bench(args):
     vps_short = len(views) / ((ms_prepare + ms_short + ms_fuse_plain) / 1000.0)
     sweep = {}
     first = None
-    for conf in [args.conf] + [c for c in args.conf_sweep if c != args.conf]:
+    runs = [(c, True) for c in [args.conf] + [c for c in args.conf_sweep if c != args.conf]]
+    runs += [(c, False) for c in args.replace_sweep]
+    replace_sweep = {}
+    for conf, extra in runs:
         (_, acc), ms_learned = timed(lambda: estimate_depth_learned(ctx, net, jobs, short, vol_k=4, conf_thr=conf,
-                                                                    download=False), steps)
+                                                                    download=False, extra=extra), steps)
         st = ctx.stats()
         learned_d = {r: ctx.get_depth(r).depth for r in range(len(views))}
         pts_learned, ms_fuse_learned = timed(fuse, 1)
@@ -381,8 +387,8 @@ def learned_bench(args):
                "ms_volume": round(st.ms_learned_volume, 2), "ms_net": round(st.ms_learned_net, 2),
                "ms_init_total": round(st.ms_init, 2), "ms_passes": round(st.ms_pass, 2),
                "ms_fusion": round(ms_fuse_learned, 2), "fused_points": int(len(pts_learned))}
-        log("conf >= %.2f: %s" % (conf, row))
-        sweep["%.2f" % conf] = row
+        log("conf >= %.2f%s: %s" % (conf, "" if extra else " (proposal replaces the sweep)", row))
+        (sweep if extra else replace_sweep)["%.2f" % conf] = row
         if first is None:
             first = row
     out = {"metric": "views/s, learned init + 2 sweeps vs plain 8 sweeps (%s)" % name,
@@ -401,6 +407,7 @@ def learned_bench(args):
                               "ms_stereo": round(ms_plain, 2), "ms_census": round(ms_prepare, 2),
                               "ms_fusion": round(ms_fuse_plain, 2), "fused_points": int(len(pts_plain))},
            "learned_by_confidence": sweep,
+           "learned_replace_mode_by_confidence": replace_sweep,
            "weights": {k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in meta.items()}}
     print(json.dumps(out), flush=True)
 

@@ -333,15 +333,25 @@ int mvs_net_debug_grads(mvs_net *net, const float *input, const float *target, i
                         int32_t h, int32_t w, float *grads, float *pred, float *loss4);
 
 /* estimate_depth with the learned initialisation: coarse sweep volume over
- * the first vol_k sources, network, then per pixel the plane proposed at its
- * coarse pixel (passing through the predicted depth on that pixel's ray)
- * where confidence >= conf_thr and both rays face it (n . r >= 0.1), else the
- * plain plane sweep; then 2 * opts->sweeps
- * passes and the gate as in mvs_estimate_depth_batch.  Results stay
- * resident and are copied to the jobs' host pointers where non-NULL.
- * n_accepted (optional): pixels that started from a proposed plane. */
+ * the first vol_k sources, network, then the plain plane sweep of every
+ * pixel with one more hypothesis where the network is confident
+ * (confidence >= conf_thr and both rays face the plane, n . r >= 0.1): the
+ * plane proposed at the pixel's coarse pixel, passing through the predicted
+ * depth on that pixel's ray, kept when its plane_cost is strictly lower;
+ * then 2 * opts->sweeps passes and the gate as in
+ * mvs_estimate_depth_batch.  Results stay resident and are copied to the
+ * jobs' host pointers where non-NULL.  n_accepted (optional): pixels with
+ * an accepted proposal. */
 int mvs_estimate_depth_learned(mvs_ctx *ctx, mvs_net *net, const mvs_job_t *jobs, int32_t njobs,
                                const mvs_options_t *opts, int32_t vol_k, float conf_thr, int64_t *n_accepted);
+/* mode 1 (what mvs_estimate_depth_learned runs): an accepted proposal is
+ * an extra hypothesis -- every pixel runs the plain sweep and the proposal
+ * replaces its result only when its plane_cost is strictly lower (a 49th
+ * hypothesis); mode 0: an accepted proposal replaces the plain sweep at its
+ * pixel (no sweep there). */
+int mvs_estimate_depth_learned_ex(mvs_ctx *ctx, mvs_net *net, const mvs_job_t *jobs, int32_t njobs,
+                                  const mvs_options_t *opts, int32_t vol_k, float conf_thr, int32_t mode,
+                                  int64_t *n_accepted);
 
 /* ---- SfM descriptor matching (sfm-lite/src/matching.rs:22-98) ----------
  * Descriptors are 128 f32 each, row-major (SiftFrame.desc, sift.rs).  A

@@ -121,6 +121,8 @@ _SIGS = {
                                       _P]),
     "mvs_estimate_depth_learned": (C.c_int, [_P, _P, C.POINTER(MvsJobC), C.c_int32, C.POINTER(MvsOptionsC),
                                              C.c_int32, C.c_float, C.POINTER(C.c_int64)]),
+    "mvs_estimate_depth_learned_ex": (C.c_int, [_P, _P, C.POINTER(MvsJobC), C.c_int32, C.POINTER(MvsOptionsC),
+                                                C.c_int32, C.c_float, C.c_int32, C.POINTER(C.c_int64)]),
     "mvs_match_pair": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, C.c_double, _P, _P, C.POINTER(C.c_int64)]),
     "mvs_upload_descriptors": (C.c_int, [_P, _P, _P, C.c_int32]),
     "mvs_match_frames": (C.c_int, [_P, _P, C.c_int32, C.c_double, _P, _P, _P]),

@@ -1270,6 +1270,7 @@ struct mvs_net {
     DBuf<float> preds;
     DBuf<u64> cnt;
     DBuf<u64> acc_list;       // accepted proposals (job, y, x) of the learned initialisation
+    DBuf<float4> acc_plane;   // their planes (extra-hypothesis mode)
 };
 
 static inline int round16(int v) { return (v + 15) & ~15; }
@@ -1291,6 +1292,7 @@ static void release_net(mvs_net* nt) {
     nt->preds.release();
     nt->cnt.release();
     nt->acc_list.release();
+    nt->acc_plane.release();
 }
 
 static void orphan_net(mvs_net* nt) {
@@ -1642,7 +1644,13 @@ int mvs_net_debug_grads(mvs_net* nt, const float* input, const float* target, in
 
 int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, int32_t njobs,
                                const mvs_options_t* opts, int32_t vol_k, float conf_thr, int64_t* n_accepted) {
-    if (!c || !nt || !jobs || njobs < 1 || !opts || vol_k < 1)
+    return mvs_estimate_depth_learned_ex(c, nt, jobs, njobs, opts, vol_k, conf_thr, 1, n_accepted);
+}
+
+int mvs_estimate_depth_learned_ex(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, int32_t njobs,
+                                  const mvs_options_t* opts, int32_t vol_k, float conf_thr, int32_t mode,
+                                  int64_t* n_accepted) {
+    if (!c || !nt || !jobs || njobs < 1 || !opts || vol_k < 1 || mode < 0 || mode > 1)
         return fail(MVS_ERR_ARG, "mvs_estimate_depth_learned: bad arguments");
     if (nt->ctx != c) return fail(MVS_ERR_STATE, "mvs_estimate_depth_learned: the network belongs to another context");
     try {
@@ -1729,9 +1737,10 @@ int mvs_estimate_depth_learned(mvs_ctx* c, mvs_net* nt, const mvs_job_t* jobs, i
         HCHK(launch_fill_u64(c->counters.p, counter_words(P), 0ULL, c->stream));
         u64* acc_counter = c->counters.p + valid_base(P) - 4;
         HCHK(nt->acc_list.need((size_t)P.n_pixels));
+        if (mode == 1) HCHK(nt->acc_plane.need((size_t)P.n_pixels));
         HCHK(launch_proposal_init(P.cfg, c->jobs.p, c->srcs.p, c->tiles.p, (int)P.tiles.size(), P.nh,
                                   (const float* const*)nt->ptrs.p, MVS_NET_STRIDE, conf_thr, c->counters.p,
-                                  acc_counter, nt->acc_list.p, c->stream));
+                                  acc_counter, nt->acc_list.p, mode == 1 ? nt->acc_plane.p : nullptr, c->stream));
         HCHK(hipEventRecord(c->ev[3], c->stream));
         for (int p = 0; p < P.run_passes; p++) {
             const float t = (float)p / (float)P.n_passes;

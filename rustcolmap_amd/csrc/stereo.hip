@@ -1059,7 +1059,8 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
                                                                const int4* __restrict__ tiles, int ntiles, int nh,
                                                                const float* const* __restrict__ preds, int stride,
                                                                float conf_thr, u64* __restrict__ counter,
-                                                               u64* __restrict__ accepted, u64* __restrict__ list) {
+                                                               u64* __restrict__ accepted, u64* __restrict__ list,
+                                                               float4* __restrict__ prop) {
     constexpr int PW = TW + 4;
     __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
@@ -1081,6 +1082,7 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
     const int x = tl.y + lx, y = tl.z + ly;
     const bool inside = x < J.w && y < J.h;
     bool accept = false;
+    float4 proposal = make_float4(0.f, 0.f, 0.f, 0.f);
     if (inside) {
         const int c = (ly + 2) * PW + (lx + 2);
         const size_t i = (size_t)y * J.w + x;
@@ -1103,8 +1105,12 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
             const float z = ((1.0f / inv) * ac) / a;
             ++ne;
             ++na;
-            J.plane[0][i] = make_float4(nx, ny, nz, z);
-        } else if (EV < 3) {
+            proposal = make_float4(nx, ny, nz, z);
+            if (!prop) J.plane[0][i] = proposal;
+        }
+        // with `prop` the proposal is an extra hypothesis: every pixel runs
+        // the plain sweep, and proposal_cost_kernel keeps the cheaper plane
+        if (EV < 3 && (!accept || prop)) {
             unsigned ne2 = 0;
             init_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne2, s_tab);
             ne += ne2;
@@ -1116,11 +1122,14 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
     __syncthreads();
     if (threadIdx.x == 0) s_base = s_acc ? atomicAdd(accepted, (u64)s_acc) : 0;
     __syncthreads();
-    if (accept) list[s_base + slot] = ((u64)tl.x << 42) | ((u64)y << 21) | (u64)x;
+    if (accept) {
+        list[s_base + slot] = ((u64)tl.x << 42) | ((u64)y << 21) | (u64)x;
+        if (prop) prop[s_base + slot] = proposal;
+    }
     if (EV >= 3) {
-        // the pixels without an accepted proposal sweep together
+        // the pixels without an accepted proposal (all, with `prop`) sweep together
         unsigned ne2 = 0;
-        init_box_tile<TW, TH>(J, srcs, tl.y, tl.z, nh, !accept, s_cen, s_ray, s_code, ne2, s_tab);
+        init_box_tile<TW, TH>(J, srcs, tl.y, tl.z, nh, !accept || prop != nullptr, s_cen, s_ray, s_code, ne2, s_tab);
         ne += ne2;
     }
     (void)na;
@@ -1133,23 +1142,33 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
 // list (job << 42 | y << 21 | x), plane_cost over all sources with no abort,
 // the reference loop order with clamped global window reads.  The list
 // length is read from the device; a fixed grid strides over it.
+// With `prop` (extra-hypothesis mode) the entry's plane comes from prop[k]
+// and replaces the plain sweep's plane only when its cost is strictly lower,
+// as a 49th hypothesis of the sweep would.
 __global__ __launch_bounds__(256) void proposal_cost_kernel(const JobDesc* __restrict__ jobs,
                                                             const SrcDesc* __restrict__ srcs,
                                                             const u64* __restrict__ list,
-                                                            const u64* __restrict__ count) {
+                                                            const u64* __restrict__ count,
+                                                            const float4* __restrict__ prop) {
     const u64 n = *count;
     for (u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (u64)gridDim.x * blockDim.x) {
         const u64 e = list[k];
         const int j = (int)(e >> 42), y = (int)((e >> 21) & 0x1FFFFF), x = (int)(e & 0x1FFFFF);
         const JobDesc& J = jobs[j];
         const size_t i = (size_t)y * J.w + x;
-        const float4 pl = J.plane[0][i];
+        const float4 pl = prop ? prop[k] : J.plane[0][i];
         const float2 ri = J.rays[i];
         const float a = (pl.x * ri.x + pl.y * ri.y) + pl.z;
         const GlobRef R{J.census, J.rays, x, y, J.w, J.h};
-        const float cst = eval_plane<false>(R, srcs + J.src_off, J.nsrc, pl.x, pl.y, pl.z, pl.w * a,
-                                            __builtin_inff(), nullptr);
-        J.cost[0][i] = fminf(cst, 1.0f);
+        const float cst = fminf(eval_plane<false>(R, srcs + J.src_off, J.nsrc, pl.x, pl.y, pl.z, pl.w * a,
+                                                  __builtin_inff(), nullptr),
+                                1.0f);
+        if (!prop) {
+            J.cost[0][i] = cst;
+        } else if (cst < J.cost[0][i]) {
+            J.plane[0][i] = pl;
+            J.cost[0][i] = cst;
+        }
     }
 }
 
@@ -1340,12 +1359,12 @@ bool proposal_supported(const TileCfg& cfg) {
 
 hipError_t launch_proposal_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles,
                                 int ntiles, int nh, const float* const* preds, int stride, float conf_thr,
-                                u64* counter, u64* accepted, u64* list, hipStream_t st) {
+                                u64* counter, u64* accepted, u64* list, float4* prop, hipStream_t st) {
     bool done = false;
 #define X(W, H, E)                                                                                              \
     if (!done && cfg.tw == W && cfg.th == H && cfg.eval == E) {                                                 \
         hipLaunchKernelGGL((proposal_init_kernel<W, H, E>), dim3(ntiles), dim3(W * H), 0, st, jobs, srcs, tiles, \
-                           ntiles, nh, preds, stride, conf_thr, counter, accepted, list);                       \
+                           ntiles, nh, preds, stride, conf_thr, counter, accepted, list, prop);                 \
         done = true;                                                                                            \
     }
     PROPOSAL_CASES(X)
@@ -1353,7 +1372,8 @@ hipError_t launch_proposal_init(const TileCfg& cfg, const JobDesc* jobs, const S
     if (!done) return hipErrorInvalidValue;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(proposal_cost_kernel, dim3(2048), dim3(256), 0, st, jobs, srcs, list, accepted);
+    hipLaunchKernelGGL(proposal_cost_kernel, dim3(2048), dim3(256), 0, st, jobs, srcs, list, accepted,
+                       (const float4*)prop);
     return hipGetLastError();
 }
 

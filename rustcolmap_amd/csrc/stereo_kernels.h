@@ -47,9 +47,12 @@ hipError_t launch_volume(const JobDesc* jobs, const SrcDesc* srcs, const int4* t
 // launch shapes the learned initialisation kernel is built for (tile
 // origins must be even so a pixel's coarse pixel lies in its tile)
 bool proposal_supported(const TileCfg& cfg);
+// prop == nullptr: an accepted proposal replaces the plain sweep at its
+// pixel; otherwise (room for one float4 per pixel) it is an extra hypothesis
+// that every pixel's plain sweep competes with by cost.
 hipError_t launch_proposal_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles,
                                 int ntiles, int nh, const float* const* preds, int stride, float conf_thr,
-                                u64* counter, u64* accepted, u64* list, hipStream_t st);
+                                u64* counter, u64* accepted, u64* list, float4* prop, hipStream_t st);
 hipError_t launch_assemble(const uint16_t* const* vols, const uint8_t* const* grays, int n, int h, int w, int Hp,
                            int Wp, int stride, int wfull, uint16_t* out, hipStream_t st);
 hipError_t launch_debug_cost(const JobDesc& J, const SrcDesc* S, const float* q, int n, float* cost, int* ham,

@@ -553,9 +553,12 @@ def sweep_volume(ctx: MvsContext, job, stride: int = 1) -> np.ndarray:
 
 
 def estimate_depth_learned(ctx: MvsContext, net: ProposalNet, jobs, opts: MvsOptions, vol_k: int = 4,
-                           conf_thr: float = 0.5, download: bool = True):
+                           conf_thr: float = 0.5, download: bool = True, extra: bool = True):
     """Learned-init stereo over `jobs` ((ref, srcs, zmin, zmax) tuples).
-    Returns ({ref: DepthMap} or None, number of pixels started from a proposal)."""
+    extra=True (default): an accepted proposal is a 49th hypothesis of its
+    pixel's plain sweep, kept only when its cost is lower; extra=False: it
+    replaces the plain sweep at its pixel.
+    Returns ({ref: DepthMap} or None, number of pixels with an accepted proposal)."""
     arr, keep = _jobs_c(jobs)
     out = {}
     if download:
@@ -571,8 +574,9 @@ def estimate_depth_learned(ctx: MvsContext, net: ProposalNet, jobs, opts: MvsOpt
             out[int(jb[0])] = DepthMap(d, c, n)
     acc = C.c_int64()
     oc = opts.to_c()
-    check(ctx._lib.mvs_estimate_depth_learned(ctx._h, net._h, arr, len(jobs), C.byref(oc), int(vol_k),
-                                              C.c_float(conf_thr), C.byref(acc)), "mvs_estimate_depth_learned")
+    check(ctx._lib.mvs_estimate_depth_learned_ex(ctx._h, net._h, arr, len(jobs), C.byref(oc), int(vol_k),
+                                                 C.c_float(conf_thr), 1 if extra else 0, C.byref(acc)),
+          "mvs_estimate_depth_learned_ex")
     return (out if download else None), int(acc.value)
 
 

@@ -9,8 +9,9 @@
   (tolerance: relative norm error 5 %, cosine 0.995 per tensor);
 * Adam training on seeded scenes lowers the loss;
 * with a confidence threshold no pixel reaches, the learned schedule
-  reproduces the plain method bit for bit (the fallback is the plain init);
-  with every proposal accepted it still produces valid, gated maps.
+  reproduces the plain method bit for bit (the fallback is the plain init),
+  in both modes (proposal replaces / proposal as an extra hypothesis); with
+  every proposal accepted both still produce valid, gated maps.
 """
 import numpy as np
 import pytest
@@ -159,12 +160,20 @@ def test_learned_schedule_fallback_is_plain_and_accept_all_runs(ctx):
         d, c, n, _, _ = ov.estimate_depth(r, s, zmin, zmax, opts)
         assert np.array_equal(bits(out[r].depth), bits(d)), r
         assert np.array_equal(bits(out[r].cost), bits(c)), r
-    out, acc = estimate_depth_learned(ctx, net, jobs, opts, vol_k=4, conf_thr=0.0)
-    _say("accepted %d proposals" % acc)
-    assert acc > 0
+    # extra-hypothesis mode: with nothing accepted it is the plain method too
+    out, acc = estimate_depth_learned(ctx, net, jobs, opts, vol_k=4, conf_thr=2.0, extra=True)
+    assert acc == 0
     for r, s, zmin, zmax in jobs:
-        z = out[r].depth
-        assert np.all(np.isfinite(z)) and np.all((z == 0) | ((z > 0.5) & (z < 20.0)))
+        d, c, n, _, _ = ov.estimate_depth(r, s, zmin, zmax, opts)
+        assert np.array_equal(bits(out[r].depth), bits(d)), r
+        assert np.array_equal(bits(out[r].cost), bits(c)), r
+    for extra in (False, True):
+        out, acc = estimate_depth_learned(ctx, net, jobs, opts, vol_k=4, conf_thr=0.0, extra=extra)
+        _say("accepted %d proposals (extra=%s)" % (acc, extra))
+        assert acc > 0
+        for r, s, zmin, zmax in jobs:
+            z = out[r].depth
+            assert np.all(np.isfinite(z)) and np.all((z == 0) | ((z > 0.5) & (z < 20.0)))
     net.close()
 
 
