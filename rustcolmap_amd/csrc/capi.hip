@@ -1983,6 +1983,7 @@ static int match_batch(mvs_ctx* c, MatchSets& S, const int32_t* pairs, int32_t n
     number(ex, mt::TILE, xt, xrp, xcp, xr, xc);
     number(fl, mt::FTILE, ft, frp, fcp, frr, fcc);
     const size_t nex = ex.size(), nfl = fl.size();
+    if (frr + fcc >= (int64_t)0x7fffffff) return fail(MVS_ERR_LIMIT, std::string(who) + ": batch too large");
     HCHK(M.pairs.need((size_t)npairs + nex + nfl + 1));
     HCHK(M.rpart.need((size_t)std::max<int64_t>(xrp, 1)));
     HCHK(M.cpart.need((size_t)std::max<int64_t>(xcp, 1)));
@@ -2004,7 +2005,6 @@ static int match_batch(mvs_ctx* c, MatchSets& S, const int32_t* pairs, int32_t n
     HCHK(M.db.need((size_t)std::max<int64_t>(out, 1)));
     HCHK(M.counts.need((size_t)std::max(npairs, 1)));
     HCHK(M.prefix.need((size_t)std::max(npairs, 1)));
-    if (frr + fcc >= (int64_t)0x7fffffff) return fail(MVS_ERR_ARG, std::string(who) + ": batch too large");
     mt::PairDesc* d_all = M.pairs.p;
     mt::PairDesc* d_ex = M.pairs.p + npairs;
     mt::PairDesc* d_fl = d_ex + nex;

@@ -408,12 +408,11 @@ __device__ __forceinline__ void two_min_row16(float& u1, float& u2) {
 
 constexpr int LROW = 40;   // LDS row stride (u16): 32 components + 8 pad
 
-// PASS 1: one record per row and per column of a 128 x 128 tile: the two
-//         smallest upper bounds U (u1 <= u2) and the elements whose lower
-//         bound L is <= u2 (count, and the first FSLOTS as tile-local index
-//         and L).  The row's global threshold T (its second smallest U over
-//         all tiles) is <= u2 of every tile, so every element with L <= T is
-//         among them.
+// PASS 1: per row and per column of a 128 x 128 tile, the two smallest
+//         upper bounds U (u1 <= u2), a lower bound of the tile's L, and a
+//         128-bit mask of the elements whose L is <= u2.  The row's global
+//         threshold T (its second smallest U over all tiles) is <= u2 of
+//         every tile, so every element with L <= T is in a mask.
 // PASS 3: (check) max over the tile of |exact - estimate| / (|a|^2 + |b|^2)
 //         into *fb.ovf_n as float bits.
 // 4 waves in a 2 x 2 grid, each 64 x 64 = 4 x 4 MFMA 16x16x32 fragments.
