@@ -158,3 +158,25 @@ def test_filter_overflow_rows(ctx):
     _check(ctx, a, b, 0.8, "clusters")
     match_pair(a, b, 0.8, ctx=ctx)
     assert match_stats(ctx)["overflow_items"] > 0
+
+
+def test_match_large_sets_and_long_sequence(ctx):
+    """Sets past 2048 (6000 and 5000 descriptors: 47 x 40 filter tiles,
+    ragged) against the oracle; then a 60-frame sequential run (174 pairs)
+    checked on sampled pairs and by symmetry on all of them."""
+    big = synthetic_frames(2, 6000, seed=31, overlap=0.4)
+    n = _check(ctx, big[0], big[1][:5000], 0.75, "6000x5000")
+    assert n > 1000
+    frames = synthetic_frames(60, 2048, seed=32, overlap=0.6)
+    fm = FrameMatcher(ctx, frames)
+    pairs = job_pairs(60, sequential=True)
+    fwd = fm.match(pairs, 0.75)
+    assert fm.stats()["filtered_pairs"] == len(pairs)
+    for p in (0, 85, len(pairs) - 1):
+        i, j = pairs[p]
+        np.testing.assert_array_equal(_as_pairs(fwd[p]), oracle.match_pair(frames[i], frames[j], 0.75))
+    bwd = fm.match([(j, i) for i, j in pairs], 0.75)
+    for f, r in zip(fwd, bwd):
+        sw = np.stack([r.b, r.a], 1)
+        sw = sw[np.argsort(sw[:, 0], kind="stable")]
+        np.testing.assert_array_equal(np.stack([f.a, f.b], 1), sw)
