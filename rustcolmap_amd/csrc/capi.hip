@@ -129,12 +129,16 @@ struct GraphCache {
 struct MatchSets {
     DBuf<float> desc;                  // [total][128]
     DBuf<float> norm;                  // squared norms
-    DBuf<uint16_t> hi, lo;             // bf16 split
+    DBuf<uint16_t> split;              // bf16 split: hi [rows][128], then lo [rows][128]
+    size_t rows = 0;
+    const uint16_t* hi() const { return split.p; }
+    const uint16_t* lo() const { return split.p + rows * mt::DIM; }
     std::vector<int64_t> off;          // first row of each set
     std::vector<int32_t> n;            // rows per set
     std::vector<int> safe;             // set may use the filtered search
     void release() {
-        desc.release(); norm.release(); hi.release(); lo.release();
+        desc.release(); norm.release(); split.release();
+        rows = 0;
         off.clear(); n.clear(); safe.clear();
     }
 };
@@ -1912,8 +1916,8 @@ static int match_prepare(mvs_ctx* c, MatchSets& S, const float* desc, const int3
     const size_t rows = (size_t)std::max<int64_t>(total, 1);
     HCHK(S.desc.need(rows * mt::DIM));
     HCHK(S.norm.need(rows));
-    HCHK(S.hi.need(rows * mt::DIM));
-    HCHK(S.lo.need(rows * mt::DIM));
+    HCHK(S.split.need(2 * rows * mt::DIM));
+    S.rows = rows;
     HCHK(M.set_off.need((size_t)std::max(nsets, 1)));
     HCHK(M.bad.need((size_t)std::max(nsets, 1)));
     hipStream_t st = c->stream;
@@ -1923,7 +1927,8 @@ static int match_prepare(mvs_ctx* c, MatchSets& S, const float* desc, const int3
         HCHK(hipMemcpyAsync(M.set_off.p, S.off.data(), sizeof(int64_t) * (size_t)nsets, hipMemcpyHostToDevice, st));
         HCHK(hipMemsetAsync(M.bad.p, 0, sizeof(int) * (size_t)nsets, st));
     }
-    HCHK(mt::launch_prep(S.desc.p, total, M.set_off.p, nsets, S.norm.p, S.hi.p, S.lo.p, M.bad.p, st));
+    HCHK(mt::launch_prep(S.desc.p, total, M.set_off.p, nsets, S.norm.p, S.split.p, S.split.p + rows * mt::DIM,
+                         M.bad.p, st));
     std::vector<int> bad((size_t)nsets);
     if (nsets > 0) HCHK(hipMemcpyAsync(bad.data(), M.bad.p, sizeof(int) * (size_t)nsets, hipMemcpyDeviceToHost, st));
     HCHK(hipStreamSynchronize(st));
@@ -2022,7 +2027,7 @@ static int match_batch(mvs_ctx* c, MatchSets& S, const int32_t* pairs, int32_t n
     HCHK(mt::launch_dist_tiles(S.desc.p, d_ex, (int)nex, xt, M.rpart.p, M.cpart.p, st));
     HCHK(mt::launch_merge_partials(d_ex, (int)nex, xr, xc, M.rpart.p, M.cpart.p, M.fwd.p, M.bwd.p, st));
     HCHK(hipEventRecord(c->ev[1], st));
-    HCHK(mt::launch_filtered(S.desc.p, S.norm.p, S.hi.p, S.lo.p, d_fl, (int)nfl, ft, frr, fcc, fb, M.fwd.p, M.bwd.p,
+    HCHK(mt::launch_filtered(S.desc.p, S.norm.p, S.hi(), S.lo(), d_fl, (int)nfl, ft, frr, fcc, fb, M.fwd.p, M.bwd.p,
                              st, c->ev + 2));
     if (ft <= 0)
         for (int e = 2; e < 4; e++) HCHK(hipEventRecord(c->ev[e], st));
@@ -2170,7 +2175,7 @@ int mvs_debug_match_bound(mvs_ctx* c, const float* desc_a, int32_t na, const flo
         HCHK(hipMemsetAsync(M.ovf_n.p, 0, sizeof(uint32_t), c->stream));
         mt::FiltBufs fb = {};
         fb.ovf_n = M.ovf_n.p;
-        HCHK(mt::launch_filter_check(M.tmp.desc.p, M.tmp.norm.p, M.tmp.hi.p, M.tmp.lo.p, M.pairs.p, 1,
+        HCHK(mt::launch_filter_check(M.tmp.desc.p, M.tmp.norm.p, M.tmp.hi(), M.tmp.lo(), M.pairs.p, 1,
                                      (int64_t)d.ta * d.tb, fb, c->stream));
         uint32_t bitsv = 0;
         HCHK(hipMemcpyAsync(&bitsv, M.ovf_n.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));

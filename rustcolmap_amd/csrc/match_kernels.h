@@ -78,6 +78,8 @@ struct FiltBufs {
 // MFMA tile pass, candidate collection, recheck and overflow scans of the
 // pairs in `pairs` (all sets safe; tiles of FTILE); writes fwd / bwd.  ev
 // (optional, 2 events) are recorded after the tile pass and the collection.
+// hi and lo are the two halves of one allocation (the tile kernel addresses
+// lo as an offset from hi).
 hipError_t launch_filtered(const float* desc, const float* norm, const uint16_t* hi, const uint16_t* lo,
                            const PairDesc* pairs, int npairs, int64_t ntiles, int64_t rows_total, int64_t cols_total,
                            const FiltBufs& fb, Best2* fwd, Best2* bwd, hipStream_t st, hipEvent_t* ev);
