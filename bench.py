@@ -527,6 +527,7 @@ def match_bench(args):
                                                                         "host_out_ms")}
     host["c_call_ms"] = round(float(np.mean(call_ms)), 4)
     host["python_step_ms"] = [round(x, 3) for x in step_ms]
+    host["wait_out_ms_per_step"] = [(round(x["host_wait_ms"], 3), round(x["host_out_ms"], 3)) for x in stages]
     filtered = st["filtered_pairs"] == st["pairs"]
     if filtered:
         # dominant kernel: the MFMA tile kernel, one launch per step; 3 bf16

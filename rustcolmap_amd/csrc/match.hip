@@ -790,6 +790,26 @@ __device__ __forceinline__ float exact_dist(const float* __restrict__ x, const f
     return s;
 }
 
+// The same with one descriptor in LDS: (own - other)^2 has the bits of
+// (other - own)^2, so which side is a does not matter.
+__device__ __forceinline__ float exact_dist_own(const float4* __restrict__ own, const float* __restrict__ other) {
+    const float4* y4 = reinterpret_cast<const float4*>(other);
+    float s = 0.0f;
+#pragma unroll 4
+    for (int k = 0; k < DIM / 4; ++k) {
+        const float4 a = own[k], b = y4[k];
+        float d = a.x - b.x;
+        s = s + d * d;
+        d = a.y - b.y;
+        s = s + d * d;
+        d = a.z - b.z;
+        s = s + d * d;
+        d = a.w - b.w;
+        s = s + d * d;
+    }
+    return s;
+}
+
 // 8 lanes per launch-local row or column: lane s rechecks candidates s and
 // s + 8; candidates may come in any order (the merge is order-free).
 __global__ __launch_bounds__(256) void filt_recheck_kernel(const float* __restrict__ desc,
@@ -797,9 +817,12 @@ __global__ __launch_bounds__(256) void filt_recheck_kernel(const float* __restri
                                                            int64_t rows_total, int64_t cols_total, FiltBufs fb,
                                                            Best2* __restrict__ fwd, Best2* __restrict__ bwd) {
     static_assert(CAP == 16, "8 lanes, two candidate slots each");
+    // the item's own descriptor, staged once for its 8 lanes (rows padded by
+    // one float4 so the 8 items of a wave read different banks)
+    __shared__ float4 s_own[32][DIM / 4 + 1];
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t item = g >> 3;
-    const int s = (int)(g & 7);
+    const int s = (int)(g & 7), li = (int)(threadIdx.x >> 3);
     const bool valid = item < rows_total + cols_total;
     const bool row = item < rows_total;
     const int64_t q = row ? item : item - rows_total;
@@ -812,11 +835,16 @@ __global__ __launch_bounds__(256) void filt_recheck_kernel(const float* __restri
         P = pairs[find_pair_by(pairs, npairs, q, !row)];
         local = q - (row ? P.lrow : P.lcol);
         cnt = row ? fb.rcnt[q] : fb.ccnt[q];
+        const float4* own = reinterpret_cast<const float4*>(desc + (row ? P.a + local : P.b + local) * DIM);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s_own[li][s + 8 * k] = own[s + 8 * k];
+    }
+    __syncthreads();
+    if (valid) {
         for (uint32_t k = (uint32_t)s; cnt <= (uint32_t)CAP && k < cnt; k += 8) {
             const uint32_t other = row ? fb.rcand[(size_t)q * CAP + k] : fb.ccand[(size_t)q * CAP + k];
-            const float* x = desc + (row ? P.a + local : P.a + other) * DIM;
-            const float* y = desc + (row ? P.b + other : P.b + local) * DIM;
-            merge(b, bi, sec, exact_dist(x, y), other, INFINITY);
+            const float* y = desc + (row ? P.b + other : P.a + other) * DIM;
+            merge(b, bi, sec, exact_dist_own(s_own[li], y), other, INFINITY);
         }
     }
     merge_lanes(b, bi, sec, 1);

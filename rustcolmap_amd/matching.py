@@ -90,8 +90,12 @@ class FrameMatcher:
         pr = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
         off = self.slots(pairs)
         tot = max(int(off[-1]), 1)
-        oa = np.empty(tot, np.uint32) if download else None
-        ob = np.empty(tot, np.uint32) if download else None
+        if download and (getattr(self, "_oa", None) is None or len(self._oa) < tot):
+            # slot arrays kept across calls: no fresh pages to fault in per batch
+            self._oa = np.empty(tot, np.uint32)
+            self._ob = np.empty(tot, np.uint32)
+        oa = self._oa if download else None
+        ob = self._ob if download else None
         cnt = np.empty(max(len(pr), 1), np.int64)
         t0 = time.perf_counter()
         check(self.ctx._lib.mvs_match_frames(self.ctx._h, ptr(pr), len(pr), float(ratio), ptr(oa), ptr(ob),
