@@ -17,7 +17,10 @@ is the copy-calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE of the same kernel
 on this workload, read from the committed summary under profiles/ (null when
 the tuned kernel is not in it).  `--graph` times graph replays instead.
 Fusion is timed for fused points/s and views/s.  The CPU baseline is the
-oracle on a bounded row band of one view, on the host's cores.
+oracle on a bounded row band of one view, on the host's cores.  The same
+line carries `sfm_matching`: the second hot path (SfM descriptor matching,
+20 frames x 2048 descriptors, all 190 pairs) measured as `--match` does,
+with its own roofline and CPU baseline (`--no-match` skips it).
 """
 from __future__ import annotations
 
@@ -116,8 +119,8 @@ def main():
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="room1080")
     ap.add_argument("--views", type=int, default=0, help="use only the first N views (0 = all)")
     ap.add_argument("--k", type=int, default=8)
@@ -148,6 +151,8 @@ def main():
     ap.add_argument("--match-frames", type=int, default=20)
     ap.add_argument("--match-features", type=int, default=2048)
     ap.add_argument("--match-exact", action="store_true", help="exact f32 tiles for every pair (no filter)")
+    ap.add_argument("--no-match", action="store_true",
+                    help="default run: skip the descriptor-matching block (sfm_matching)")
     ap.add_argument("--profile", action="store_true",
                     help="one directly launched batch plus a 1 GiB device copy (for counter calibration), no JSON")
     args = ap.parse_args()
@@ -207,9 +212,12 @@ def main():
         ctx.estimate_depth_batch(jobs, opts, mode=mode, download=False)
     torch.cuda.synchronize()
     acc = {"ms_init": 0.0, "ms_pass": 0.0, "ms_gate": 0.0, "n_eval_init": 0, "n_eval_pass": 0}
+    step_walls = []
     t1 = time.perf_counter()
     for _ in range(args.steps):
+        t2 = time.perf_counter()
         ctx.estimate_depth_batch(jobs, opts, mode=mode, download=False)
+        step_walls.append(1000.0 * (time.perf_counter() - t2))
         if not args.graph:
             st = ctx.stats()
             for k_ in acc:
@@ -286,6 +294,7 @@ def main():
                      "bytes_per_launch": int(pass_bytes / n_passes), "avg_launch_ms": round(avg_pass_ms, 4),
                      "launch_time_source": "graph-free run after the timed steps" if args.graph
                      else "HIP events around the pass launches of the timed steps"},
+        "step_wall_ms": [round(x, 2) for x in step_walls],
         "stages_ms": {"census_rays": round(ms_prepare, 3), "init": round(ms_init, 3),
                       "passes": round(ms_pass, 3), "gate": round(ms_gate, 3),
                       "fusion": round(sf.ms_fuse, 3)},
@@ -297,6 +306,13 @@ def main():
         if ms_init > 0 else None,
         "cpu_baseline": cpu,
     }
+    if not args.no_match:
+        # the second hot path, SfM descriptor matching, measured in the same run
+        margs = argparse.Namespace(**{**vars(args), "steps": 30, "warmup": 3, "match_exact": False,
+                                      "profile": False})
+        m = match_bench(margs, emit=False)
+        out["sfm_matching"] = {k: m[k] for k in ("metric", "value", "unit", "ms_per_step", "config", "matches",
+                                                 "stage_ms", "roofline", "cpu_baseline") if k in m}
     print(json.dumps(out), flush=True)
 
 
@@ -468,7 +484,7 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0             # dense bf16 MFMA
 MATCH_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "match_filt_tile_match20x2048_traffic.txt")
 
 
-def match_bench(args):
+def match_bench(args, emit=True):
     """Descriptor matching of run_sfm (sfm-lite/src/lib.rs:197-224):
     `--match-frames` frames of `--match-features` SIFT-shaped descriptors
     (synthetic sweep, 50 % overlap between neighbours), every pair i < j
@@ -568,7 +584,10 @@ def match_bench(args):
                         "flops_per_launch": int(flops), "avg_launch_ms": round(launch_ms, 4), "kernel": kname}}
     if not args.no_cpu:
         out["cpu_baseline"] = match_cpu_baseline(frames, pairs, args.cpu_seconds)
-    print(json.dumps(out), flush=True)
+    ctx.close()
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def match_cpu_baseline(frames, pairs, seconds):
