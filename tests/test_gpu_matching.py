@@ -94,10 +94,30 @@ def test_match_frames_batch(ctx):
 
 
 def test_match_pair_bad_args(ctx):
-    from rustcolmap_amd._lib import MvsError
+    import ctypes as C
+    from rustcolmap_amd._lib import MvsError, check
     fm = FrameMatcher(ctx, synthetic_frames(2, 10, seed=1))
     with pytest.raises(MvsError):
         fm.match([(0, 2)])
+    with pytest.raises(MvsError):
+        fm.match([(-1, 0)])
+    with pytest.raises(MvsError):
+        check(ctx._lib.mvs_match_set_mode(ctx._h, 2), "mvs_match_set_mode")
+    counts = np.array([3, -1], np.int32)
+    d = np.zeros((3, 128), np.float32)
+    with pytest.raises(MvsError):
+        check(ctx._lib.mvs_upload_descriptors(ctx._h, d.ctypes.data_as(C.c_void_p),
+                                              counts.ctypes.data_as(C.c_void_p), 2), "mvs_upload_descriptors")
+    tiny = synthetic_frames(2, 8, seed=2)
+    tiny[0] = tiny[0].copy()
+    tiny[0][0, 0] = 1e-30   # outside [2^-40, 2^40]: the bound check refuses the set
+    with pytest.raises(MvsError):
+        bound_check(ctx, tiny[0], tiny[1])
+    # the context still works after the errors
+    fm2 = FrameMatcher(ctx, synthetic_frames(2, 100, seed=3))
+    res = fm2.match([(0, 1)], 0.75)
+    fr = synthetic_frames(2, 100, seed=3)
+    np.testing.assert_array_equal(_as_pairs(res[0]), oracle.match_pair(fr[0], fr[1], 0.75))
 
 
 def test_match_fullsize_2048(ctx):
