@@ -132,7 +132,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--graph", action="store_true", help="time HIP graph replays (stage times from an extra run)")
     ap.add_argument("--learned", action="store_true",
-                    help="learned initialisation + 2 sweeps against the plain 8-sweep judge (own JSON line)")
+                    help="learned initialisation + 2 sweeps against the plain 8-sweep run (own JSON line)")
     ap.add_argument("--combined", action="store_true",
                     help="100 views 1080p on a 360-degree orbit, K=10: learned init + 2 sweeps + fusion (own JSON line)")
     ap.add_argument("--weights", default=None, help="proposal network weights (.npz)")
@@ -328,7 +328,7 @@ def _agree(ref, other, tol=0.01):
 
 
 def learned_bench(args):
-    """Plain 8-sweep stereo (the judge) vs the learned initialisation
+    """Plain 8-sweep stereo (the reference run) vs the learned initialisation
     followed by 2 sweeps, on the same views; fusion after each."""
     import torch
     from rustcolmap_amd.learned import DEFAULT_WEIGHTS, load_weights
@@ -370,9 +370,9 @@ def learned_bench(args):
         return ctx.fuse_resident([True] * len(views), order, args.cell, plain)
 
     steps = max(1, args.steps)
-    # plain judge
+    # plain 8-sweep reference run
     _, ms_plain = timed(lambda: ctx.estimate_depth_batch(jobs, plain, mode=0, download=False), steps)
-    judge = {r: ctx.get_depth(r).depth for r in range(len(views))}
+    ref8 = {r: ctx.get_depth(r).depth for r in range(len(views))}
     pts_plain, ms_fuse_plain = timed(fuse, 1)
     # plain method with 2 sweeps, no network
     _, ms_short = timed(lambda: ctx.estimate_depth_batch(jobs, short, mode=0, download=False), steps)
@@ -380,7 +380,7 @@ def learned_bench(args):
     # learned initialisation + 2 sweeps, at each confidence threshold
     net = ProposalNet(ctx, seed=0)
     meta = load_weights(net, args.weights or DEFAULT_WEIGHTS)
-    agree_s = float(np.mean([_agree(judge[r], short_d[r]) for r in judge]))
+    agree_s = float(np.mean([_agree(ref8[r], short_d[r]) for r in ref8]))
     valid = lambda d: float(np.mean([(d[r] > 0).mean() for r in d]))
     vps_plain = len(views) / ((ms_prepare + ms_plain + ms_fuse_plain) / 1000.0)
     vps_short = len(views) / ((ms_prepare + ms_short + ms_fuse_plain) / 1000.0)
@@ -395,7 +395,7 @@ def learned_bench(args):
         st = ctx.stats()
         learned_d = {r: ctx.get_depth(r).depth for r in range(len(views))}
         pts_learned, ms_fuse_learned = timed(fuse, 1)
-        agree_l = float(np.mean([_agree(judge[r], learned_d[r]) for r in judge]))
+        agree_l = float(np.mean([_agree(ref8[r], learned_d[r]) for r in ref8]))
         vps_learned = len(views) / ((ms_prepare + ms_learned + ms_fuse_learned) / 1000.0)
         row = {"views_per_s": round(vps_learned, 3), "speedup_vs_plain_8": round(vps_learned / vps_plain, 3),
                "agreement_with_plain_8": round(agree_l, 4), "valid_fraction": round(valid(learned_d), 4),
@@ -419,7 +419,7 @@ def learned_bench(args):
                                                     "inverse depth within 1 %"},
            "plain_2_sweeps": {"views_per_s": round(vps_short, 3), "agreement_with_plain_8": round(agree_s, 4),
                               "valid_fraction": round(valid(short_d), 4), "ms_stereo": round(ms_short, 2)},
-           "plain_8_sweeps": {"views_per_s": round(vps_plain, 3), "valid_fraction": round(valid(judge), 4),
+           "plain_8_sweeps": {"views_per_s": round(vps_plain, 3), "valid_fraction": round(valid(ref8), 4),
                               "ms_stereo": round(ms_plain, 2), "ms_census": round(ms_prepare, 2),
                               "ms_fusion": round(ms_fuse_plain, 2), "fused_points": int(len(pts_plain))},
            "learned_by_confidence": sweep,

@@ -10,7 +10,7 @@ sources plus the gray level on the same coarse grid.  The confidence learns
 whether a proposal lies within 2 % of a valid target.  (Counting pixels
 where the plain method finds no valid depth as positives too was tried: it
 accepts almost every proposal and lowers the agreement with the 8-sweep
-judge below that of two plain sweeps.)
+run below that of two plain sweeps.)
 
     python -m rustcolmap_amd.learned --scenes 2000 --steps 3000 --out weights.npz
 """

@@ -1,7 +1,7 @@
 """What the trained proposal network gains over the method without it.
 
 Held-out scenes of the training distribution (seeds 5000-5002, never used
-for training), 8 views of 640x480, K = 4.  The judge is the plain method
+for training), 8 views of 640x480, K = 4.  The reference run is the plain method
 with 8 sweeps; both challengers get 2 sweeps: the plain method, and the
 learned initialisation with the committed weights
 (rustcolmap_amd/weights/proposal_unet.npz) at confidence 0.5, where an
@@ -43,7 +43,7 @@ def test_trained_network_gains_over_plain_two_sweeps(ctx):
         ctx.upload_views(views)
         srcs = select_sources(sc.views, 4)
         jobs = [(r, srcs[r], 1.0, 10.0) for r in range(len(views))]
-        judge = ctx.estimate_depth_batch(jobs, MvsOptions(num_sources=4, sweeps=8), mode=4)
+        ref8 = ctx.estimate_depth_batch(jobs, MvsOptions(num_sources=4, sweeps=8), mode=4)
         short = MvsOptions(num_sources=4, sweeps=2)
         plain2 = ctx.estimate_depth_batch(jobs, short, mode=4)
         evals["plain2"] += ctx.stats().n_eval_init
@@ -55,16 +55,16 @@ def test_trained_network_gains_over_plain_two_sweeps(ctx):
         npix += sum(v.w * v.h for v in views)
         for name, out in (("plain2", plain2), ("learned2", learned2), ("untrained2", untrained2)):
             for r in range(len(views)):
-                stats[name].append((_agree(judge[r].depth, out[r].depth), float((out[r].depth > 0).mean()),
-                                    float((judge[r].depth > 0).mean())))
+                stats[name].append((_agree(ref8[r].depth, out[r].depth), float((out[r].depth > 0).mean()),
+                                    float((ref8[r].depth > 0).mean())))
     summary = {k: (float(np.nanmean([a for a, _, _ in v])), float(np.mean([b for _, b, _ in v]))) for k, v in
                stats.items()}
-    _say("agreement with plain-8 / valid fraction: %s; judge valid %.4f; accepted %.4f; init evaluations %s"
+    _say("agreement with plain-8 / valid fraction: %s; 8-sweep valid %.4f; accepted %.4f; init evaluations %s"
          % (summary, float(np.mean([c for _, _, c in stats["plain2"]])), acc_total / npix, evals))
     assert acc_total > 0
     # every pixel ran the plain sweep, the accepted ones one proposal more ...
     assert evals["learned2"] == evals["plain2"] + acc_total
-    # ... and after two sweeps the result agrees with the 8-sweep judge more
+    # ... and after two sweeps the result agrees with the 8-sweep run more
     # often than the plain method with two sweeps and than with an untrained
     # network
     assert summary["learned2"][0] > summary["plain2"][0]
