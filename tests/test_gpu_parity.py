@@ -392,3 +392,43 @@ def test_run_mvs_matches_oracle_pipeline(ctx, tmp_path):
     assert web == formats.dense_web_json(pts)
     import json
     assert json.loads(web)["n"] <= len(exp)
+
+
+_UNEVEN = {}
+
+
+def _uneven_case():
+    """Six views of three odd sizes (partial tiles on both axes; sources of
+    other sizes than the reference), with the oracle's depth and cost."""
+    if not _UNEVEN:
+        import oracle
+        from rustcolmap_amd.mvs import MvsOptions, ViewData
+        from rustcolmap_amd.scene import SceneSpec, render_scene, select_sources
+        sc = render_scene(SceneSpec(seed=21, n_views=6, sizes=((317, 203), (250, 190), (333, 251)),
+                                    focal_rel=0.86, arc_deg=50.0, radius=4.0, arc_center=-0.5,
+                                    n_objects=2, tex_freq=8.0), device="cpu")
+        views = [ViewData.from_scene_view(v) for v in sc.views]
+        ov = oracle.OracleViews(views)
+        opts = MvsOptions(num_sources=3, sweeps=2)
+        srcs = select_sources(sc.views, 3)
+        jobs = [(r, srcs[r], 1.0, 10.0) for r in range(len(views))]
+        exp = {r: ov.estimate_depth(r, s, zmin, zmax, opts)[:2] for r, s, zmin, zmax in jobs}
+        _UNEVEN.update(views=views, opts=opts, jobs=jobs, exp=exp)
+    return _UNEVEN
+
+
+@pytest.mark.parametrize("shape", ["8x8pw", "16x16w", "32x8w", "16x8r"])
+def test_uneven_sizes_batch_is_exact(ctx, shape, monkeypatch):
+    """One batch over views of different, odd sizes in several launch
+    families, single-pass and persistent schedules alike."""
+    monkeypatch.setenv("MVS_TILE", shape)
+    case = _uneven_case()
+    views = case["views"]
+    ctx.upload_views(views)
+    _say("uneven sizes, launch shape %s" % shape)
+    for mode in (0, 4):
+        out = ctx.estimate_depth_batch(case["jobs"], case["opts"], mode=mode)
+        for r, _, _, _ in case["jobs"]:
+            d, c = case["exp"][r]
+            assert np.array_equal(bits(out[r].depth), bits(d)), (shape, mode, r, views[r].w, views[r].h)
+            assert np.array_equal(bits(out[r].cost), bits(c)), (shape, mode, r)
