@@ -327,6 +327,21 @@ def _agree(ref, other, tol=0.01):
     return float(ok.mean())
 
 
+def _gt_within(depth_maps, gt_maps, tol=0.01):
+    """Against the renderer's depth: the share of valid pixels within `tol`
+    (relative depth) of the truth, and the share of all foreground pixels
+    that are valid and that close."""
+    ok = val = fg = 0
+    for r, d in depth_maps.items():
+        g = gt_maps[r]
+        m = (d > 0) & (g > 0)
+        good = m & (np.abs(d - g) <= tol * np.where(g > 0, g, 1.0))
+        ok += int(good.sum())
+        val += int((d > 0).sum())
+        fg += int((g > 0).sum())
+    return {"of_valid": round(ok / max(val, 1), 4), "of_foreground": round(ok / max(fg, 1), 4)}
+
+
 def learned_bench(args):
     """Plain 8-sweep stereo (the reference run) vs the learned initialisation
     followed by 2 sweeps, on the same views; fusion after each."""
@@ -341,9 +356,10 @@ def learned_bench(args):
     K = 10 if args.combined else args.k
     zmax = 20.0 if args.combined else 10.0
     t0 = time.time()
-    sc = render_scene(spec, device="cuda", with_gt=False)
+    sc = render_scene(spec, device="cuda", with_gt=True)
     log("rendered %d views of %s in %.1fs" % (len(sc.views), name, time.time() - t0))
     views = [ViewData.from_scene_view(v) for v in sc.views]
+    gt = {r: np.asarray(v.gt_depth, np.float32) for r, v in enumerate(sc.views)}
     K = min(K, len(views) - 1)
     srcs = select_sources(sc.views, K)
     jobs = [(r, srcs[r], 1.0, zmax) for r in range(len(views))]
@@ -377,6 +393,19 @@ def learned_bench(args):
     # plain method with 2 sweeps, no network
     _, ms_short = timed(lambda: ctx.estimate_depth_batch(jobs, short, mode=0, download=False), steps)
     short_d = {r: ctx.get_depth(r).depth for r in range(len(views))}
+    # the plain method at other sweep counts: the speed / agreement curve the
+    # learned initialisation has to beat
+    by_sweeps = {}
+    for ns in (1, 3, 4):
+        o = MvsOptions(num_sources=K, sweeps=ns)
+        _, ms_s = timed(lambda: ctx.estimate_depth_batch(jobs, o, mode=0, download=False), steps)
+        d_s = {r: ctx.get_depth(r).depth for r in range(len(views))}
+        by_sweeps[str(ns)] = {
+            "views_per_s": round(len(views) / ((ms_prepare + ms_s + ms_fuse_plain) / 1000.0), 3),
+            "agreement_with_plain_8": round(float(np.mean([_agree(ref8[r], d_s[r]) for r in ref8])), 4),
+            "valid_fraction": round(float(np.mean([(d_s[r] > 0).mean() for r in d_s])), 4),
+            "gt_within_1pct": _gt_within(d_s, gt), "ms_stereo": round(ms_s, 2)}
+        log("plain %d sweeps: %s" % (ns, by_sweeps[str(ns)]))
     # learned initialisation + 2 sweeps, at each confidence threshold
     net = ProposalNet(ctx, seed=0)
     meta = load_weights(net, args.weights or DEFAULT_WEIGHTS)
@@ -402,7 +431,8 @@ def learned_bench(args):
                "accepted_fraction": round(acc / npix, 4), "ms_stereo": round(ms_learned, 2),
                "ms_volume": round(st.ms_learned_volume, 2), "ms_net": round(st.ms_learned_net, 2),
                "ms_init_total": round(st.ms_init, 2), "ms_passes": round(st.ms_pass, 2),
-               "ms_fusion": round(ms_fuse_learned, 2), "fused_points": int(len(pts_learned))}
+               "ms_fusion": round(ms_fuse_learned, 2), "fused_points": int(len(pts_learned)),
+               "gt_within_1pct": _gt_within(learned_d, gt)}
         log("conf >= %.2f%s: %s" % (conf, "" if extra else " (proposal replaces the sweep)", row))
         (sweep if extra else replace_sweep)["%.2f" % conf] = row
         if first is None:
@@ -418,10 +448,15 @@ def learned_bench(args):
                                       "definition": "pixels valid in the 8-sweep plain map that are valid with "
                                                     "inverse depth within 1 %"},
            "plain_2_sweeps": {"views_per_s": round(vps_short, 3), "agreement_with_plain_8": round(agree_s, 4),
-                              "valid_fraction": round(valid(short_d), 4), "ms_stereo": round(ms_short, 2)},
+                              "valid_fraction": round(valid(short_d), 4), "ms_stereo": round(ms_short, 2),
+                              "gt_within_1pct": _gt_within(short_d, gt)},
            "plain_8_sweeps": {"views_per_s": round(vps_plain, 3), "valid_fraction": round(valid(ref8), 4),
                               "ms_stereo": round(ms_plain, 2), "ms_census": round(ms_prepare, 2),
-                              "ms_fusion": round(ms_fuse_plain, 2), "fused_points": int(len(pts_plain))},
+                              "ms_fusion": round(ms_fuse_plain, 2), "fused_points": int(len(pts_plain)),
+                              "gt_within_1pct": _gt_within(ref8, gt)},
+           "plain_by_sweeps": by_sweeps,
+           "gt_definition": "renderer depth: share of valid pixels within 1 % relative depth (of_valid), share of "
+                            "foreground pixels valid and that close (of_foreground)",
            "learned_by_confidence": sweep,
            "learned_replace_mode_by_confidence": replace_sweep,
            "weights": {k: (v.tolist() if hasattr(v, "tolist") else v) for k, v in meta.items()}}
