@@ -2121,7 +2121,7 @@ int mvs_match_pair(mvs_ctx* c, const float* desc_a, int32_t na, const float* des
     if (int r = set_device(c)) return r;
     try {
         // the two sets side by side in one host buffer
-        std::vector<float> both((size_t)(na + nb) * mt::DIM);
+        std::vector<float> both(((size_t)na + (size_t)nb) * mt::DIM);
         if (na > 0) memcpy(both.data(), desc_a, sizeof(float) * mt::DIM * (size_t)na);
         if (nb > 0) memcpy(both.data() + (size_t)na * mt::DIM, desc_b, sizeof(float) * mt::DIM * (size_t)nb);
         const int32_t counts[2] = {na, nb};
@@ -2155,7 +2155,7 @@ int mvs_debug_match_bound(mvs_ctx* c, const float* desc_a, int32_t na, const flo
     if (int r = set_device(c)) return r;
     MatchState& M = c->match;
     try {
-        std::vector<float> both((size_t)(na + nb) * mt::DIM);
+        std::vector<float> both(((size_t)na + (size_t)nb) * mt::DIM);
         memcpy(both.data(), desc_a, sizeof(float) * mt::DIM * (size_t)na);
         memcpy(both.data() + (size_t)na * mt::DIM, desc_b, sizeof(float) * mt::DIM * (size_t)nb);
         const int32_t counts[2] = {na, nb};
