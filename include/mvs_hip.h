@@ -346,8 +346,9 @@ int mvs_estimate_depth_learned(mvs_ctx *ctx, mvs_net *net, const mvs_job_t *jobs
                                const mvs_options_t *opts, int32_t vol_k, float conf_thr, int64_t *n_accepted);
 /* mode 1 (what mvs_estimate_depth_learned runs): an accepted proposal is
  * an extra hypothesis -- every pixel runs the plain sweep and the proposal
- * replaces its result only when its plane_cost is strictly lower (a 49th
- * hypothesis); mode 0: an accepted proposal replaces the plain sweep at its
+ * replaces its result only when its plane_cost (in full, without the
+ * sweep's early abort) is strictly lower (a 49th hypothesis); mode 0: an
+ * accepted proposal replaces the plain sweep at its
  * pixel (no sweep there). */
 int mvs_estimate_depth_learned_ex(mvs_ctx *ctx, mvs_net *net, const mvs_job_t *jobs, int32_t njobs,
                                   const mvs_options_t *opts, int32_t vol_k, float conf_thr, int32_t mode,

@@ -1143,8 +1143,10 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
 // the reference loop order with clamped global window reads.  The list
 // length is read from the device; a fixed grid strides over it.
 // With `prop` (extra-hypothesis mode) the entry's plane comes from prop[k]
-// and replaces the plain sweep's plane only when its cost is strictly lower,
-// as a 49th hypothesis of the sweep would.
+// and replaces the plain sweep's plane only when its cost is strictly lower:
+// a 49th hypothesis, costed in full (the sweep's early abort does not apply
+// to it, so a proposal whose partial mean once passed the sweep's threshold
+// still competes with its final cost).
 __global__ __launch_bounds__(256) void proposal_cost_kernel(const JobDesc* __restrict__ jobs,
                                                             const SrcDesc* __restrict__ srcs,
                                                             const u64* __restrict__ list,
