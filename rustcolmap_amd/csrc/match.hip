@@ -395,15 +395,31 @@ __device__ __forceinline__ void two_min_merge(float& u1, float& u2, float v1, fl
 // row_half_mirror, 0x140 = row_mirror.  The four steps pair every lane with
 // each of the other 15 through the merge tree.
 template <int CTRL>
-__device__ __forceinline__ float dpp_f(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+__device__ __forceinline__ int dpp_i(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, true);
+}
+
+// two_min_merge with the partner lane's pair under DPP pattern CTRL, on the
+// bit patterns: the upper bounds are >= +0 or +inf (U >= the exact distance
+// >= 0), where signed-integer order is float order.  Integer min / max take
+// the DPP operand directly (v_min_i32_dpp); float min / max would first
+// canonicalize the moved value.
+template <int CTRL>
+__device__ __forceinline__ void two_min_dpp(int& u1, int& u2) {
+    const int a = max(u1, dpp_i<CTRL>(u1));
+    const int b = min(u2, dpp_i<CTRL>(u2));
+    u1 = min(u1, dpp_i<CTRL>(u1));
+    u2 = min(a, b);
 }
 
 __device__ __forceinline__ void two_min_row16(float& u1, float& u2) {
-    two_min_merge(u1, u2, dpp_f<0xB1>(u1), dpp_f<0xB1>(u2));
-    two_min_merge(u1, u2, dpp_f<0x4E>(u1), dpp_f<0x4E>(u2));
-    two_min_merge(u1, u2, dpp_f<0x141>(u1), dpp_f<0x141>(u2));
-    two_min_merge(u1, u2, dpp_f<0x140>(u1), dpp_f<0x140>(u2));
+    int a = __float_as_int(u1), b = __float_as_int(u2);
+    two_min_dpp<0xB1>(a, b);
+    two_min_dpp<0x4E>(a, b);
+    two_min_dpp<0x141>(a, b);
+    two_min_dpp<0x140>(a, b);
+    u1 = __int_as_float(a);
+    u2 = __int_as_float(b);
 }
 
 constexpr int LROW = 40;   // LDS row stride (u16): 32 components + 8 pad
