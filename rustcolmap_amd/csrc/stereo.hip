@@ -866,10 +866,12 @@ __global__ __launch_bounds__(TW* TH) void init_kernel(const JobDesc* __restrict_
     block_count<TW, TH>(ne, &s_cnt, counter);
 }
 
-// The per-tap evaluators (3, 4) fit five waves per SIMD in 96 registers with
-// a 28-byte spill; measured 2-7 % faster than the four waves their natural
-// 104-109 registers allow.
-#define PASS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(EV >= 3 ? 5 : 1, 8)))
+// The per-tap evaluators (3, 4) need 104-109 registers (four waves per
+// SIMD).  Held to 80 they run six waves with a ~100-byte spill, all of it
+// outside the tap loops: measured 2-7 % faster than four waves at five
+// (96 registers), and six beat five again by 0.4-0.7 % on the 8x8 shapes
+// and 3 % on the 16x16 / 16x8 ones (same GPU, alternating runs).
+#define PASS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(EV >= 3 ? 6 : 1, 8)))
 
 template <int TW, int TH, int EV>
 __global__ __launch_bounds__(TW* TH) PASS_OCCUPANCY void pass_kernel(const JobDesc* __restrict__ jobs, const SrcDesc* __restrict__ srcs,
