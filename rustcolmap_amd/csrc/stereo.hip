@@ -867,11 +867,12 @@ __global__ __launch_bounds__(TW* TH) void init_kernel(const JobDesc* __restrict_
 }
 
 // The per-tap evaluators (3, 4) need 104-109 registers (four waves per
-// SIMD).  Held to 80 they run six waves with a ~100-byte spill, all of it
-// outside the tap loops: measured 2-7 % faster than four waves at five
-// (96 registers), and six beat five again by 0.4-0.7 % on the 8x8 shapes
-// and 3 % on the 16x16 / 16x8 ones (same GPU, alternating runs).
-#define PASS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(EV >= 3 ? 6 : 1, 8)))
+// SIMD).  Held to 96 they run five waves with a 28-byte spill: 2-7 % faster
+// than four.  Six waves (80 registers, a ~100-byte spill outside the tap
+// loops) were another 0.4-0.7 % faster on the 8x8 shapes, but the spill
+// slots of 6144 resident waves overflow L2 and doubled the pass kernel's
+// HBM traffic (10.8 -> 22.6 GB per launch), so five it is.
+#define PASS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(EV >= 3 ? 5 : 1, 8)))
 
 template <int TW, int TH, int EV>
 __global__ __launch_bounds__(TW* TH) PASS_OCCUPANCY void pass_kernel(const JobDesc* __restrict__ jobs, const SrcDesc* __restrict__ srcs,
