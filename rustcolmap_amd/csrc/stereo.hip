@@ -542,7 +542,10 @@ __device__ __forceinline__ float eval_v4_body(const u64* __restrict__ s_cen, con
     // the run-time source count multiplies the kernel's code about tenfold
 #pragma nounroll
     for (int si = 1; si < nsrc; ++si) {
-        const SrcP P = load_src(S[si]);
+        // si is the same in every lane still here; saying so keeps the
+        // source parameters in scalar registers (a loop with lane-dependent
+        // exits is otherwise treated as divergent and loads them per lane)
+        const SrcP P = load_src(S[__builtin_amdgcn_readfirstlane(si)]);
         const double fs = (double)(si + 1) * 5.0;
         const double cap = si + 1 < nsrc ? (double)(si + 2) * 5.0 : 1e30;
         // an opaque copy of the window index keeps the compiler from hoisting
@@ -866,13 +869,13 @@ __global__ __launch_bounds__(TW* TH) void init_kernel(const JobDesc* __restrict_
     block_count<TW, TH>(ne, &s_cnt, counter);
 }
 
-// The per-tap evaluators (3, 4) need 104-109 registers (four waves per
-// SIMD).  Held to 96 they run five waves with a 28-byte spill: 2-7 % faster
-// than four.  Six waves (80 registers, a ~100-byte spill outside the tap
-// loops) were another 0.4-0.7 % faster on the 8x8 shapes, but the spill
-// slots of 6144 resident waves overflow L2 and doubled the pass kernel's
-// HBM traffic (10.8 -> 22.6 GB per launch), so five it is.
-#define PASS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(EV >= 3 ? 5 : 1, 8)))
+// With the source parameters of sources >= 1 in scalar registers the
+// per-tap evaluators (3, 4) need 89 registers: five waves per SIMD with no
+// spill.  Held to 80 they run six waves with a 20-byte spill outside the
+// tap loops, measured 1.8 % faster on the 8x8 shapes and 2-4 % on the
+// 16x16 / 16x8 ones (same GPU, alternating runs) for 22 % more HBM traffic
+// (spill slots; 9.5 -> 11.6 GB per pass launch, far below the HBM bound).
+#define PASS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(EV >= 3 ? 6 : 1, 8)))
 
 template <int TW, int TH, int EV>
 __global__ __launch_bounds__(TW* TH) PASS_OCCUPANCY void pass_kernel(const JobDesc* __restrict__ jobs, const SrcDesc* __restrict__ srcs,
