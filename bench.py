@@ -43,6 +43,7 @@ def log(msg):
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "pass_kernel_room1080_traffic.txt")
+KERNEL_SUMMARY = os.path.join(ROOT, "profiles", "stereo_room1080_kernel_trace.txt")
 
 
 def profiled_traffic(kernel, path=TRAFFIC_SUMMARY):
@@ -64,6 +65,31 @@ def profiled_traffic(kernel, path=TRAFFIC_SUMMARY):
     if fetch is None or write is None:
         return None
     return fetch + write, os.path.relpath(path, ROOT)
+
+
+def profiled_counter(kernel, counter, path):
+    """Average per dispatch of `counter` for `kernel` from a committed
+    --rocprof-summary file; None when the file or the line is absent."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith(kernel + " ") and (" %s " % counter) in line and " avg " in line:
+                return float(line.split(" avg ")[1].split()[0]), os.path.relpath(path, ROOT)
+    return None
+
+
+def valu_roofline(kernel, launch_ms, path=KERNEL_SUMMARY):
+    """The bound that actually limits the pass kernel: wave-level VALU
+    instructions per launch (SQ_INSTS_VALU from the committed profile of the
+    same launch shape) x 64 lanes over the live launch time, against the
+    VALU rate of 256 CUs x 128 lanes x 2.4 GHz."""
+    c = profiled_counter(kernel, "SQ_INSTS_VALU", path)
+    if c is None or launch_ms <= 0:
+        return None
+    achieved = c[0] * 64.0 / (launch_ms / 1000.0) / 1e12
+    return {"instructions_per_launch": c[0], "achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2),
+            "unit": "T lane-ops/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "source": c[1]}
 
 
 def rocprof_summary(path, copy_bytes=1 << 30, only=("mvs::", "mt::")):
@@ -292,6 +318,7 @@ def main():
                      "measured_hbm_frac": None if traffic is None else
                      round(traffic[0] / (avg_pass_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
                      "bytes_per_launch": int(pass_bytes / n_passes), "avg_launch_ms": round(avg_pass_ms, 4),
+                     "valu": valu_roofline(pass_kernel, avg_pass_ms),
                      "launch_time_source": "graph-free run after the timed steps" if args.graph
                      else "HIP events around the pass launches of the timed steps"},
         "step_wall_ms": [round(x, 2) for x in step_walls],
