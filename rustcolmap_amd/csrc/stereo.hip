@@ -992,12 +992,23 @@ __device__ __forceinline__ unsigned fronto_tap_cost(u64 cref, float2 ray, float 
     return ok ? (unsigned)__popcll(cref ^ P.cen[idx]) : 48u;
 }
 
+// One source's tap costs of this thread's pixels of the tile + apron.
+template <int NP, int NT, bool PIN, bool FD>
+__device__ __forceinline__ void vol_costs(const u64* s_cen, const float2* s_ray, float d, const SrcP& P,
+                                          unsigned* tot) {
+#pragma unroll
+    for (int k = 0; k < (NP + NT - 1) / NT; ++k) {
+        const int i = (int)threadIdx.x + k * NT;
+        if (i < NP) tot[k] += fronto_tap_cost<PIN, FD>(s_cen[i], s_ray[i], d, P);
+    }
+}
+
 template <int TW, int TH, int S>
 __global__ __launch_bounds__(TW* TH) void volume_kernel(const JobDesc* __restrict__ jobs,
                                                         const SrcDesc* __restrict__ srcs,
                                                         const int4* __restrict__ tiles, int ntiles, int nplanes,
                                                         const VolOut* __restrict__ outs) {
-    constexpr int PW = S * TW + 4, PH = S * TH + 4, NP = PW * PH;
+    constexpr int PW = S * TW + 4, PH = S * TH + 4, NP = PW * PH, NT = TW * TH, NIT = (NP + NT - 1) / NT;
     __shared__ u64 s_cen[NP];
     __shared__ float2 s_ray[NP];
     __shared__ unsigned short s_cost[NP];
@@ -1027,19 +1038,28 @@ __global__ __launch_bounds__(TW* TH) void volume_kernel(const JobDesc* __restric
         const float inv = J.inv_min + (J.inv_max - J.inv_min) * (float)hi / (float)(nplanes - 1);
         const float d = 1.0f / inv;
         __syncthreads();   // staging done / previous plane's sums read
-        for (int i = threadIdx.x; i < NP; i += TW * TH) {
-            const u64 cref = s_cen[i];
-            const float2 ray = s_ray[i];
-            unsigned tot = 0;
-            for (int si = 0; si < J.nsrc; ++si) {
-                const SrcP P = load_src(Sd[si]);
-                const bool pin = pin_ok && P.k == 0.0f;
-                tot += fast ? (pin ? fronto_tap_cost<true, true>(cref, ray, d, P)
-                                   : fronto_tap_cost<false, true>(cref, ray, d, P))
-                            : (pin ? fronto_tap_cost<true, false>(cref, ray, d, P)
-                                   : fronto_tap_cost<false, false>(cref, ray, d, P));
-            }
-            s_cost[i] = (unsigned short)tot;
+        // sources outside, the thread's NIT pixels inside (unrolled): the
+        // NIT census gathers of a source are in flight together; the
+        // integer sums do not depend on the order
+        unsigned tot[NIT];
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) tot[k] = 0;
+        for (int si = 0; si < J.nsrc; ++si) {
+            const SrcP P = load_src(Sd[si]);
+            const bool pin = pin_ok && P.k == 0.0f;
+            if (fast && pin)
+                vol_costs<NP, NT, true, true>(s_cen, s_ray, d, P, tot);
+            else if (fast)
+                vol_costs<NP, NT, false, true>(s_cen, s_ray, d, P, tot);
+            else if (pin)
+                vol_costs<NP, NT, true, false>(s_cen, s_ray, d, P, tot);
+            else
+                vol_costs<NP, NT, false, false>(s_cen, s_ray, d, P, tot);
+        }
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int i = threadIdx.x + k * NT;
+            if (i < NP) s_cost[i] = (unsigned short)tot[k];
         }
         __syncthreads();
         if (mine) {
