@@ -665,6 +665,18 @@ __device__ __forceinline__ unsigned fronto_code(u64 cref, float2 ray, float d, c
     return ok ? (unsigned)__popcll(cref ^ P.cen[idx]) : 48u;
 }
 
+// One source's codes of the tile + apron, this thread's pixels unrolled so
+// that their census gathers are in flight together.
+template <int NP, int NT, bool PIN, bool FD>
+__device__ __forceinline__ void box_codes(const u64* s_cen, const float2* s_ray, float d, const SrcP& P,
+                                          unsigned char* s_code) {
+#pragma unroll
+    for (int k = 0; k < (NP + NT - 1) / NT; ++k) {
+        const int i = (int)threadIdx.x + k * NT;
+        if (i < NP) s_code[i] = (unsigned char)fronto_code<PIN, FD>(s_cen[i], s_ray[i], d, P);
+    }
+}
+
 template <int TW, int TH>
 __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* __restrict__ srcs, int x0, int y0,
                                               int nh, bool want, const u64* s_cen, const float2* s_ray,
@@ -692,13 +704,14 @@ __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* _
             if (!__syncthreads_or(live)) break;
             const SrcP P = load_src(S[si]);
             const bool pin = pin_ok && P.k == 0.0f;
-            for (int i = threadIdx.x; i < NP; i += NT) {
-                const unsigned v = fast ? (pin ? fronto_code<true, true>(s_cen[i], s_ray[i], d, P)
-                                               : fronto_code<false, true>(s_cen[i], s_ray[i], d, P))
-                                        : (pin ? fronto_code<true, false>(s_cen[i], s_ray[i], d, P)
-                                               : fronto_code<false, false>(s_cen[i], s_ray[i], d, P));
-                s_code[i] = (unsigned char)v;
-            }
+            if (fast && pin)
+                box_codes<NP, NT, true, true>(s_cen, s_ray, d, P, s_code);
+            else if (fast)
+                box_codes<NP, NT, false, true>(s_cen, s_ray, d, P, s_code);
+            else if (pin)
+                box_codes<NP, NT, true, false>(s_cen, s_ray, d, P, s_code);
+            else
+                box_codes<NP, NT, false, false>(s_cen, s_ray, d, P, s_code);
             __syncthreads();
             if (live) {
                 const double fs = (double)(si + 1) * 5.0;
