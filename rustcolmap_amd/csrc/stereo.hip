@@ -590,17 +590,22 @@ __device__ __forceinline__ void fill_pc_table(float* tab) {
 // tile staging
 // --------------------------------------------------------------------------
 
+// Global-address-space loads (not flat ones, which also count against the
+// LDS wait counter).
 template <int TW, int TH>
 __device__ __forceinline__ void stage_tile(const JobDesc& J, int x0, int y0, u64* s_cen, float2* s_ray) {
     constexpr int PW = TW + 4, PH = TH + 4;
+    gu64* cen = (gu64*)J.census;
+    gu64* ray = (gu64*)J.rays;   // a float2 moved as its 8 bytes
+    u64* s_ray8 = reinterpret_cast<u64*>(s_ray);
     const int w = J.w, h = J.h;
     for (int i = threadIdx.x; i < PW * PH; i += TW * TH) {
         const int ay = i / PW, ax = i % PW;
         const int gx = min(max(x0 - 2 + ax, 0), w - 1);
         const int gy = min(max(y0 - 2 + ay, 0), h - 1);
         const size_t g = (size_t)gy * w + gx;
-        s_cen[i] = J.census[g];
-        s_ray[i] = J.rays[g];
+        s_cen[i] = cen[g];
+        s_ray8[i] = ray[g];
     }
 }
 
