@@ -167,6 +167,10 @@ __device__ __forceinline__ float eval_plane(const Ref& R, const SrcDesc* __restr
 //     flight together;
 //   * source census reads go through the global address space.
 typedef __attribute__((address_space(1))) const u64 gu64;
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const f32x4v gplane;
+typedef __attribute__((address_space(1))) const float gfloat;
+__device__ __forceinline__ float4 as_float4(f32x4v v) { return make_float4(v.x, v.y, v.z, v.w); }
 
 struct SrcP {
     gu64* cen;
@@ -768,11 +772,13 @@ __device__ __forceinline__ void pass_tile(const JobDesc& J, const SrcDesc* __res
     const SrcDesc* S = srcs + J.src_off;
     const int nsrc = J.nsrc;
     const size_t i = (size_t)y * w + x;
-    const float4* __restrict__ prev = J.plane[par];
+    // planes and costs of the previous pass through global-address-space
+    // loads (flat ones would also hold up the LDS waits of the window reads)
+    gplane* prev = (gplane*)J.plane[par];
     const float2 ri = s_ray[c];
     u64 seed = pixel_rng(x, y, salt);
-    const float4 cur = prev[i];
-    const float cur_cost = J.cost[par][i];
+    const float4 cur = as_float4(prev[i]);
+    const float cur_cost = ((gfloat*)J.cost[par])[i];
     float best_c = cur_cost;
     float b0 = cur.x, b1 = cur.y, b2 = cur.z;
     float bd = cur.w * ((b0 * ri.x + b1 * ri.y) + b2);
@@ -791,7 +797,7 @@ __device__ __forceinline__ void pass_tile(const JobDesc& J, const SrcDesc* __res
             const int dy = q == 2 ? -1 : (q == 3 ? 1 : 0);
             const int xx = x + dx, yy = y + dy;
             if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
-            const float4 nb = prev[(size_t)yy * w + xx];
+            const float4 nb = as_float4(prev[(size_t)yy * w + xx]);
             const float2 rj = s_ray[c + dy * PW + dx];
             const float a_nb = (nb.x * rj.x + nb.y * rj.y) + nb.z;
             if (a_nb < 0.1f) continue;
