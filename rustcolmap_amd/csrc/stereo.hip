@@ -899,6 +899,7 @@ __global__ __launch_bounds__(TW* TH) void init_kernel(const JobDesc* __restrict_
 // tap loops, measured 1.8 % faster on the 8x8 shapes and 2-4 % on the
 // 16x16 / 16x8 ones (same GPU, alternating runs) for 22 % more HBM traffic
 // (spill slots; 9.5 -> 11.6 GB per pass launch, far below the HBM bound).
+// Seven waves (72 registers, a 44-byte spill) measured no faster than six.
 #define PASS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(EV >= 3 ? 6 : 1, 8)))
 
 template <int TW, int TH, int EV>
