@@ -644,29 +644,26 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
             for (int i = 0; i < 4; ++i) {
                 const int R = wr * 64 + mf * 16 + kq * 4 + i;
                 const float tr = sur[R].y;
-                uint32_t rw0 = 0u, rw1 = 0u;   // columns wc*64 + (0..31), wc*64 + (32..63)
+                // the row test's wave mask holds, for each of the four rows
+                // of this (mf, i) (kq = 0..3), its 16 columns of block nf at
+                // bits kq*16 .. kq*16+15: no lane-by-lane OR is needed
+                uint64_t bl[4];
 #pragma unroll
                 for (int nf = 0; nf < 4; ++nf) {
                     const float tU = nrow[mf][i] + ncol[nf];
                     const float U = __builtin_fmaf(-2.0f, acc[mf][nf][i], tU);
                     const float L = __builtin_fmaf(-kb, tU, U);
-                    const uint32_t rbit = L <= tr ? 1u << ((nf & 1) * 16 + r16) : 0u;
-                    if (nf < 2)
-                        rw0 |= rbit;
-                    else
-                        rw1 |= rbit;
+                    bl[nf] = __ballot(L <= tr);
                     cw[nf][mf >> 1] |= L <= tc[nf] ? 1u << ((mf & 1) * 16 + kq * 4 + i) : 0u;
                 }
-                // OR over the 16 lanes of the row group; lane r16 == 0 owns the
-                // row's two words of this column half (no other wave writes them)
-                rw0 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw0, 0xB1, 0xF, 0xF, false);
-                rw1 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw1, 0xB1, 0xF, 0xF, false);
-                rw0 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw0, 0x4E, 0xF, 0xF, false);
-                rw1 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw1, 0x4E, 0xF, 0xF, false);
-                rw0 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw0, 0x141, 0xF, 0xF, false);
-                rw1 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw1, 0x141, 0xF, 0xF, false);
-                rw0 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw0, 0x140, 0xF, 0xF, false);
-                rw1 |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)rw1, 0x140, 0xF, 0xF, false);
+                // this lane's row: the 16-bit fields of blocks 0, 1 (columns
+                // wc*64 + 0..31) and 2, 3 (32..63), paired by a byte permute
+                const uint32_t rw0 = __builtin_amdgcn_perm((uint32_t)(bl[1] >> (16 * kq)),
+                                                           (uint32_t)(bl[0] >> (16 * kq)), 0x05040100u);
+                const uint32_t rw1 = __builtin_amdgcn_perm((uint32_t)(bl[3] >> (16 * kq)),
+                                                           (uint32_t)(bl[2] >> (16 * kq)), 0x05040100u);
+                // lane r16 == 0 owns the row's two words of this column half
+                // (no other wave writes them)
                 if (r16 == 0) {
                     srm[R * 4 + wc * 2 + 0] = rw0;
                     srm[R * 4 + wc * 2 + 1] = rw1;
