@@ -42,6 +42,8 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12   # lane-ops/s: 256 CUs x 128 lanes x 2.4 GHz
+MFMA_BF16_PEAK_TFLOPS = 2500.0             # dense bf16 MFMA
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "pass_kernel_room1080_traffic.txt")
 KERNEL_SUMMARY = os.path.join(ROOT, "profiles", "stereo_room1080_kernel_trace.txt")
 
@@ -541,8 +543,6 @@ def batched_bench(args):
 
 # f32 VALU operations/s without packed math: 256 CUs x 128 lanes/clock x 2.4 GHz (the published 157.3 TFLOP/s
 # vector f32 rate counts an FMA as two)
-VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
-MFMA_BF16_PEAK_TFLOPS = 2500.0             # dense bf16 MFMA
 MATCH_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "match_filt_tile_match20x2048_traffic.txt")
 
 
