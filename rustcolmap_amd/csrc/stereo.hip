@@ -167,6 +167,8 @@ __device__ __forceinline__ float eval_plane(const Ref& R, const SrcDesc* __restr
 //     flight together;
 //   * source census reads go through the global address space.
 typedef __attribute__((address_space(1))) const u64 gu64;
+// global-address-space views of a plane (as a plain vector type: float4's
+// class assignment does not take another address space) and of a cost
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const f32x4v gplane;
 typedef __attribute__((address_space(1))) const float gfloat;
