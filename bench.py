@@ -499,7 +499,7 @@ def batched_bench(args):
     view and stage."""
     import torch
     from rustcolmap_amd.mvs import MvsContext, MvsOptions, ViewData
-    from rustcolmap_amd.scene import config_spec, render_scene, select_sources
+    from rustcolmap_amd.scene import config_spec, render_scene, uneven_jobs
     spec = config_spec("mixed64")
     if args.views:
         spec.n_views = args.views
@@ -507,9 +507,8 @@ def batched_bench(args):
     sc = render_scene(spec, device="cuda", with_gt=False)
     log("rendered %d views in %.1fs" % (len(sc.views), time.time() - t0))
     views = [ViewData.from_scene_view(v) for v in sc.views]
-    rng = np.random.default_rng(spec.seed + 100)
-    ks = rng.integers(4, 13, size=len(views))
-    jobs = [(r, select_sources(sc.views, int(ks[r]))[r], 1.0, 10.0) for r in range(len(views))]
+    jobs = uneven_jobs(sc)
+    ks = np.array([len(j[1]) for j in jobs])
     opts = MvsOptions(num_sources=12, sweeps=args.sweeps)
     npix = sum(v.w * v.h for v in views)
     ctx = MvsContext(0)

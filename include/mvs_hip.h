@@ -240,6 +240,16 @@ int mvs_debug_copy(mvs_ctx *ctx, int64_t bytes);
 int mvs_debug_rng(mvs_ctx *ctx, const int32_t *xy, const uint64_t *salt, int32_t n, int32_t ndraw,
                   uint64_t *seed_out, float *uniform_out);
 
+/* Instrumentation: passes pass_lo .. pass_hi-1 of a run of n_passes passes
+ * (the pass loop of stereo.rs:271-371, decay schedule of stereo.rs:275-277)
+ * for reference view `ref`, started from caller-given planes instead of the
+ * init sweep: planes holds 5 floats per pixel (nx, ny, nz, z, cost) and is
+ * overwritten with the planes after the last pass; n_eval (optional) gets
+ * the evaluation count of each pass run.  Uses the context's tuned launch
+ * shape. */
+int mvs_debug_run_passes(mvs_ctx *ctx, int32_t ref, const int32_t *srcs, int32_t k, double zmin, double zmax,
+                         int32_t pass_lo, int32_t pass_hi, int32_t n_passes, float *planes, int64_t *n_eval);
+
 /* ------------------------------------------------------------------------
  * run_mvs orchestration (mvs/src/lib.rs:294-517)
  * ------------------------------------------------------------------------ */

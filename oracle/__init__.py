@@ -86,6 +86,8 @@ def load():
         "orc_estimate_depth_band": (C.c_int, [P, C.c_int, C.c_int, P, C.c_int, C.c_double, C.c_double,
                                               C.POINTER(OrcOptions), C.c_int, C.c_int, C.c_int, C.c_int,
                                               P, P, P, P, P]),
+        "orc_run_passes": (C.c_int, [P, C.c_int, C.c_int, P, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                     C.c_int, C.c_int, C.c_int, P, P]),
         "orc_sample_depth": (C.c_double, [P, C.c_int, C.c_int, C.c_double, C.c_double]),
         "orc_fusion_cell": (C.c_double, [P, C.c_int64, C.c_double]),
         "orc_fuse": (C.c_int64, [P, C.c_int, P, P, P, C.c_int, C.c_double, C.c_double, C.c_int,
@@ -131,7 +133,8 @@ def census(gray: np.ndarray) -> np.ndarray:
 class OracleViews:
     """Holds the oracle's view table (census computed by the oracle)."""
 
-    def __init__(self, views):
+    def __init__(self, views, with_census: bool = True):
+        """with_census=False skips the census (fusion reads none)."""
         lib = load()
         self.keep = []
         self.rgb_keep = []
@@ -139,7 +142,7 @@ class OracleViews:
         self.views = list(views)
         for i, v in enumerate(views):
             g = np.ascontiguousarray(v.gray, np.uint8)
-            cen = census(g)
+            cen = census(g) if with_census else np.zeros(1, np.uint64)
             self.keep += [g, cen]   # keep[2i] gray, keep[2i+1] census
             c = self.arr[i]
             c.w, c.h = int(v.w), int(v.h)
@@ -157,7 +160,7 @@ class OracleViews:
             for j in range(3):
                 c.t[j] = float(t[j])
             c.gray = g.ctypes.data
-            c.census = cen.ctypes.data
+            c.census = cen.ctypes.data if with_census else None
             if v.rgb is not None:
                 self.rgb_keep.append(np.ascontiguousarray(v.rgb, np.uint8))
                 c.rgb = self.rgb_keep[-1].ctypes.data
@@ -193,6 +196,23 @@ class OracleViews:
             raise RuntimeError("orc_estimate_depth failed")
         del n
         return depth, cost, normals, planes, n_eval
+
+    def run_passes(self, ref, srcs, zmin, zmax, planes, pass_lo, pass_hi, n_passes, nthreads=None, variant=0):
+        """Passes pass_lo..pass_hi-1 (of n_passes) of stereo.rs:271-371 from
+        the given (h, w, 5) planes instead of the init sweep.  Returns the
+        planes after the last pass and the evaluation count of each pass.
+        variant > 0 runs one of the oracle's planted mistakes (tests only)."""
+        lib = load()
+        v = self.views[ref]
+        s = np.ascontiguousarray(srcs, np.int32)
+        pl = np.ascontiguousarray(planes, np.float32).reshape(v.h, v.w, 5).copy()
+        ne = np.zeros(max(pass_hi - pass_lo, 1), np.int64)
+        r = lib.orc_run_passes(C.cast(self.arr, C.c_void_p), len(self.views), int(ref), _p(s), len(s), float(zmin),
+                               float(zmax), int(pass_lo), int(pass_hi), int(n_passes),
+                               int(nthreads or os.cpu_count() or 1), int(variant), _p(pl), _p(ne))
+        if r != 0:
+            raise RuntimeError("orc_run_passes failed")
+        return pl, ne[: pass_hi - pass_lo]
 
     def fuse(self, depth_maps: Sequence[Optional[np.ndarray]], sources, min_consistent, depth_diff, cell,
              nthreads=None, want_candidates=False):

@@ -433,7 +433,18 @@ typedef struct {
     float sig_inv, sig_n;
     int64_t n_eval;
     int y_lo;          /* first row of the computed band */
+    int variant;       /* 0 = the reference; >0 = a planted mistake (orc_run_passes) */
 } est_ctx;
+
+/* Planted mistakes of orc_run_passes, used only by the tests that show the
+ * pass pins reject them. */
+enum {
+    ORC_PASS_REFERENCE = 0,
+    ORC_PASS_IN_PLACE = 1,        /* neighbours and cur read from the buffer being written */
+    ORC_PASS_DRAW_ALL = 2,        /* all 12 uniforms drawn before the inv_new / zcand test */
+    ORC_PASS_PERTURB_BEST = 3,    /* perturbations centred on the best plane so far */
+    ORC_PASS_STORE_ZCAND = 4,     /* the accepted candidate's zcand stored instead of bd / a_here */
+};
 
 static void init_row(void *c, int item) {
     est_ctx *e = (est_ctx *)c;
@@ -464,18 +475,22 @@ static void pass_row(void *c, int item) {
     const float zlo = ws->zmin * 0.5f, zhi = ws->zmax * 2.0f;
     int64_t ne = 0;
     static const int NB[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+    const int var = e->variant;
+    /* the reference reads every plane from the previous buffer */
+    const orc_plane *src = var == ORC_PASS_IN_PLACE ? e->next : e->prev;
     for (int x = 0; x < w; x++) {
         const size_t i = (size_t)y * w + x;
         uint64_t seed = orc_pixel_rng(x, y, e->pass + 7);
-        const orc_plane cur = e->prev[i];
+        const orc_plane cur = src[i];
         float best_c = cur.cost;
         float bn0 = cur.nx, bn1 = cur.ny, bn2 = cur.nz;
         float bd = cur.z * ((bn0 * rx[i] + bn1 * ry[i]) + bn2);
+        float bz = cur.z;   /* zcand of the best candidate (planted mistake 4 only) */
         for (int q = 0; q < 4; q++) {
             const int xx = x + NB[q][0], yy = y + NB[q][1];
             if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
             const size_t j = (size_t)yy * w + xx;
-            const orc_plane nb = e->prev[j];
+            const orc_plane nb = src[j];
             const float a_nb = (nb.nx * rx[j] + nb.ny * ry[j]) + nb.nz;
             if (a_nb < 0.1f) continue;
             const float d_plane = nb.z * a_nb;
@@ -485,21 +500,33 @@ static void pass_row(void *c, int item) {
             if (!(zlo <= zcand && zcand <= zhi)) continue;
             const float cst = orc_plane_cost(ws, x, y, nb.nx, nb.ny, nb.nz, d_plane, best_c + 0.02f, NULL);
             ne++;
-            if (cst < best_c) { best_c = cst; bn0 = nb.nx; bn1 = nb.ny; bn2 = nb.nz; bd = d_plane; }
+            if (cst < best_c) { best_c = cst; bn0 = nb.nx; bn1 = nb.ny; bn2 = nb.nz; bd = d_plane; bz = zcand; }
         }
         for (int r = 0; r < 3; r++) {
-            const float inv = 1.0f / fmaxf(cur.z, 1e-6f);
+            /* the reference perturbs cur; planted mistake 3 perturbs the best plane so far */
+            float cz = cur.z, cnx = cur.nx, cny = cur.ny, cnz = cur.nz;
+            if (var == ORC_PASS_PERTURB_BEST) {
+                cnx = bn0; cny = bn1; cnz = bn2;
+                cz = bd / ((bn0 * rx[i] + bn1 * ry[i]) + bn2);
+            }
+            const float inv = 1.0f / fmaxf(cz, 1e-6f);
             const float g0 = orc_gauss(&seed);
+            float g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
+            if (var == ORC_PASS_DRAW_ALL) {
+                g1 = orc_gauss(&seed);
+                g2 = orc_gauss(&seed);
+                g3 = orc_gauss(&seed);
+            }
             const float inv_new = inv + e->sig_inv * (e->inv_max - e->inv_min) * g0;
             if (inv_new <= 0.0f) continue;
             const float zcand = 1.0f / inv_new;
             if (!(zlo <= zcand && zcand <= zhi)) continue;
-            const float g1 = orc_gauss(&seed);
-            float nnx = cur.nx + e->sig_n * g1;
-            const float g2 = orc_gauss(&seed);
-            float nny = cur.ny + e->sig_n * g2;
-            const float g3 = orc_gauss(&seed);
-            float nnz = cur.nz + e->sig_n * g3;
+            if (var != ORC_PASS_DRAW_ALL) g1 = orc_gauss(&seed);
+            float nnx = cnx + e->sig_n * g1;
+            if (var != ORC_PASS_DRAW_ALL) g2 = orc_gauss(&seed);
+            float nny = cny + e->sig_n * g2;
+            if (var != ORC_PASS_DRAW_ALL) g3 = orc_gauss(&seed);
+            float nnz = cnz + e->sig_n * g3;
             const float len = sqrtf((nnx * nnx + nny * nny) + nnz * nnz);
             if (len < 1e-6f) continue;
             nnx = nnx / len; nny = nny / len; nnz = nnz / len;
@@ -508,12 +535,12 @@ static void pass_row(void *c, int item) {
             const float d_plane = zcand * a_here;
             const float cst = orc_plane_cost(ws, x, y, nnx, nny, nnz, d_plane, best_c + 0.02f, NULL);
             ne++;
-            if (cst < best_c) { best_c = cst; bn0 = nnx; bn1 = nny; bn2 = nnz; bd = d_plane; }
+            if (cst < best_c) { best_c = cst; bn0 = nnx; bn1 = nny; bn2 = nnz; bd = d_plane; bz = zcand; }
         }
         if (best_c < cur.cost) {
             const float a_here = (bn0 * rx[i] + bn1 * ry[i]) + bn2;
             if (a_here > 0.05f) {
-                orc_plane p = {bn0, bn1, bn2, bd / a_here, fminf(best_c, 1.0f)};
+                orc_plane p = {bn0, bn1, bn2, var == ORC_PASS_STORE_ZCAND ? bz : bd / a_here, fminf(best_c, 1.0f)};
                 e->next[i] = p;
             }
         }
@@ -592,6 +619,54 @@ ORC_API int orc_estimate_depth_band(const orc_view *views, int nviews, int ref, 
         if (depth) depth[i] = prev[i].z;
         if (cost) cost[i] = prev[i].cost;
         if (normals) { normals[3 * i] = prev[i].nx; normals[3 * i + 1] = prev[i].ny; normals[3 * i + 2] = prev[i].nz; }
+    }
+    free(prev);
+    free(next);
+    orc_ws_free(ws);
+    return 0;
+}
+
+/*
+ * Passes pass_lo .. pass_hi-1 of a run of n_passes passes (stereo.rs:271-371),
+ * started from caller-given planes (5 floats/px: nx, ny, nz, z, cost) that
+ * stand in for the init sweep; `planes` is updated in place.  n_eval gets
+ * one counter per pass run.  `variant` 0 is the reference; the others are
+ * the planted mistakes of the enum above, for the tests that show the pass
+ * pins reject them (ORC_PASS_IN_PLACE is only meaningful with nthreads 1).
+ */
+ORC_API int orc_run_passes(const orc_view *views, int nviews, int ref, const int *srcs, int k, double zmin,
+                           double zmax, int pass_lo, int pass_hi, int n_passes, int nthreads, int variant,
+                           float *planes, int64_t *n_eval) {
+    if (pass_lo < 0 || pass_hi < pass_lo || n_passes < 1 || !planes) return -1;
+    orc_ws *ws = orc_ws_new(views, nviews, ref, srcs, k, zmin, zmax);
+    if (!ws) return -1;
+    const size_t n = (size_t)ws->w * ws->h;
+    orc_plane *prev = (orc_plane *)malloc(n * sizeof(orc_plane));
+    orc_plane *next = (orc_plane *)malloc(n * sizeof(orc_plane));
+    for (size_t i = 0; i < n; i++) {
+        orc_plane p = {planes[5 * i], planes[5 * i + 1], planes[5 * i + 2], planes[5 * i + 3], planes[5 * i + 4]};
+        prev[i] = p;
+    }
+    est_ctx e;
+    memset(&e, 0, sizeof(e));
+    e.ws = ws;
+    e.inv_min = 1.0f / ws->zmax;
+    e.inv_max = 1.0f / ws->zmin;
+    e.variant = variant;
+    for (int pass = pass_lo; pass < pass_hi; pass++) {
+        orc_schedule((uint32_t)pass, (uint32_t)n_passes, &e.sig_inv, &e.sig_n);
+        memcpy(next, prev, n * sizeof(orc_plane));
+        e.prev = prev;
+        e.next = next;
+        e.pass = (uint64_t)pass;
+        e.n_eval = 0;
+        parallel_for(ws->h, nthreads, pass_row, &e);
+        if (n_eval) n_eval[pass - pass_lo] = e.n_eval;
+        orc_plane *tmp = prev; prev = next; next = tmp;
+    }
+    for (size_t i = 0; i < n; i++) {
+        planes[5 * i + 0] = prev[i].nx; planes[5 * i + 1] = prev[i].ny; planes[5 * i + 2] = prev[i].nz;
+        planes[5 * i + 3] = prev[i].z; planes[5 * i + 4] = prev[i].cost;
     }
     free(prev);
     free(next);

@@ -103,6 +103,8 @@ _SIGS = {
     "mvs_debug_cvt": (C.c_int, [_P, _P, C.c_int32, _P]),
     "mvs_resize_image": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
     "mvs_debug_div": (C.c_int, [_P, _P, _P, C.c_int32, _P]),
+    "mvs_debug_run_passes": (C.c_int, [_P, C.c_int32, _P, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                       C.c_int32, _P, _P]),
     "mvs_plan": (C.c_int, [C.POINTER(MvsViewC), C.c_int32, _P, _P, _P, C.c_int64, C.POINTER(MvsOptionsC), _P, _P,
                           _P, _P, _P, _P]),
     "mvs_run_mvs": (C.c_int, [_P, C.POINTER(MvsViewC), C.c_int32, _P, _P, _P, C.c_int64, C.POINTER(MvsOptionsC),

@@ -750,6 +750,79 @@ int mvs_estimate_depth_batch(mvs_ctx* c, const mvs_job_t* jobs, int32_t njobs, c
     }
 }
 
+int mvs_debug_run_passes(mvs_ctx* c, int32_t ref, const int32_t* srcs, int32_t k, double zmin, double zmax,
+                         int32_t pass_lo, int32_t pass_hi, int32_t n_passes, float* planes, int64_t* n_eval) {
+    if (!c || !srcs || !planes) return fail(MVS_ERR_ARG, "mvs_debug_run_passes: NULL argument");
+    if (pass_lo < 0 || pass_hi < pass_lo || n_passes < 1 || pass_hi - pass_lo > n_passes)
+        return fail(MVS_ERR_ARG, "mvs_debug_run_passes: need 0 <= pass_lo <= pass_hi <= pass_lo + n_passes");
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (int r = set_device(c)) return r;
+        mvs_job_t jb{};
+        jb.ref = ref;
+        jb.k = k;
+        jb.srcs = srcs;
+        jb.zmin = zmin;
+        jb.zmax = zmax;
+        mvs_options_t o;
+        mvs_options_default(&o);
+        o.sweeps = (uint32_t)((n_passes + 1) / 2);   // counter room for n_passes passes
+        BatchPlan P;
+        if (int r = plan_batch(c, &jb, 1, &o, 0, -1, P)) return r;
+        // the descriptor buffers are rewritten: a cached graph must not replay them
+        c->graph.release();
+        HCHK(c->jobs.need(1));
+        HCHK(c->srcs.need(P.srcs.size()));
+        HCHK(c->tiles.need(P.tiles.size()));
+        HCHK(c->counters.need(counter_words(P)));
+        P.jobs[0].valid_count = c->counters.p + valid_base(P);
+        HCHK(hipMemcpyAsync(c->jobs.p, P.jobs.data(), sizeof(JobDesc), hipMemcpyHostToDevice, c->stream));
+        HCHK(hipMemcpyAsync(c->srcs.p, P.srcs.data(), P.srcs.size() * sizeof(SrcDesc), hipMemcpyHostToDevice,
+                            c->stream));
+        HCHK(hipMemcpyAsync(c->tiles.p, P.tiles.data(), P.tiles.size() * sizeof(int4), hipMemcpyHostToDevice,
+                            c->stream));
+        ViewState& rv = c->views[ref];
+        const size_t np = (size_t)rv.w * rv.h;
+        std::vector<float4> pl(np);
+        std::vector<float> co(np);
+        for (size_t i = 0; i < np; i++) {
+            pl[i] = make_float4(planes[5 * i], planes[5 * i + 1], planes[5 * i + 2], planes[5 * i + 3]);
+            co[i] = planes[5 * i + 4];
+        }
+        const int par0 = pass_lo & 1;
+        HCHK(hipMemcpyAsync(rv.plane[par0].p, pl.data(), np * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+        HCHK(hipMemcpyAsync(rv.cost[par0].p, co.data(), np * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HCHK(launch_fill_u64(c->counters.p, counter_words(P), 0ULL, c->stream));
+        const int ncnt = 1 + P.n_passes;
+        for (int p = pass_lo; p < pass_hi; p++) {
+            const float t = (float)p / (float)n_passes;
+            const float sig_inv = 0.02f * powf(0.28f, t * 4.0f);
+            const float sig_n = 0.30f * powf(0.42f, t * 4.0f);
+            const int q = p - pass_lo;
+            HCHK(launch_pass(P.cfg, c->jobs.p, c->srcs.p, c->tiles.p, (int)P.tiles.size(), p & 1, (u64)p + 7, sig_inv,
+                             sig_n, c->counters.p + 1 + q, ((int*)(c->counters.p + ncnt)) + q, P.grid_p, c->stream));
+        }
+        const int par1 = pass_hi & 1;
+        HCHK(hipMemcpyAsync(pl.data(), rv.plane[par1].p, np * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        HCHK(hipMemcpyAsync(co.data(), rv.cost[par1].p, np * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        std::vector<u64> cnt(counter_words(P));
+        HCHK(hipMemcpyAsync(cnt.data(), c->counters.p, cnt.size() * sizeof(u64), hipMemcpyDeviceToHost, c->stream));
+        HCHK(hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < np; i++) {
+            planes[5 * i] = pl[i].x;
+            planes[5 * i + 1] = pl[i].y;
+            planes[5 * i + 2] = pl[i].z;
+            planes[5 * i + 3] = pl[i].w;
+            planes[5 * i + 4] = co[i];
+        }
+        if (n_eval)
+            for (int q = 0; q < pass_hi - pass_lo; q++) n_eval[q] = (int64_t)cnt[1 + q];
+        return MVS_OK;
+    } catch (const std::exception& e) {
+        return fail(MVS_ERR_NOMEM, std::string("mvs_debug_run_passes: ") + e.what());
+    }
+}
+
 int mvs_autotune(mvs_ctx* c, const mvs_job_t* jobs, int32_t njobs, const mvs_options_t* opts, int32_t passes,
                  int32_t* best, float* cand_ms, int32_t* cand_cfg, int32_t* n_cand) {
     if (!c || !opts || !jobs || njobs < 1) return fail(MVS_ERR_ARG, "mvs_autotune: bad arguments");

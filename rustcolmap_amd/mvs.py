@@ -352,6 +352,19 @@ class MvsContext:
                                              ptr(ninv), ptr(peak)), "mvs_debug_plane_cost")
         return cost, ham, ninv, peak
 
+    def debug_run_passes(self, ref: int, srcs: Sequence[int], zmin: float, zmax: float, planes: np.ndarray,
+                         pass_lo: int, pass_hi: int, n_passes: int):
+        """Passes pass_lo..pass_hi-1 (of n_passes) from the given (h, w, 5)
+        planes instead of the init sweep; returns (planes, n_eval per pass)."""
+        v = self.views[ref]
+        s = np.ascontiguousarray(srcs, np.int32)
+        pl = np.ascontiguousarray(planes, np.float32).reshape(v.h, v.w, 5).copy()
+        ne = np.zeros(max(pass_hi - pass_lo, 1), np.int64)
+        check(self._lib.mvs_debug_run_passes(self._h, int(ref), ptr(s), len(s), float(zmin), float(zmax),
+                                             int(pass_lo), int(pass_hi), int(n_passes), ptr(pl), ptr(ne)),
+              "mvs_debug_run_passes")
+        return pl, ne[: pass_hi - pass_lo]
+
     def debug_rng(self, xy: np.ndarray, salt: np.ndarray, ndraw: int):
         xy = np.ascontiguousarray(xy, np.int32).reshape(-1, 2)
         salt = np.ascontiguousarray(salt, np.uint64).reshape(-1)

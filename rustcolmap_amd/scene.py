@@ -391,6 +391,15 @@ def select_sources(views: Sequence[View], k: int) -> List[List[int]]:
     return out
 
 
+def uneven_jobs(sc: Scene, zmin: float = 1.0, zmax: float = 10.0):
+    """Stereo jobs of the uneven batch: per view a support-set size K drawn
+    uniformly from [4, 12] with the scene's seed + 100, its K nearest views
+    by viewing direction as sources.  Returns [(ref, srcs, zmin, zmax)]."""
+    rng = np.random.default_rng(sc.spec.seed + 100)
+    ks = rng.integers(4, 13, size=len(sc.views))
+    return [(r, select_sources(sc.views, int(ks[r]))[r], zmin, zmax) for r in range(len(sc.views))]
+
+
 def sparse_points(sc: Scene, n: int, seed: int = 0, rel_tol: float = 0.01):
     """A synthetic sparse reconstruction: n surface points sampled from the
     ground-truth depth of random views, each with the track of views that
