@@ -114,6 +114,19 @@ void mvs_ctx_destroy(mvs_ctx *ctx);
  * clamp-to-edge, set when the neighbour is darker.  Host in, host out. */
 int mvs_census(mvs_ctx *ctx, const uint8_t *gray, int32_t w, int32_t h, uint64_t *out);
 
+/* No counterpart in the reference: mvs_census for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and result as
+ * mvs_census and the same kernel, but the device staging (image, codes,
+ * launch descriptor) lives in one file-scope device array per GPU, set up
+ * when a context is created on that GPU, so the call allocates nothing.
+ * Images above MVS_NOALLOC_MAX_PIXELS pixels get MVS_ERR_LIMIT.  Every GPU
+ * the library runs on holds the array (about 36 MiB) until the process
+ * ends; the calls on one GPU share it and are serialised by the library,
+ * and each call rewrites everything it reads back. */
+#define MVS_NOALLOC_MAX_PIXELS (1 << 22)
+int mvs_census_noalloc(mvs_ctx *ctx, const uint8_t *gray, int32_t w, int32_t h, uint64_t *out);
+
 /* load_views (mvs/src/lib.rs:144-218) after decode/resize: uploads n views,
  * replacing any previous set, and computes each view's census and rays on
  * the device (one launch for all views). */

@@ -78,6 +78,7 @@ _SIGS = {
     "mvs_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "mvs_ctx_destroy": (None, [_P]),
     "mvs_census": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P]),
+    "mvs_census_noalloc": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P]),
     "mvs_upload_views": (C.c_int, [_P, C.POINTER(MvsViewC), C.c_int32]),
     "mvs_view_census": (C.c_int, [_P, C.c_int32, _P]),
     "mvs_estimate_depth": (C.c_int, [_P, C.c_int32, _P, C.c_int32, C.c_double, C.c_double,

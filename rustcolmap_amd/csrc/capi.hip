@@ -206,6 +206,43 @@ static int set_device(mvs_ctx* c) {
     return MVS_OK;
 }
 
+// --------------------------------------------------------------------------
+// allocation-free census: staging in one file-scope device array per GPU
+// --------------------------------------------------------------------------
+//
+// Layout of `scratch` (every part 256-byte aligned): the u8 image, the u64
+// census codes and the one-view PrepView descriptor of the prepare kernel.
+// In host code `scratch` names a host-side stand-in, never device memory:
+// host code uses only the per-device address from hipGetSymbolAddress.
+static constexpr size_t NA_GRAY_OFF = 0;
+static constexpr size_t NA_CENSUS_OFF = NA_GRAY_OFF + (((size_t)MVS_NOALLOC_MAX_PIXELS + 255) & ~(size_t)255);
+static constexpr size_t NA_DESC_OFF = NA_CENSUS_OFF + (size_t)MVS_NOALLOC_MAX_PIXELS * sizeof(u64);
+static constexpr size_t NA_BYTES = NA_DESC_OFF + ((sizeof(PrepView) + 255) & ~(size_t)255);
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+static constexpr int NA_MAX_DEVICES = 64;
+struct NoAllocSlot {
+    std::mutex mu;            // serialises the calls on one device (they share its scratch)
+    bool looked_up = false;
+    hipError_t err = hipSuccess;
+    unsigned char* ptr = nullptr;
+};
+static NoAllocSlot g_noalloc[NA_MAX_DEVICES];
+static std::mutex g_noalloc_init;
+
+// Once per device, with that device current: the device address of its copy
+// of `scratch`.  Called from mvs_ctx_create.
+static void noalloc_lookup(int device) {
+    if (device < 0 || device >= NA_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_noalloc_init);
+    NoAllocSlot& s = g_noalloc[device];
+    if (s.looked_up) return;
+    void* p = nullptr;
+    s.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    s.ptr = s.err == hipSuccess ? (unsigned char*)p : nullptr;
+    s.looked_up = true;
+}
+
 extern "C" {
 
 void mvs_options_default(mvs_options_t* o) {
@@ -239,6 +276,7 @@ int mvs_ctx_create(int device, mvs_ctx** out) {
         }
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+        noalloc_lookup(device);
         for (auto& e : c->ev) (void)hipEventCreate(&e);
         memset(&c->stats, 0, sizeof(c->stats));
         *out = c;
@@ -300,6 +338,42 @@ int mvs_census(mvs_ctx* c, const uint8_t* gray, int32_t w, int32_t h, uint64_t* 
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     g.release(); cen.release(); pv.release();
     if (e != hipSuccess) return fail(MVS_ERR_HIP, std::string("mvs_census: ") + hipGetErrorString(e));
+    return MVS_OK;
+}
+
+int mvs_census_noalloc(mvs_ctx* c, const uint8_t* gray, int32_t w, int32_t h, uint64_t* out) {
+    if (!c || !gray || !out || w <= 0 || h <= 0) return fail(MVS_ERR_ARG, "mvs_census_noalloc: bad arguments");
+    if ((int64_t)w * h > (int64_t)MVS_NOALLOC_MAX_PIXELS)
+        return fail(MVS_ERR_LIMIT, "mvs_census_noalloc: image has more than MVS_NOALLOC_MAX_PIXELS (" +
+                                       std::to_string((long long)MVS_NOALLOC_MAX_PIXELS) + ") pixels");
+    if (c->device < 0 || c->device >= NA_MAX_DEVICES)
+        return fail(MVS_ERR_LIMIT, "mvs_census_noalloc: device index beyond the scratch table");
+    NoAllocSlot& slot = g_noalloc[c->device];
+    if (!slot.ptr)
+        return fail(MVS_ERR_HIP, std::string("mvs_census_noalloc: no scratch address for this device: ") +
+                                     hipGetErrorString(slot.looked_up ? slot.err : hipErrorNotInitialized));
+    std::lock_guard<std::mutex> lk(c->mu);
+    std::lock_guard<std::mutex> lk2(slot.mu);
+    if (int r = set_device(c)) return r;
+    const size_t n = (size_t)w * h;
+    uint8_t* d_gray = slot.ptr + NA_GRAY_OFF;
+    u64* d_census = (u64*)(slot.ptr + NA_CENSUS_OFF);
+    PrepView* d_desc = (PrepView*)(slot.ptr + NA_DESC_OFF);
+    PrepView v{};
+    v.gray = d_gray;
+    v.census = d_census;
+    v.rays = nullptr;
+    v.w = w;
+    v.h = h;
+    const int nb = prepare_tiles(w, h, v.tiles_x);
+    v.tile_off = 0;
+    // every census code of the image is written by the kernel: nothing to clear
+    hipError_t e = hipMemcpyAsync(d_gray, gray, n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_desc, &v, sizeof(v), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_prepare(d_desc, 1, nb, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_census, n * sizeof(u64), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(MVS_ERR_HIP, std::string("mvs_census_noalloc: ") + hipGetErrorString(e));
     return MVS_OK;
 }
 
