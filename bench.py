@@ -42,7 +42,10 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12   # lane-ops/s: 256 CUs x 128 lanes x 2.4 GHz
+# VALU issue peak: 256 CUs x 128 lanes per clock x 2.4 GHz = 78.6 T lane-ops/s.  The published MI355X FP32
+# vector rate, 157.3 TFLOP/s, is this rate with an FMA counted as two operations; the pass kernel's own SQ
+# counters agree (SQ_ACTIVE_INST_VALU ~ SQ_INSTS_VALU issue cycles at two clocks per wave64 instruction).
+VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
 MFMA_BF16_PEAK_TFLOPS = 2500.0             # dense bf16 MFMA
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "pass_kernel_room1080_traffic.txt")
 KERNEL_SUMMARY = os.path.join(ROOT, "profiles", "stereo_room1080_kernel_trace.txt")
@@ -82,16 +85,88 @@ def profiled_counter(kernel, counter, path):
 
 
 def valu_roofline(kernel, launch_ms, path=KERNEL_SUMMARY):
-    """The bound that actually limits the pass kernel: wave-level VALU
+    """The bound that actually limits the stereo kernels: wave-level VALU
     instructions per launch (SQ_INSTS_VALU from the committed profile of the
-    same launch shape) x 64 lanes over the live launch time, against the
-    VALU rate of 256 CUs x 128 lanes x 2.4 GHz."""
+    same kernel and workload; the count is fixed by the seeded inputs) x 64
+    lanes over the live launch time, against VALU_PEAK_TOPS."""
     c = profiled_counter(kernel, "SQ_INSTS_VALU", path)
     if c is None or launch_ms <= 0:
         return None
     achieved = c[0] * 64.0 / (launch_ms / 1000.0) / 1e12
     return {"instructions_per_launch": c[0], "achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2),
             "unit": "T lane-ops/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "source": c[1]}
+
+
+def stereo_roofline(pass_kernel, avg_pass_ms, traffic, floor, alg_gbs, alg_bytes_per_launch, time_source):
+    """roofline object of the default line: the pass kernel against the VALU
+    issue peak (the bound that limits it), with its measured HBM traffic next
+    to the unique-bytes floor and the algorithmic byte rate of the
+    measurement plan as separately labelled fields."""
+    valu = valu_roofline(pass_kernel, avg_pass_ms)
+    t = None if traffic is None else traffic[0]
+    return {"bound": "valu", "kernel": pass_kernel,
+            "achieved": None if valu is None else valu["achieved"],
+            "peak": round(VALU_PEAK_TOPS, 2), "unit": "T lane-ops/s",
+            "frac": None if valu is None else valu["frac"],
+            "traffic": None if t is None else int(t),
+            "avg_launch_ms": round(avg_pass_ms, 4),
+            "valu_instructions_per_launch": None if valu is None else valu["instructions_per_launch"],
+            "valu_source": None if valu is None else valu["source"],
+            "peak_basis": "256 CUs x 128 lanes/clock x 2.4 GHz: the published 157.3 TFLOP/s FP32 vector rate with "
+                          "an FMA counted as two operations",
+            "hbm": {"measured_bytes": None if t is None else int(t),
+                    "source": None if traffic is None else traffic[1],
+                    "measured_GBs": None if t is None else round(t / (avg_pass_ms / 1000.0) / 1e9, 1),
+                    "frac": None if t is None else round(t / (avg_pass_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+                    "unique_bytes_floor": int(floor),
+                    "traffic_over_floor": None if t is None else round(t / floor, 3),
+                    "peak_GBs": HBM_PEAK_GBS},
+            "algorithmic": {"GBs": round(alg_gbs, 1), "bytes_per_launch": int(alg_bytes_per_launch),
+                            "definition": "120 B/px + n_eval x (400 + 200 K) B: every evaluated hypothesis with its "
+                                          "full 5x5 window, as if every tap came from HBM (the early abort stops "
+                                          "after ~18 of 200 taps, and windows come from LDS); a work rate, not an "
+                                          "HBM rate"},
+            "launch_time_source": time_source}
+
+
+def host_threads():
+    """Threads of the CPU baselines: every core this process may run on."""
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return max(1, os.cpu_count() or 1)
+
+
+def pass_unique_bytes(views, jobs):
+    """HBM floor of one pass launch: every byte it must touch, once -- the
+    previous plane + cost read and the next written (2 x 20 B/px), the
+    reference census and rays (16 B/px), and the census of every source
+    view of every job (8 B/px), each read a single time."""
+    total = 0
+    for ref, srcs, _, _ in jobs:
+        v = views[ref]
+        total += v.w * v.h * 56
+        total += sum(views[s].w * views[s].h * 8 for s in srcs)
+    return total
+
+
+# U-Net layer table of the proposal network (rustcolmap_amd/csrc/net.hip): (input channels, output channels,
+# level); a level-l layer runs on the coarse grid subsampled by 2^l per axis
+NET_LAYERS = [(64, 32, 0), (32, 32, 0), (32, 64, 1), (64, 64, 1), (64, 96, 2), (96, 96, 2), (96, 128, 3),
+              (128, 128, 3), (128, 160, 4), (160, 160, 4), (288, 128, 3), (128, 128, 3), (224, 96, 2), (96, 96, 2),
+              (160, 64, 1), (64, 64, 1), (96, 32, 0), (32, 32, 0)]
+
+
+def net_conv_flops(views, stride=2):
+    """Algorithmic MFMA work of one network forward over all views: 2 x 9 x
+    cin x cout per output pixel of every 3x3 convolution, on each view's
+    coarse grid (every stride-th pixel) at each layer's level."""
+    total = 0
+    for v in views:
+        ch, cw = (v.h + stride - 1) // stride, (v.w + stride - 1) // stride
+        for cin, cout, lev in NET_LAYERS:
+            total += 2 * 9 * cin * cout * (ch >> lev) * (cw >> lev)
+    return total
 
 
 def rocprof_summary(path, copy_bytes=1 << 30, only=("mvs::", "mt::")):
@@ -181,6 +256,8 @@ def main():
     ap.add_argument("--match-exact", action="store_true", help="exact f32 tiles for every pair (no filter)")
     ap.add_argument("--no-match", action="store_true",
                     help="default run: skip the descriptor-matching block (sfm_matching)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="default run: skip the learned-initialisation block (learned)")
     ap.add_argument("--profile", action="store_true",
                     help="one directly launched batch plus a 1 GiB device copy (for counter calibration), no JSON")
     args = ap.parse_args()
@@ -289,9 +366,17 @@ def main():
     views_per_s = len(views) / ((ms_prepare + ms_step + sf.ms_fuse) / 1000.0)
     log("fusion %.1f ms, %d candidates -> %d points" % (sf.ms_fuse, sf.n_candidates, len(pts)))
 
+    learned = None
+    if not args.no_learned:
+        learned = learned_block(ctx, sc, views, jobs, K, opts, args, ms_prepare, ms_step, sf.ms_fuse)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(views, srcs, opts, args.cpu_seconds)
+
+    init_kernel = "mvs::init_kernel<%d, %d, %d>" % (launch[0], launch[1], launch[3])
+    init_valu = valu_roofline(init_kernel, ms_init)
+    floor = pass_unique_bytes(views, jobs)
 
     out = {
         "metric": "depth Mpix.iter/s at 1080p K=8 (plain patch-match, whole stereo stage)",
@@ -313,16 +398,9 @@ def main():
                    "launch": None if tuned is None else
                    {"tile": "%dx%d" % tuned[:2], "persistent": tuned[2], "evaluator": tuned[3],
                     "candidates_ms": {n: round(t, 1) for n, t in ctx.last_tune}}},
-        "roofline": {"bound": "hbm", "kernel": pass_kernel, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None if traffic is None else int(traffic[0]),
-                     "traffic_source": None if traffic is None else traffic[1],
-                     "measured_hbm_frac": None if traffic is None else
-                     round(traffic[0] / (avg_pass_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
-                     "bytes_per_launch": int(pass_bytes / n_passes), "avg_launch_ms": round(avg_pass_ms, 4),
-                     "valu": valu_roofline(pass_kernel, avg_pass_ms),
-                     "launch_time_source": "graph-free run after the timed steps" if args.graph
-                     else "HIP events around the pass launches of the timed steps"},
+        "roofline": stereo_roofline(pass_kernel, avg_pass_ms, traffic, floor, achieved, pass_bytes / n_passes,
+                                    "graph-free run after the timed steps" if args.graph
+                                    else "HIP events around the pass launches of the timed steps"),
         "step_wall_ms": [round(x, 2) for x in step_walls],
         "stages_ms": {"census_rays": round(ms_prepare, 3), "init": round(ms_init, 3),
                       "passes": round(ms_pass, 3), "gate": round(ms_gate, 3),
@@ -331,8 +409,11 @@ def main():
         "views_per_s": round(views_per_s, 3),
         "fused_points_per_s": round(fused_pps, 1),
         "fused_points": int(len(pts)),
-        "init_roofline_frac": round(init_bytes / (ms_init / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
-        if ms_init > 0 else None,
+        "init": {"kernel": init_kernel, "ms": round(ms_init, 3),
+                 "valu_frac": None if init_valu is None else init_valu["frac"],
+                 "valu_achieved": None if init_valu is None else init_valu["achieved"],
+                 "algorithmic_GBs": round(init_bytes / (ms_init / 1000.0) / 1e9, 1) if ms_init > 0 else None},
+        "learned": learned,
         "cpu_baseline": cpu,
     }
     if not args.no_match:
@@ -369,6 +450,77 @@ def _gt_within(depth_maps, gt_maps, tol=0.01):
         val += int((d > 0).sum())
         fg += int((g > 0).sum())
     return {"of_valid": round(ok / max(val, 1), 4), "of_foreground": round(ok / max(fg, 1), 4)}
+
+
+def learned_block(ctx, sc, views, jobs, K, opts, args, ms_prepare, ms_plain, ms_fuse_plain):
+    """The second half of the headline metric on the same workload and
+    context: learned initialisation + 2 sweeps (confidence 0.5, proposals as
+    a 49th hypothesis) against the plain 8-sweep run just timed -- views/s
+    of both (census + stereo + fusion), agreement of the learned maps with
+    the plain 8-sweep maps (pixels valid there that are valid with inverse
+    depth within 1 %), and the network stage's MFMA rate."""
+    import torch
+    from rustcolmap_amd.learned import DEFAULT_WEIGHTS, load_weights
+    from rustcolmap_amd.mvs import MvsOptions, ProposalNet, estimate_depth_learned
+    from rustcolmap_amd.scene import select_sources
+    ref8 = {r: ctx.get_depth(r).depth for r in range(len(views))}   # the timed plain 8-sweep run's maps
+    net = ProposalNet(ctx, seed=0)
+    load_weights(net, DEFAULT_WEIGHTS)
+    short = MvsOptions(num_sources=K, sweeps=2)
+    conf = args.conf
+    estimate_depth_learned(ctx, net, jobs, short, vol_k=4, conf_thr=conf, download=False)
+    torch.cuda.synchronize()
+    steps = max(1, args.steps)
+    vol = netm = 0.0
+    t = time.perf_counter()
+    for _ in range(steps):
+        _, acc = estimate_depth_learned(ctx, net, jobs, short, vol_k=4, conf_thr=conf, download=False)
+        st = ctx.stats()
+        vol += st.ms_learned_volume
+        netm += st.ms_learned_net
+    torch.cuda.synchronize()
+    ms_learned = 1000.0 * (time.perf_counter() - t) / steps
+    vol /= steps
+    netm /= steps
+    learned_d = {r: ctx.get_depth(r).depth for r in range(len(views))}
+    order = select_sources(sc.views, len(views) - 1)
+    ctx.fuse_resident([True] * len(views), order, args.cell, short)
+    ms_fuse = ctx.stats().ms_fuse
+    agree = float(np.mean([_agree(ref8[r], learned_d[r]) for r in ref8]))
+    agree_both = _agree_valid_both(ref8, learned_d)
+    vps_plain = len(views) / ((ms_prepare + ms_plain + ms_fuse_plain) / 1000.0)
+    vps_learned = len(views) / ((ms_prepare + ms_learned + ms_fuse) / 1000.0)
+    flops = net_conv_flops(views)
+    net.close()
+    out = {"views_per_s": round(vps_learned, 3), "plain_8_views_per_s": round(vps_plain, 3),
+           "speedup": round(vps_learned / vps_plain, 3),
+           "agreement_with_plain_8": round(agree, 4),
+           "agreement_definition": "pixels valid in the plain 8-sweep map that are valid with inverse depth "
+                                   "within 1 % of it",
+           "within_1pct_of_valid_in_both": round(agree_both, 4),
+           "confidence_threshold": conf, "accepted_fraction": round(acc / sum(v.w * v.h for v in views), 4),
+           "ms_stereo": round(ms_learned, 2), "ms_volume": round(vol, 2), "ms_net": round(netm, 2),
+           "ms_fusion": round(ms_fuse, 2),
+           "network": {"conv_flops": int(flops), "ms": round(netm, 2),
+                       "achieved_TFLOPs": round(flops / (netm / 1000.0) / 1e12, 1) if netm > 0 else None,
+                       "mfma_frac": round(flops / (netm / 1000.0) / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4)
+                       if netm > 0 else None,
+                       "definition": "2 x 9 x cin x cout per output pixel of the 18 3x3 convolutions, over the "
+                                     "whole network stage time (convolutions, pooling, upsampling, head)"}}
+    log("learned: %s" % out)
+    return out
+
+
+def _agree_valid_both(ref, other, tol=0.01):
+    """Share of the pixels valid in both maps whose inverse depths agree
+    within `tol` (relative)."""
+    ok = tot = 0
+    for r in ref:
+        m = (ref[r] > 0) & (other[r] > 0)
+        a, b = ref[r][m], other[r][m]
+        ok += int((np.abs(1.0 / b - 1.0 / a) <= tol / a).sum())
+        tot += int(m.sum())
+    return ok / max(tot, 1)
 
 
 def learned_bench(args):
@@ -656,7 +808,7 @@ def match_cpu_baseline(frames, pairs, seconds):
     in turn, one pair per host thread, until about `seconds` of wall time."""
     import oracle
     from concurrent.futures import ThreadPoolExecutor
-    ncores = min(16, os.cpu_count() or 1)
+    ncores = host_threads()
     oracle.load()
     done = 0
     t = time.perf_counter()
@@ -679,7 +831,7 @@ def cpu_baseline(views, srcs, opts, seconds):
     row band of the next one, until about `seconds` of wall time; scaled to
     the same metric (pixels x sweeps / time)."""
     import oracle
-    ncores = min(16, os.cpu_count() or 1)
+    ncores = host_threads()
 
     def run(r, lo, hi):
         ov = oracle.OracleViews([views[r]] + [views[s] for s in srcs[r]])

@@ -59,6 +59,8 @@ typedef struct mvs_view {
     double t[3];
     const uint8_t *gray;      /* w*h */
     const uint8_t *rgb;       /* 3*w*h, or NULL = gray replicated (to_rgb8 of luma) */
+    const char *name;         /* ViewData.name (image file name), for the log lines and the depth_vis file
+                                 name; NULL = "view<index>" */
 } mvs_view_t;
 
 /* One reference view of a batched stereo run. */
@@ -109,6 +111,18 @@ const char *mvs_last_error(void);
 /* Context: device index (as HIP numbers them) and one stream. */
 int mvs_ctx_create(int device, mvs_ctx **out);
 void mvs_ctx_destroy(mvs_ctx *ctx);
+
+/* The `log: &(dyn Fn(&str) + Sync)` argument of run_mvs / load_views /
+ * fuse_depth_maps (mvs/src/lib.rs:148, 299; mvs/src/fuse.rs:70), as a
+ * context-level callback: fn(line, user) receives each progress line the
+ * reference writes (lib.rs:212-216, 344-347, 362-371, 376-380, 391,
+ * 509-515; fuse.rs:89-92, 170-174) from mvs_run_mvs, mvs_fuse and
+ * mvs_fuse_depth_maps, on the calling thread.  fn = NULL switches it off. */
+typedef void (*mvs_log_fn)(const char *line, void *user);
+int mvs_set_log(mvs_ctx *ctx, mvs_log_fn fn, void *user);
+/* Host-only probe: `format!("{:?}", x)` of an f64 as the log lines print
+ * the depth range (lib.rs:362-371), into out (cap bytes, NUL-terminated). */
+int mvs_debug_rust_f64(double x, char *out, int32_t cap);
 
 /* census_map (mvs/src/lib.rs:119-142): 7x7 census, bit (dy+3)*7+(dx+3),
  * clamp-to-edge, set when the neighbour is darker.  Host in, host out. */
@@ -204,6 +218,9 @@ void mvs_free_points(mvs_point_t *pts);
  * 2 %/98 % extent of the sparse points -> diag * voxel_rel (>= 1e-9).
  * Returns < 0 when n == 0 (the reference panics there). */
 double mvs_fusion_cell(const double *xyz, int64_t n, double voxel_rel);
+/* The robust sparse diagonal itself (fuse.rs:72-87), for the cell's log
+ * line (fuse.rs:89-92).  Returns < 0 when n == 0. */
+double mvs_fusion_diag(const double *xyz, int64_t n);
 
 /* write_ply_binary (mvs/src/fuse.rs:221-239). */
 int mvs_write_ply(const char *path, const mvs_point_t *pts, int64_t n);
@@ -287,12 +304,18 @@ typedef struct mvs_run_stats {   /* MvsStats (lib.rs:72-79) + timings */
     int64_t n_candidates;
     double cell;
     double stereo_ms, fuse_ms;   /* device time */
-    int64_t per_view_valid[4096];
+    int64_t *per_view_valid;     /* MvsStats.per_view_valid: set by the caller to an array of nviews
+                                    entries (valid pixels per view, 0 without a depth map), or NULL */
 } mvs_run_stats_t;
 
 /* run_mvs for views at working resolution: plan, batched stereo of every
  * view with sources, fusion with the robust cell, and (out_dir != NULL)
- * <out_dir>/dense/dense.ply, dense_web.json and mvs_stats.json. */
+ * <out_dir>/dense/dense.ply, dense_web.json and mvs_stats.json; with
+ * opts->save_depth_vis also <out_dir>/dense/depth_vis/<stem>_depth.png of
+ * every depth map (save_depth_vis, lib.rs:383-392, fuse.rs:241-269: the
+ * same 8-bit grayscale pixels, this library's own PNG encoding).  Progress
+ * lines go to the context's log callback (mvs_set_log); the per-view line's
+ * time is the batched stereo stage's, which every view spans. */
 int mvs_run_mvs(mvs_ctx *ctx, const mvs_view_t *views, int32_t nviews, const double *xyz, const int32_t *track_off,
                 const int32_t *track_view, int64_t npoints, const mvs_options_t *opts, const char *out_dir,
                 mvs_run_stats_t *stats);

@@ -27,8 +27,11 @@ class MvsViewC(C.Structure):
     _fields_ = [
         ("w", C.c_int32), ("h", C.c_int32), ("f", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
         ("k", C.c_double), ("R", C.c_double * 9), ("t", C.c_double * 3),
-        ("gray", C.POINTER(C.c_uint8)), ("rgb", C.POINTER(C.c_uint8)),
+        ("gray", C.POINTER(C.c_uint8)), ("rgb", C.POINTER(C.c_uint8)), ("name", C.c_char_p),
     ]
+
+
+LOG_FN = C.CFUNCTYPE(None, C.c_char_p, C.c_void_p)   # mvs_log_fn
 
 
 class MvsJobC(C.Structure):
@@ -58,7 +61,7 @@ class MvsStatsC(C.Structure):
 class MvsRunStatsC(C.Structure):
     _fields_ = [("views", C.c_int32), ("views_with_depth", C.c_int32), ("dense_points", C.c_int64),
                 ("n_candidates", C.c_int64), ("cell", C.c_double), ("stereo_ms", C.c_double),
-                ("fuse_ms", C.c_double), ("per_view_valid", C.c_int64 * 4096)]
+                ("fuse_ms", C.c_double), ("per_view_valid", C.POINTER(C.c_int64))]
 
 
 class MvsMatchStatsC(C.Structure):
@@ -77,6 +80,8 @@ _SIGS = {
     "mvs_last_error": (C.c_char_p, []),
     "mvs_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "mvs_ctx_destroy": (None, [_P]),
+    "mvs_set_log": (C.c_int, [_P, LOG_FN, C.c_void_p]),
+    "mvs_debug_rust_f64": (C.c_int, [C.c_double, C.c_char_p, C.c_int32]),
     "mvs_census": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P]),
     "mvs_census_noalloc": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P]),
     "mvs_upload_views": (C.c_int, [_P, C.POINTER(MvsViewC), C.c_int32]),
@@ -95,6 +100,7 @@ _SIGS = {
                                       C.POINTER(C.POINTER(MvsPointC)), C.POINTER(C.c_int64)]),
     "mvs_free_points": (None, [C.POINTER(MvsPointC)]),
     "mvs_fusion_cell": (C.c_double, [_P, C.c_int64, C.c_double]),
+    "mvs_fusion_diag": (C.c_double, [_P, C.c_int64]),
     "mvs_write_ply": (C.c_int, [C.c_char_p, _P, C.c_int64]),
     "mvs_write_dense_web_json": (C.c_int, [C.c_char_p, _P, C.c_int64]),
     "mvs_get_stats": (C.c_int, [_P, C.POINTER(MvsStatsC)]),

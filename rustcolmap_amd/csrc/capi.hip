@@ -8,10 +8,14 @@
 //     recorded once per batch shape into a HIP graph and replayed;
 //   * fusion inputs (fusion order, depth-map presence) and the robust voxel
 //     cell (fuse.rs:72-88); PLY writing (fuse.rs:221-239).
+#include <errno.h>
 #include <math.h>
+#include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include <sys/stat.h>
+#include <zlib.h>
 
 #include <algorithm>
 #include <chrono>
@@ -199,11 +203,153 @@ struct mvs_ctx {
     Tuner tuner;
     std::vector<mvs_net*> nets;   // proposal networks bound to this context
     MatchState match;
+    mvs_log_fn log_fn = nullptr;  // the reference's `log` callback (mvs_set_log)
+    void* log_user = nullptr;
+    std::vector<std::string> names;   // view names of the last upload (log lines, depth_vis files)
 };
 
 static int set_device(mvs_ctx* c) {
     HCHK(hipSetDevice(c->device));
     return MVS_OK;
+}
+
+// --------------------------------------------------------------------------
+// log lines (the reference's `log` callback) and their Rust number formats
+// --------------------------------------------------------------------------
+
+static void emit(mvs_ctx* c, const std::string& line) {
+    if (c->log_fn) c->log_fn(line.c_str(), c->log_user);
+}
+
+static std::string strf(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    return std::string(buf);
+}
+
+// `{:?}` of an f64: the shortest digits that read back as x (of the
+// candidates with that many digits, the nearest), in positional notation for
+// 1e-4 <= |x| < 1e16 (with ".0" when integral), else as <d>[.<ddd>]e<exp>.
+static std::string rust_f64_debug(double x) {
+    if (std::isnan(x)) return "NaN";
+    if (std::isinf(x)) return x > 0 ? "inf" : "-inf";
+    if (x == 0.0) return std::signbit(x) ? "-0.0" : "0.0";
+    char buf[64];
+    std::string digits;
+    int exp10 = 0;
+    for (int p = 1; p <= 17 && digits.empty(); p++) {
+        // glibc's %.*e is correctly rounded: v is the nearest p-digit decimal to x.  When v does not read
+        // back as x, a neighbour in the last digit still can (the rounding interval of a power of two is
+        // narrower below it than above it).
+        snprintf(buf, sizeof(buf), "%.*e", p - 1, fabs(x));
+        std::string m;
+        for (const char* q = buf; *q && *q != 'e'; q++)
+            if (*q != '.') m += *q;
+        const int e = atoi(strchr(buf, 'e') + 1);
+        const long long v = atoll(m.c_str());
+        const long long order[3] = {v, v + 1, v - 1};
+        for (long long cand : order) {
+            if (cand <= 0) continue;
+            char tmp[64];
+            snprintf(tmp, sizeof(tmp), "%llde%d", cand, e - (p - 1));
+            if (strtod(tmp, nullptr) != fabs(x)) continue;
+            const std::string cs = std::to_string(cand);
+            exp10 = e + ((int)cs.size() - p);   // a carry (9..9 + 1) adds a digit
+            digits = cs;
+            break;
+        }
+    }
+    while (digits.size() > 1 && digits.back() == '0') digits.pop_back();
+    const std::string sign = x < 0 ? "-" : "";
+    const double ax = fabs(x);
+    if (ax < 1e-4 || ax >= 1e16) {
+        std::string r = sign + digits.substr(0, 1);
+        if (digits.size() > 1) r += "." + digits.substr(1);
+        return r + "e" + std::to_string(exp10);
+    }
+    const int point = exp10 + 1;   // digits before the decimal point
+    if (point <= 0) return sign + "0." + std::string(-point, '0') + digits;
+    if (point >= (int)digits.size()) return sign + digits + std::string(point - digits.size(), '0') + ".0";
+    return sign + digits.substr(0, point) + "." + digits.substr(point);
+}
+
+// Path::file_stem of a view name (the depth_vis file name, fuse.rs:262-266)
+static std::string file_stem(std::string name) {
+    while (name.size() > 1 && name.back() == '/') name.pop_back();   // "dir/" names "dir"
+    const size_t sl = name.find_last_of('/');
+    const std::string file = sl == std::string::npos ? name : name.substr(sl + 1);
+    if (file.empty() || file == "." || file == "..") return "view";   // no file name
+    const size_t dot = file.find_last_of('.');
+    if (dot == std::string::npos || dot == 0) return file;          // "a", ".hidden"
+    return file.substr(0, dot);                                      // "a.b.png" -> "a.b"
+}
+
+// save_depth_vis pixels (fuse.rs:243-260): 2 % / 98 % of the valid depths,
+// near = bright, invalid = 0, f32 arithmetic, `as u8` truncation.
+static std::vector<uint8_t> depth_vis_u8(const float* d, size_t n) {
+    std::vector<float> ds;
+    for (size_t i = 0; i < n; i++)
+        if (d[i] > 0.0f) ds.push_back(d[i]);
+    std::sort(ds.begin(), ds.end());
+    float lo = 0.0f, hi = 1.0f;
+    if (ds.size() >= 32) {
+        lo = ds[(size_t)((double)ds.size() * 0.02)];
+        hi = ds[(size_t)((double)ds.size() * 0.98) - 1];
+    }
+    const float range = std::max(hi - lo, 1e-9f);
+    std::vector<uint8_t> out(n);
+    for (size_t i = 0; i < n; i++) {
+        const float z = d[i];
+        if (!(z > 0.0f)) {
+            out[i] = 0;
+            continue;
+        }
+        float t = (z - lo) / range;
+        t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+        const float v = 255.0f * (1.0f - t);
+        out[i] = (uint8_t)(v >= 255.0f ? 255 : (v > 0.0f ? (int)v : 0));
+    }
+    return out;
+}
+
+// 8-bit grayscale PNG (filter type 0 rows, zlib stream)
+static bool write_png_gray8(const std::string& path, const uint8_t* img, int w, int h) {
+    std::vector<uint8_t> raw((size_t)h * (w + 1));
+    for (int y = 0; y < h; y++) {
+        raw[(size_t)y * (w + 1)] = 0;
+        memcpy(&raw[(size_t)y * (w + 1) + 1], img + (size_t)y * w, w);
+    }
+    uLongf zlen = compressBound(raw.size());
+    std::vector<uint8_t> z(zlen);
+    if (compress2(z.data(), &zlen, raw.data(), raw.size(), 6) != Z_OK) return false;
+    FILE* f = fopen(path.c_str(), "wb");
+    if (!f) return false;
+    auto be32 = [](uint32_t v, uint8_t* o) { o[0] = v >> 24; o[1] = v >> 16; o[2] = v >> 8; o[3] = v; };
+    auto chunk = [&](const char* tag, const uint8_t* data, uint32_t len) {
+        uint8_t hdr[8];
+        be32(len, hdr);
+        memcpy(hdr + 4, tag, 4);
+        uLong crc = crc32(0L, (const Bytef*)tag, 4);
+        if (len) crc = crc32(crc, data, len);
+        uint8_t tail[4];
+        be32((uint32_t)crc, tail);
+        fwrite(hdr, 1, 8, f);
+        if (len) fwrite(data, 1, len, f);
+        fwrite(tail, 1, 4, f);
+    };
+    static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    fwrite(sig, 1, 8, f);
+    uint8_t ihdr[13];
+    be32((uint32_t)w, ihdr);
+    be32((uint32_t)h, ihdr + 4);
+    ihdr[8] = 8; ihdr[9] = 0; ihdr[10] = 0; ihdr[11] = 0; ihdr[12] = 0;
+    chunk("IHDR", ihdr, 13);
+    chunk("IDAT", z.data(), (uint32_t)zlen);
+    chunk("IEND", nullptr, 0);
+    return fclose(f) == 0;
 }
 
 // --------------------------------------------------------------------------
@@ -307,6 +453,22 @@ void mvs_ctx_destroy(mvs_ctx* c) {
     delete c;
 }
 
+int mvs_debug_rust_f64(double x, char* out, int32_t cap) {
+    if (!out || cap < 1) return fail(MVS_ERR_ARG, "mvs_debug_rust_f64: no output buffer");
+    const std::string r = rust_f64_debug(x);
+    if ((int32_t)r.size() + 1 > cap) return fail(MVS_ERR_LIMIT, "mvs_debug_rust_f64: buffer too small");
+    memcpy(out, r.c_str(), r.size() + 1);
+    return MVS_OK;
+}
+
+int mvs_set_log(mvs_ctx* c, mvs_log_fn fn, void* user) {
+    if (!c) return fail(MVS_ERR_ARG, "mvs_set_log: NULL context");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->log_fn = fn;
+    c->log_user = user;
+    return MVS_OK;
+}
+
 // --------------------------------------------------------------------------
 // census / upload
 // --------------------------------------------------------------------------
@@ -394,6 +556,8 @@ int mvs_upload_views(mvs_ctx* c, const mvs_view_t* views, int32_t n) {
         for (size_t i = n; i < c->views.size(); i++) c->views[i].release();
     }
     c->views.resize(n);
+    c->names.assign(n, std::string());
+    for (int i = 0; i < n; i++) c->names[i] = views[i].name ? views[i].name : "view" + std::to_string(i);
     std::vector<PrepView> pv(n);
     int tile_off = 0;
     for (int i = 0; i < n; i++) {
@@ -1045,6 +1209,11 @@ static int fuse_common(mvs_ctx* c, const std::vector<const float*>& dmaps, const
     c->stats.ms_fuse = ms;
     c->stats.n_candidates = ncand;
     c->stats.n_points = npts;
+    {
+        int ndm = 0;
+        for (const float* d : dmaps) ndm += d != nullptr;
+        emit(c, strf("mvs: %lld consistency-passed candidates from %d depth maps", (long long)ncand, ndm));
+    }
     mvs_point_t* out = (mvs_point_t*)malloc((size_t)(npts > 0 ? npts : 1) * sizeof(mvs_point_t));
     if (!out) return fail(MVS_ERR_NOMEM, "fuse: host allocation failed");
     if (npts > 0) {
@@ -1107,8 +1276,10 @@ int mvs_fuse_depth_maps(mvs_ctx* c, const float* const* depth_maps, const int32_
 
 void mvs_free_points(mvs_point_t* p) { free(p); }
 
-double mvs_fusion_cell(const double* xyz, int64_t n, double voxel_rel) {
-    if (!xyz || n <= 0) return -1.0;
+}  // extern "C"
+
+// Robust sparse diagonal of fuse_depth_maps (fuse.rs:72-88).
+static double robust_diag(const double* xyz, int64_t n) {
     std::vector<double> ax((size_t)n);
     double ext[3];
     for (int a = 0; a < 3; a++) {
@@ -1121,8 +1292,19 @@ double mvs_fusion_cell(const double* xyz, int64_t n, double voxel_rel) {
         ih = std::min<uint64_t>(ih, (uint64_t)n - 1);
         ext[a] = ax[ih] - ax[il];
     }
-    const double diag = sqrt(ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2]);
-    const double cell = diag * voxel_rel;
+    return sqrt(ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2]);
+}
+
+extern "C" {
+
+double mvs_fusion_diag(const double* xyz, int64_t n) {
+    if (!xyz || n <= 0) return -1.0;
+    return robust_diag(xyz, n);
+}
+
+double mvs_fusion_cell(const double* xyz, int64_t n, double voxel_rel) {
+    if (!xyz || n <= 0) return -1.0;
+    const double cell = robust_diag(xyz, n) * voxel_rel;
     return cell > 1e-9 ? cell : 1e-9;
 }
 
@@ -1966,6 +2148,7 @@ extern "C" int mvs_run_mvs(mvs_ctx* c, const mvs_view_t* views, int32_t nviews, 
         return fail(MVS_ERR_STATE, "sparse reconstruction too small for MVS (" + std::to_string(npoints) +
                                        " points, need >= 8)");
     try {
+        const auto t_all = std::chrono::steady_clock::now();
         const int n = nviews;
         std::vector<int32_t> src_off(n + 1), srcs((size_t)n * std::max<uint32_t>(opts->num_sources, 1));
         std::vector<double> zr(2 * n);
@@ -1974,6 +2157,7 @@ extern "C" int mvs_run_mvs(mvs_ctx* c, const mvs_view_t* views, int32_t nviews, 
                          fus_off.data(), fus.data(), nullptr);
         if (r) return fail(r, "mvs_run_mvs: planning failed");
         if ((r = mvs_upload_views(c, views, n))) return r;
+        emit(c, strf("mvs: loaded %d views at working resolution (max_dim %u)", n, opts->max_dim));
         std::vector<mvs_job_t> jobs;
         for (int i = 0; i < n; i++) {
             if (src_off[i + 1] == src_off[i]) continue;   // no source view: no depth map (lib.rs:355-357)
@@ -1985,7 +2169,9 @@ extern "C" int mvs_run_mvs(mvs_ctx* c, const mvs_view_t* views, int32_t nviews, 
             j.zmax = zr[2 * i + 1];
             jobs.push_back(j);
         }
+        emit(c, strf("mvs: %d views with >=1 source view (K = %u)", (int)jobs.size(), opts->num_sources));
         mvs_run_stats_t rs{};
+        rs.per_view_valid = st ? st->per_view_valid : nullptr;
         rs.views = n;
         rs.views_with_depth = (int32_t)jobs.size();
         double stereo_ms = 0.0;
@@ -1995,7 +2181,41 @@ extern "C" int mvs_run_mvs(mvs_ctx* c, const mvs_view_t* views, int32_t nviews, 
         }
         std::vector<int32_t> has(n, 0);
         for (auto& j : jobs) has[j.ref] = 1;
+        std::vector<int64_t> valid(n, 0);
+        {
+            std::lock_guard<std::mutex> lk(c->mu);
+            for (int i = 0; i < n; i++) valid[i] = has[i] ? c->views[i].n_valid : 0;
+        }
+        // every view of the batch spans the whole batched stereo stage
+        const float stereo_s = (float)(stereo_ms / 1000.0);
+        for (auto& j : jobs)
+            emit(c, "mvs: [" + c->names[j.ref] + "] depth " + rust_f64_debug(j.zmin) + "-" + rust_f64_debug(j.zmax) +
+                        strf(" (%d srcs) valid %lld/%lld px in %.1fs", j.k, (long long)valid[j.ref],
+                             (long long)views[j.ref].w * views[j.ref].h, (double)stereo_s));
+        emit(c, strf("mvs: stereo done in %.1fs (%d depth maps)", (double)stereo_s, (int)jobs.size()));
+        const std::string dense = out_dir ? std::string(out_dir) + "/dense" : std::string();
+        if (out_dir) {
+            (void)mkdir(out_dir, 0755);
+            (void)mkdir(dense.c_str(), 0755);
+        }
+        if (opts->save_depth_vis && out_dir) {
+            const std::string vis = dense + "/depth_vis";
+            if (mkdir(vis.c_str(), 0755) != 0 && errno != EEXIST)
+                return fail(MVS_ERR_ARG, "mvs_run_mvs: cannot create " + vis);
+            for (auto& j : jobs) {
+                const size_t np = (size_t)views[j.ref].w * views[j.ref].h;
+                std::vector<float> d(np);
+                if ((r = mvs_get_depth(c, j.ref, d.data(), nullptr, nullptr))) return r;
+                const std::vector<uint8_t> img = depth_vis_u8(d.data(), np);
+                // a failed image write is skipped, as the reference ignores it (lib.rs:387)
+                (void)write_png_gray8(vis + "/" + file_stem(c->names[j.ref]) + "_depth.png", img.data(),
+                                      views[j.ref].w, views[j.ref].h);
+            }
+            emit(c, "mvs: depth visualizations saved");
+        }
+        const double diag = robust_diag(xyz, npoints);
         const double cell = mvs_fusion_cell(xyz, npoints, opts->voxel_rel);
+        emit(c, strf("mvs: fusion voxel cell %.4f (robust sparse diag %.2f)", cell, diag));
         mvs_point_t* pts = nullptr;
         int64_t npts = 0;
         if ((r = mvs_fuse(c, has.data(), fus_off.data(), fus.data(), cell, opts, &pts, &npts))) return r;
@@ -2004,15 +2224,12 @@ extern "C" int mvs_run_mvs(mvs_ctx* c, const mvs_view_t* views, int32_t nviews, 
         rs.stereo_ms = stereo_ms;
         rs.fuse_ms = c->stats.ms_fuse;
         rs.n_candidates = c->stats.n_candidates;
-        {
-            std::lock_guard<std::mutex> lk(c->mu);
-            for (int i = 0; i < n && i < 4096; i++) rs.per_view_valid[i] = has[i] ? c->views[i].n_valid : 0;
-        }
+        if (rs.per_view_valid)
+            for (int i = 0; i < n; i++) rs.per_view_valid[i] = valid[i];
+        std::string ply;
         if (out_dir) {
-            const std::string dense = std::string(out_dir) + "/dense";
-            (void)mkdir(out_dir, 0755);
-            (void)mkdir(dense.c_str(), 0755);
-            r = mvs_write_ply((dense + "/dense.ply").c_str(), pts, npts);
+            ply = dense + "/dense.ply";
+            r = mvs_write_ply(ply.c_str(), pts, npts);
             if (r == MVS_OK) r = mvs_write_dense_web_json((dense + "/dense_web.json").c_str(), pts, npts);
             if (r) {
                 mvs_free_points(pts);
@@ -2030,6 +2247,11 @@ extern "C" int mvs_run_mvs(mvs_ctx* c, const mvs_view_t* views, int32_t nviews, 
             }
         }
         mvs_free_points(pts);
+        const float total_s =
+            std::chrono::duration<float>(std::chrono::steady_clock::now() - t_all).count();
+        emit(c, strf("mvs: fused %lld dense points -> %s in %.1fs (total %.1fs)", (long long)npts,
+                     out_dir ? ply.c_str() : "(not written)", (double)(float)(rs.fuse_ms / 1000.0),
+                     (double)total_s));
         if (st) *st = rs;
         return MVS_OK;
     } catch (const std::exception& e) {

@@ -161,6 +161,31 @@ def rust_f64(x: float) -> str:
     return sign + s
 
 
+def rust_f64_debug(x: float) -> str:
+    """`format!("{:?}", x)` for an f64: the shortest round-trip digits in
+    positional notation with at least one fraction digit ("1.0", "0.0001")
+    for 1e-4 <= |x| < 1e16 and zero, otherwise as <d>[.<ddd>]e<exp>
+    ("1e-5", "1.5e16"); NaN, inf, -inf."""
+    x = float(x)
+    if math.isnan(x):
+        return "NaN"
+    if math.isinf(x):
+        return "inf" if x > 0 else "-inf"
+    if x == 0.0:
+        return "-0.0" if math.copysign(1.0, x) < 0 else "0.0"
+    sign = "-" if x < 0 else ""
+    m, e = ("%r" % abs(x)).lower().split("e") if "e" in repr(abs(x)).lower() else (repr(abs(x)), "0")
+    ip, _, fp = m.partition(".")
+    digits = (ip + fp).lstrip("0")
+    lead = len(ip + fp) - len(digits)
+    exp10 = len(ip) - 1 + int(e) - lead          # power of ten of the first significant digit
+    digits = digits.rstrip("0") or "0"
+    if abs(x) < 1e-4 or abs(x) >= 1e16:
+        return sign + digits[0] + ("." + digits[1:] if len(digits) > 1 else "") + "e%d" % exp10
+    s = rust_f64(abs(x))
+    return sign + (s if "." in s else s + ".0")
+
+
 def _wrap(v: int, bits: int, signed: bool) -> int:
     """Rust `as` between integer types: keep the low `bits` bits."""
     v &= (1 << bits) - 1

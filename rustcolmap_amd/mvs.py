@@ -144,6 +144,9 @@ def views_c(views: Sequence[ViewData]):
         for j in range(3):
             c.t[j] = float(t[j])
         c.gray = g.ctypes.data_as(C.POINTER(C.c_uint8))
+        name = v.name.encode("utf-8") if v.name else None   # None: the library names it view<i>
+        keep.append(name)
+        c.name = name
         if v.rgb is not None:
             rgb = np.ascontiguousarray(v.rgb, np.uint8)
             if rgb.size != 3 * v.w * v.h:
@@ -177,6 +180,21 @@ class MvsContext:
             self.close()
         except Exception:
             pass
+
+    # -- log -------------------------------------------------------------------
+    def set_log(self, log: Optional[Callable[[str], None]]):
+        """The reference's `log` callback (lib.rs:299, fuse.rs:70): receives
+        the progress lines of run_mvs and fusion; None switches it off.
+        Returns the previous callback."""
+        prev = getattr(self, "_log_py", None)
+        if log is None:
+            cb = _lib.LOG_FN()
+        else:
+            cb = _lib.LOG_FN(lambda line, _user: log(line.decode("utf-8", "replace")))
+        check(self._lib.mvs_set_log(self._h, cb, None), "mvs_set_log")
+        self._log_cb = cb          # the C side keeps only the pointer
+        self._log_py = log
+        return prev
 
     # -- views ---------------------------------------------------------------
     def upload_views(self, views: Sequence[ViewData]):
@@ -430,12 +448,16 @@ def fuse_depth_maps(views: Sequence[ViewData], depth_maps: Sequence[Optional[Dep
     ctx.ensure_views(views)
     if cell is None:
         cell = fusion_cell(sparse, opts.voxel_rel)
-    if log:
-        log("mvs: fusion voxel cell %.4f" % cell)
-    pts = ctx.fuse_depth_maps(depth_maps, sources, cell, opts)
-    if log:
-        log("mvs: %d consistency-passed candidates from %d depth maps" %
-            (ctx.stats().n_candidates, sum(1 for d in depth_maps if d is not None)))
+        if log:
+            xyz = np.ascontiguousarray(sparse, np.float64).reshape(-1, 3)
+            diag = float(_lib.load().mvs_fusion_diag(ptr(xyz), len(xyz)))
+            log("mvs: fusion voxel cell %.4f (robust sparse diag %.2f)" % (cell, diag))
+    prev = ctx.set_log(log) if log else None
+    try:
+        pts = ctx.fuse_depth_maps(depth_maps, sources, cell, opts)   # logs the candidate count
+    finally:
+        if log:
+            ctx.set_log(prev)
     return pts
 
 
@@ -626,18 +648,28 @@ def plan(views: Sequence[ViewData], xyz: np.ndarray, tracks: Sequence[Sequence[i
 
 
 def run_mvs(views: Sequence[ViewData], xyz: np.ndarray, tracks: Sequence[Sequence[int]], out_dir: Optional[str],
-            opts: MvsOptions, ctx: Optional[MvsContext] = None) -> dict:
+            opts: MvsOptions, ctx: Optional[MvsContext] = None,
+            log: Optional[Callable[[str], None]] = None) -> dict:
     """run_mvs (lib.rs:294-517) for views at working resolution: plan,
-    batched stereo, fusion, <out_dir>/dense/dense.ply and mvs_stats.json.
-    Returns MvsStats' fields plus timings."""
+    batched stereo, fusion, <out_dir>/dense/dense.ply, dense_web.json and
+    mvs_stats.json, and with opts.save_depth_vis the depth_vis PNGs; `log`
+    receives the reference's progress lines.  Returns MvsStats' fields plus
+    timings."""
     ctx = ctx or default_context()
     arr, keep = views_c(views)
     xyz, off, flat = _tracks_c(xyz, tracks)
     st = _lib.MvsRunStatsC()
+    valid = np.zeros(len(views), np.int64)
+    st.per_view_valid = valid.ctypes.data_as(C.POINTER(C.c_int64))
     oc = opts.to_c()
-    check(ctx._lib.mvs_run_mvs(ctx._h, arr, len(views), ptr(xyz), ptr(off), ptr(flat), len(xyz), C.byref(oc),
-                               None if out_dir is None else str(out_dir).encode(), C.byref(st)), "mvs_run_mvs")
+    prev = ctx.set_log(log) if log else None
+    try:
+        check(ctx._lib.mvs_run_mvs(ctx._h, arr, len(views), ptr(xyz), ptr(off), ptr(flat), len(xyz), C.byref(oc),
+                                   None if out_dir is None else str(out_dir).encode(), C.byref(st)), "mvs_run_mvs")
+    finally:
+        if log:
+            ctx.set_log(prev)
     ctx.views = list(views)
     return {"views": st.views, "views_with_depth": st.views_with_depth, "dense_points": st.dense_points,
-            "per_view_valid": [int(st.per_view_valid[i]) for i in range(min(len(views), 4096))],
+            "per_view_valid": [int(v) for v in valid],
             "n_candidates": st.n_candidates, "cell": st.cell, "stereo_ms": st.stereo_ms, "fuse_ms": st.fuse_ms}

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts():
     assert C.sizeof(_lib.MvsOptionsC) == 48
     assert C.sizeof(_lib.MvsPointC) == 32
-    assert C.sizeof(_lib.MvsViewC) == 8 + 4 * 8 + 12 * 8 + 16
+    assert C.sizeof(_lib.MvsViewC) == 8 + 4 * 8 + 12 * 8 + 24
     assert C.sizeof(_lib.MvsJobC) == 56
     assert C.sizeof(_lib.MvsStatsC) == 11 * 8 + 2 * 4 + 2 * 8 + 4 * 4
 

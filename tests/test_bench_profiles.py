@@ -55,3 +55,46 @@ def test_matching_tile_kernel_traffic_parses():
     kt = os.path.join(ROOT, "profiles", "match_filt_tile_match20x2048_kernel_trace.txt")
     ns = _avg_ns(kt, TILE)
     assert ns is not None and 5e5 < ns < 5e6
+
+
+def _room1080_like():
+    """views and jobs of the room1080 shape (20 x 1920x1080, K = 8) without
+    rendering: the floor depends on sizes and source lists only."""
+    from types import SimpleNamespace
+    views = [SimpleNamespace(w=1920, h=1080) for _ in range(20)]
+    jobs = [(r, [s for s in range(20) if s != r][:8], 1.0, 10.0) for r in range(20)]
+    return views, jobs
+
+
+def test_roofline_object_is_a_bound():
+    """The default line's roofline: the pass kernel against the VALU issue
+    peak, 0 < frac <= 1 at the committed profile's launch time; the measured
+    HBM traffic is a fraction of the HBM peak and at least the unique-bytes
+    floor; the algorithmic byte rate is a separate, labelled field."""
+    ns = _avg_ns(os.path.join(ROOT, "profiles", "stereo_room1080_kernel_trace.txt"), PASS)
+    views, jobs = _room1080_like()
+    floor = bench.pass_unique_bytes(views, jobs)
+    assert floor == 20 * 1920 * 1080 * (56 + 8 * 8)
+    r = bench.stereo_roofline(PASS, ns / 1e6, bench.profiled_traffic(PASS), floor, 49917.9, 5.85e11, "test")
+    assert r["bound"] == "valu" and r["unit"] == "T lane-ops/s"
+    assert 0.0 < r["frac"] <= 1.0
+    assert r["achieved"] == pytest.approx(r["frac"] * r["peak"], rel=1e-3)
+    assert 0.0 < r["hbm"]["frac"] <= 1.0
+    assert r["hbm"]["traffic_over_floor"] >= 1.0
+    assert r["algorithmic"]["GBs"] == 49917.9
+    # the init kernel on the same basis
+    init = "mvs::init_kernel<8, 8, 3>"
+    ns_i = _avg_ns(os.path.join(ROOT, "profiles", "stereo_room1080_kernel_trace.txt"), init)
+    v = bench.valu_roofline(init, ns_i / 1e6)
+    assert v is not None and 0.0 < v["frac"] <= 1.0
+
+
+def test_network_flops_from_the_layer_table():
+    """net_conv_flops counts 2 x 9 x cin x cout per output pixel: at one
+    level-0 pixel of a 2x2-subsampled 2x2 view every level > 0 layer sees
+    0 pixels."""
+    from types import SimpleNamespace
+    tiny = [SimpleNamespace(w=2, h=2)]
+    lvl0 = sum(2 * 9 * ci * co for ci, co, lev in bench.NET_LAYERS if lev == 0)
+    assert bench.net_conv_flops(tiny) == lvl0
+    assert bench.net_conv_flops([SimpleNamespace(w=1920, h=1080)]) > 1e11

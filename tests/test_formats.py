@@ -339,3 +339,38 @@ def test_depth_map_container(tmp_path):
     open(p, "wb").write(b"garbage!" + b[8:])
     with pytest.raises(F.FormatError):
         F.read_depth_map(p)
+
+
+def test_rust_f64_debug():
+    """`{:?}` of f64 as Rust prints it (core::fmt::float: positional for
+    1e-4 <= |x| < 1e16, exponential outside, shortest round-trip digits)."""
+    from rustcolmap_amd.formats import rust_f64_debug as d
+    cases = [(1.0, "1.0"), (10.0, "10.0"), (0.1, "0.1"), (-2.5, "-2.5"), (0.0, "0.0"), (-0.0, "-0.0"),
+             (1e-4, "0.0001"), (9.99e-5, "9.99e-5"), (1e-5, "1e-5"), (1e16, "1e16"), (1.5e16, "1.5e16"),
+             (9999999999999998.0, "9999999999999998.0"), (123456.789, "123456.789"), (1.5e300, "1.5e300"),
+             (2.2250738585072014e-308, "2.2250738585072014e-308"), (0.30000000000000004, "0.30000000000000004"),
+             (12345678901234567.0, "1.2345678901234568e16"), (float("inf"), "inf"), (float("-inf"), "-inf"),
+             (4.0 * 0.8, "3.2"), (1.25 * 10, "12.5")]
+    for x, want in cases:
+        assert d(x) == want, (x, d(x), want)
+    assert d(float("nan")) == "NaN"
+
+
+def test_library_rust_f64_debug_matches():
+    """The library's own `{:?}` formatter (host code of the log lines) gives
+    the same strings as the restatement above, on edge cases and on random
+    doubles over the whole exponent range."""
+    import ctypes as C
+    import numpy as np
+    from rustcolmap_amd import _lib
+    from rustcolmap_amd.formats import rust_f64_debug
+    lib = _lib.load()
+    buf = C.create_string_buffer(64)
+    rng = np.random.default_rng(4)
+    xs = [1.0, 10.0, 0.1, -2.5, 0.0, -0.0, 1e-4, 9.99e-5, 1e-5, 1e16, 1.5e16, 9999999999999998.0, 5e-324,
+          1.7976931348623157e308, 0.30000000000000004, 12345678901234567.0, 2.0 ** 53, 1.0 / 3.0]
+    xs += list(rng.uniform(0.1, 100.0, 2000)) + list(np.exp(rng.uniform(-700, 700, 2000)) * rng.choice([-1, 1], 2000))
+    xs += list(np.round(rng.uniform(0, 50, 500), 2))
+    for x in xs:
+        _lib.check(lib.mvs_debug_rust_f64(float(x), buf, 64), "mvs_debug_rust_f64")
+        assert buf.value.decode() == rust_f64_debug(float(x)), (repr(x), buf.value, rust_f64_debug(float(x)))

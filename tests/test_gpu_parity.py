@@ -359,8 +359,9 @@ def test_run_mvs_matches_oracle_pipeline(ctx, tmp_path):
     sc = scene("corner320")
     views = view_data(sc)
     xyz, tracks = sparse_points(sc, 400, seed=1)
-    opts = MvsOptions(num_sources=4, sweeps=2)
-    st = run_mvs(views, xyz, tracks, str(tmp_path), opts, ctx=ctx)
+    opts = MvsOptions(num_sources=4, sweeps=2, save_depth_vis=True)
+    lines = []
+    st = run_mvs(views, xyz, tracks, str(tmp_path), opts, ctx=ctx, log=lines.append)
     src, ranges, fus, _ = plan(views, xyz, tracks, opts)
     ov = oracle.OracleViews(views)
     dms = []
@@ -392,6 +393,34 @@ def test_run_mvs_matches_oracle_pipeline(ctx, tmp_path):
     assert web == formats.dense_web_json(pts)
     import json
     assert json.loads(web)["n"] <= len(exp)
+    # save_depth_vis (lib.rs:383-392): <stem>_depth.png of every depth map,
+    # the pixels of fuse.rs:243-260 for the (oracle-equal) depth map
+    for i, d in enumerate(dms):
+        png = tmp_path / "dense" / "depth_vis" / ("v%03d_depth.png" % i)
+        assert png.exists() == (d is not None), i
+        if d is not None:
+            assert np.array_equal(formats.read_png_gray8(png.read_bytes()), formats.depth_vis(d)), i
+    # the reference's log lines (lib.rs:212-216, 344-347, 362-371, 376-380,
+    # 391, 509-515; fuse.rs:89-92, 170-174), views in index order
+    import re
+    dbg = formats.rust_f64_debug
+    with_src = [i for i in range(len(views)) if src[i]]
+    want = [re.escape("mvs: loaded %d views at working resolution (max_dim 900)" % len(views)),
+            re.escape("mvs: %d views with >=1 source view (K = 4)" % len(with_src))]
+    want += [re.escape("mvs: [v%03d] depth %s-%s (%d srcs) valid %d/%d px in " % (
+        i, dbg(ranges[i][0]), dbg(ranges[i][1]), len(src[i]), int((dms[i] > 0).sum()), views[i].w * views[i].h))
+        + r"\d+\.\ds" for i in with_src]
+    want += [r"mvs: stereo done in \d+\.\ds \(%d depth maps\)" % len(with_src),
+             re.escape("mvs: depth visualizations saved")]
+    ext = [np.sort(xyz[:, a])[[int(len(xyz) * 0.02), min(int(len(xyz) * 0.98), len(xyz) - 1)]] for a in range(3)]
+    diag = float(np.sqrt(sum((hi - lo) ** 2 for lo, hi in ext)))
+    want += [re.escape("mvs: fusion voxel cell %.4f (robust sparse diag %.2f)" % (cell, diag)),
+             re.escape("mvs: %d consistency-passed candidates from %d depth maps" % (ncand, len(with_src))),
+             re.escape("mvs: fused %d dense points -> %s/dense/dense.ply in " % (len(exp), tmp_path))
+             + r"\d+\.\ds \(total \d+\.\ds\)"]
+    assert len(lines) == len(want), lines
+    for got, pat in zip(lines, want):
+        assert re.fullmatch(pat, got), (got, pat)
 
 
 _UNEVEN = {}
