@@ -18,13 +18,19 @@ algorithm (imageops/sample.rs `resize`, `vertical_sample`,
   their f32 sum (accumulated in tap order), and the output is
   sum over taps of value * weight accumulated in f32 in tap order;
   the horizontal pass clamps to [0, 255] and rounds half away from zero.
-* luma of RGB: 0.2126 r + 0.7152 g + 0.0722 b in f32, left to right,
-  truncated to u8; a one-channel image is its own luma.
+* luma of RGB (color.rs `rgb_to_luma` with `SRGB_LUMA` = [2126, 7152, 722]
+  and `SRGB_LUMA_DIV` = 10000): the weighted sum in integers of the
+  `Enlargeable` larger type (u32 for u8), divided by 10000 with truncation;
+  a one-channel image is its own luma.
 
-Parity unpinned: the crate cannot be run here and the reference ships no
-vectors for this stage.  The restatement is pinned by closed forms only
-(exact 2x downscale weights [1, 3, 3, 1] / 8, constant images stay constant,
-identity at equal size, separability); see tests/test_input_stage.py.
+Parity: the crate cannot be run here and the reference ships no vectors for
+this stage.  The resize restatement is pinned by closed forms only (exact
+2x downscale weights [1, 3, 3, 1] / 8, constant images stay constant,
+identity at equal size, separability) and stays "parity unpinned" for
+general inputs.  The luma is integer arithmetic whose closed forms fix it:
+every grey (v, v, v) maps to v (the weights sum to the divisor), the
+primaries to floor(255 w / 10000) and each value to floor(sum / 10000);
+see tests/test_input_stage.py.
 """
 from __future__ import annotations
 
@@ -89,12 +95,16 @@ def resize_triangle(img: np.ndarray, mw: int, mh: int) -> np.ndarray:
     return res[:, :, 0] if squeeze else res
 
 
+SRGB_LUMA = (2126, 7152, 722)
+SRGB_LUMA_DIV = 10000
+
+
 def rgb_to_luma(rgb: np.ndarray) -> np.ndarray:
-    """u8 (h, w, 3) -> u8 (h, w): trunc((0.2126 r + 0.7152 g) + 0.0722 b) in f32."""
-    p = np.asarray(rgb, np.uint8).astype(np.float32)
-    l = ((F(0.2126) * p[..., 0]).astype(np.float32) + (F(0.7152) * p[..., 1]).astype(np.float32)).astype(np.float32)
-    l = (l + (F(0.0722) * p[..., 2]).astype(np.float32)).astype(np.float32)
-    return np.trunc(l).astype(np.uint8)
+    """u8 (h, w, 3) -> u8 (h, w): (2126 r + 7152 g + 722 b) / 10000 in u32,
+    truncating (the result never exceeds 255, so the clamp is a no-op)."""
+    p = np.asarray(rgb, np.uint8).astype(np.uint32)
+    l = SRGB_LUMA[0] * p[..., 0] + SRGB_LUMA[1] * p[..., 1] + SRGB_LUMA[2] * p[..., 2]
+    return (l // SRGB_LUMA_DIV).astype(np.uint8)
 
 
 def load_view_pixels(img: np.ndarray, mw: int, mh: int):

@@ -9,8 +9,9 @@
 // centre = (o + 0.5) * ratio, weights max(1 - |(i - (centre - 0.5)) / sratio|, 0)
 // normalised by their f32 sum, values accumulated in f32 in tap order; the
 // horizontal pass clamps to [0, 255] and rounds half away from zero.  Luma
-// of RGB is trunc((0.2126 r + 0.7152 g) + 0.0722 b) in f32.  One thread per
-// output pixel and pass; all channels of a pixel in the same thread.
+// of RGB is rgb_to_luma of the crate's color.rs: (2126 r + 7152 g + 722 b)
+// / 10000 in u32 with truncation.  One thread per output pixel and pass;
+// all channels of a pixel in the same thread.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -83,8 +84,8 @@ __global__ void resize_horizontal_kernel(const float* __restrict__ tmp, int w, i
 __global__ void luma_kernel(const uint8_t* __restrict__ rgb, int64_t n, uint8_t* __restrict__ gray) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float r = (float)rgb[3 * i], g = (float)rgb[3 * i + 1], b = (float)rgb[3 * i + 2];
-    gray[i] = (uint8_t)truncf((0.2126f * r + 0.7152f * g) + 0.0722f * b);
+    const uint32_t l = 2126u * rgb[3 * i] + 7152u * rgb[3 * i + 1] + 722u * rgb[3 * i + 2];
+    gray[i] = (uint8_t)(l / 10000u);   // <= 255: the sum of the weights is the divisor
 }
 
 hipError_t launch_resize(const uint8_t* src, int channels, int w, int h, int mw, int mh, float* tmp, uint8_t* out,

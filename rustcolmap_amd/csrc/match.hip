@@ -301,15 +301,16 @@ __global__ __launch_bounds__(1024) void select_kernel(const PairDesc* __restrict
 //
 // For safe sets (every nonzero component in [2^-40, 2^40]) the distance
 // s(i, j) is first estimated from the MFMA product of the bf16 split
-// x = hi + lo (+ r, |r| <= 2^-18 |x|):
+// x = hi + lo + r (|lo| <= 2^-8 |x|, |r| <= 2^-17 |x|: bf16 keeps 8
+// significant bits):
 //   A(i, j) = |a_i|^2 + |b_j|^2 - 2 (hi_a.hi_b + hi_a.lo_b + lo_a.hi_b),
 // with |s(i, j) - A(i, j)| <= EPS_REL (|a_i|^2 + |b_j|^2) =: E(i, j).  The
-// bound covers the neglected lo.lo and r terms (<= 3.1 * 2^-18 sum |a_k b_k|),
-// f32 accumulation of the 384 exact bf16 products and of the norms, the
-// final combination, and the rounding of the reference's own sum
-// (<= 130 * 2^-24 |a - b|^2); all together below 6e-5 (|a|^2 + |b|^2),
-// an eighth of EPS_REL.  The component range keeps every product and
-// partial sum a normal f32, so the relative bounds hold.
+// bound covers the neglected lo.lo and r terms (<= 2^-15 sum |a_k b_k|, so
+// <= 2^-15 (|a|^2 + |b|^2) in A), f32 accumulation of the 384 exact bf16
+// products and of the norms, the final combination, and the rounding of the
+// reference's own sum (<= 130 * 2^-24 |a - b|^2); all together below
+// 8e-5 (|a|^2 + |b|^2), a sixth of EPS_REL.  The component range keeps
+// every product and partial sum a normal f32, so the relative bounds hold.
 //
 // Pass 1: per descriptor, T = the second smallest upper bound U = A + E.
 //   At least two exact distances are <= T, so the exact second-nearest

@@ -38,3 +38,18 @@ def test_views_from_native_images(ctx):
         eg, ec = I.load_view_pixels(native[im.name][1], 400, 300)
         assert (v.w, v.h) == (400, 300) and np.array_equal(v.gray, eg) and np.array_equal(v.rgb, ec)
         assert v.cam.f == 350.0 and v.cam.cx == 200.0 and v.cam.cy == 150.0
+
+
+def test_luma_every_colour(ctx):
+    """Every one of the 2^24 RGB colours (a 4096 x 4096 image, resized at
+    equal size, which is a copy) gets the integer luma of the restatement."""
+    from oracle import image_ops as I
+    from rustcolmap_amd.mvs import resize_image
+    v = np.arange(1 << 24, dtype=np.uint32)
+    rgb = np.stack([(v >> 16) & 255, (v >> 8) & 255, v & 255], axis=-1).astype(np.uint8).reshape(4096, 4096, 3)
+    g, c = resize_image(ctx, rgb, 4096, 4096)
+    assert np.array_equal(c, rgb)
+    assert np.array_equal(g, I.rgb_to_luma(rgb))
+    grey = rgb[..., 0] == rgb[..., 1]
+    grey &= rgb[..., 1] == rgb[..., 2]
+    assert np.array_equal(g[grey], rgb[..., 0][grey])

@@ -47,10 +47,25 @@ def test_resize_closed_forms():
 
 
 def test_luma_closed_forms():
+    """image 0.24.9's integer luma: every grey is its own luma (the weights
+    2126 + 7152 + 722 sum to the divisor 10000), the primaries give
+    floor(255 w / 10000), adding 1 to one channel never lowers the luma,
+    and every value is the floor of the weighted sum / 10000 -- checked
+    against exact rational arithmetic over a sample of all 2^24 colours."""
+    grey = np.repeat(np.arange(256, dtype=np.uint8)[None, :, None], 3, 2)
+    assert np.array_equal(I.rgb_to_luma(grey)[0], np.arange(256))
     rgb = np.array([[[255, 0, 0], [0, 255, 0], [0, 0, 255], [0, 0, 0], [10, 10, 10]]], np.uint8)
-    g = I.rgb_to_luma(rgb)[0].tolist()
-    assert g[:4] == [54, 182, 18, 0]               # trunc(0.2126 * 255), trunc(0.7152 * 255), trunc(0.0722 * 255)
-    assert g[4] in (9, 10)                         # 10 * (0.2126 + 0.7152 + 0.0722) in f32, truncated
+    assert I.rgb_to_luma(rgb)[0].tolist() == [54, 182, 18, 0, 10]
+    from fractions import Fraction
+    rng = np.random.default_rng(2)
+    c = rng.integers(0, 256, size=(1, 4000, 3), dtype=np.uint8)
+    got = I.rgb_to_luma(c)[0]
+    for (r, g, b), l in zip(c[0].tolist(), got.tolist()):
+        assert l == int(Fraction(2126 * r + 7152 * g + 722 * b, 10000))
+    for ch in range(3):
+        up = c.copy()
+        up[..., ch] = np.minimum(up[..., ch].astype(int) + 1, 255).astype(np.uint8)
+        assert np.all(I.rgb_to_luma(up) >= I.rgb_to_luma(c))
     gray = np.arange(12, dtype=np.uint8).reshape(3, 4)
     gg, rr = I.load_view_pixels(gray, 4, 3)
     assert np.array_equal(gg, gray) and np.array_equal(rr, np.repeat(gray[:, :, None], 3, 2))

@@ -122,3 +122,43 @@ def test_filter_model(name, a, b):
             assert np.median(nc) <= 4 and nfull <= len(M) // 50
         if name == "clusters":
             assert nfull > 0   # the full-scan fallback is exercised
+
+
+def _tie_components(rng, n, scale_exp):
+    """Components at bf16 rounding ties: x = s (1 + 2^-8) rounds to hi = s
+    (lo = 2^-8 s, the largest lo), x = s (1 + 2^-8 + 2^-17) leaves
+    r = 2^-17 s (the largest residual); mixed signs and binary scales."""
+    base = np.array([1.0 + 2.0 ** -8, 1.0 + 2.0 ** -8 + 2.0 ** -17, 1.0 + 3 * 2.0 ** -9, 1.0 - 2.0 ** -9], np.float64)
+    x = rng.choice(base, size=(n, 128)) * rng.choice([-1.0, 1.0], size=(n, 128))
+    x *= 2.0 ** rng.integers(-scale_exp, scale_exp + 1, size=(n, 1))
+    return x.astype(np.float32)
+
+
+def test_split_residuals_within_their_bounds():
+    """bf16 keeps 8 significant bits: |lo| <= 2^-8 |x| and |r| <= 2^-17 |x|,
+    both reached by the tie components."""
+    rng = np.random.default_rng(9)
+    x = np.concatenate([_tie_components(rng, 64, 6).ravel(), rng.normal(size=100000).astype(np.float32)])
+    hi, lo = split(x)
+    r = x.astype(np.float64) - hi.astype(np.float64) - lo.astype(np.float64)
+    ax = np.abs(x.astype(np.float64))
+    assert np.all(np.abs(lo) <= 2.0 ** -8 * ax)
+    assert np.all(np.abs(r) <= 2.0 ** -17 * ax)
+    assert np.max(np.abs(lo) / ax) > 2.0 ** -8 * 0.99        # the bounds are reached
+    assert np.max(np.abs(r) / ax) > 2.0 ** -17 * 0.99
+
+
+@pytest.mark.parametrize("scale_exp", [0, 5])
+def test_estimate_error_on_bf16_ties(scale_exp):
+    """Adversarial sets whose components sit at bf16 rounding ties (largest
+    lo and residual): the estimate's error against the reference's f32 sum
+    stays below 8e-5 (|a|^2 + |b|^2), a sixth of EPS_REL; identical and
+    sign-flipped descriptors are included (exact distance 0 and the largest
+    cross terms)."""
+    rng = np.random.default_rng(10 + scale_exp)
+    a = _tie_components(rng, 96, scale_exp)
+    b = np.concatenate([a[:32], -a[32:48], _tie_components(rng, 48, scale_exp)])
+    ref = seq_f32(a, b).astype(np.float64)
+    t, A = estimate(a, b)
+    rel = np.abs(A.astype(np.float64) - ref) / t.astype(np.float64)
+    assert rel.max() < 8e-5 < EPS_REL / 6
