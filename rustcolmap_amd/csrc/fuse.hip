@@ -13,15 +13,17 @@
 //   4. integer voxel hash: 63-bit packed keys (21 bits per axis, relative to
 //      the per-axis minimum), open addressing with 64-bit atomicCAS; each
 //      candidate learns its slot.
-//   5. stable radix sort of (slot, candidate) pairs: candidates of one
-//      voxel become contiguous and stay in reference order, so the f64
-//      coordinate sums and the wrapping u16 colour sums are accumulated in
-//      exactly the reference's sequence (fuse.rs:178-201).
-//   6. one thread per voxel run: sums, averages, round() as u8.
+//   5. grouping by slot: member counts per slot (atomics), their exclusive
+//      scan, placement of every candidate into its slot's segment, then each
+//      segment ordered by candidate index -- candidates of one voxel become
+//      contiguous and in reference order, so the f64 coordinate sums and the
+//      wrapping u16 colour sums are accumulated in exactly the reference's
+//      sequence (fuse.rs:178-201).
+//   6. one thread per voxel segment: sums, averages, round() as u8.
 //   7. compaction of voxel results by their first candidate index: output
 //      in first-insertion order.
-#include <hipcub/hipcub.hpp>
-
+// The compactions and scans are this file's own three-phase kernels (block
+// sums, a one-block scan of the block sums, block-local scan + scatter).
 #include "stereo_kernels.h"
 
 namespace mvs {
@@ -192,9 +194,9 @@ __global__ __launch_bounds__(256) void insert_kernel(const long long* __restrict
     slot_of[i] = 0;
 }
 
-__global__ void iota_kernel(uint32_t* p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = (uint32_t)i;
+__global__ void fill_u32_k(uint32_t* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 __global__ void fill_u64_k(u64* p, size_t n, u64 v) {
@@ -207,21 +209,156 @@ __global__ void fill_u8_k(uint8_t* p, size_t n) {
     if (i < n) p[i] = 0;
 }
 
-// One thread per voxel run of the sorted pairs (fuse.rs:185-212).
-__global__ __launch_bounds__(256) void reduce_kernel(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval,
-                                                     int n, const int32_t* __restrict__ cid,
-                                                     const double* __restrict__ xyz, const uchar4* __restrict__ rgb,
-                                                     DevPoint* __restrict__ by_first, uint8_t* __restrict__ first_flag) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t key = skey[p];
-    if (p > 0 && skey[p - 1] == key) return;
+// --------------------------------------------------------------------------
+// order-preserving compaction and exclusive scan (three phases)
+// --------------------------------------------------------------------------
+
+static constexpr int SCAN_T = 256;             // threads per block of phases 1 and 3
+static constexpr int SCAN_I = 16;              // consecutive items per thread
+static constexpr int SCAN_B = SCAN_T * SCAN_I; // items per block
+static constexpr int SCAN2_T = 1024;           // threads of the one-block phase 2
+
+// exclusive scan of one value per thread over a block of NT threads; returns
+// the block total
+template <int NT>
+__device__ __forceinline__ uint32_t block_exscan(uint32_t v, uint32_t& total) {
+    __shared__ uint32_t s_w[NT / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_w[wv] = x;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+    for (int q = 0; q < NT / 64; ++q) {
+        if (q < wv) base += s_w[q];
+        tot += s_w[q];
+    }
+    __syncthreads();   // s_w is reused by the next call
+    total = tot;
+    return base + x - v;
+}
+
+// phase 1: the number of selected items (flag != 0) / the sum of counts of each block
+template <class T>
+__global__ __launch_bounds__(SCAN_T) void block_sum_kernel(const T* __restrict__ in, int64_t n,
+                                                           uint32_t* __restrict__ bsum) {
+    const int64_t b0 = (int64_t)blockIdx.x * SCAN_B + (int64_t)threadIdx.x * SCAN_I;
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_I; ++k)
+        if (b0 + k < n) c += (uint32_t)in[b0 + k];
+    uint32_t tot;
+    (void)block_exscan<SCAN_T>(c, tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// phase 2: exclusive scan of the nb block sums in place (one block); total -> *total
+__global__ __launch_bounds__(SCAN2_T) void scan_bsum_kernel(uint32_t* __restrict__ bsum, int nb,
+                                                            uint32_t* __restrict__ total) {
+    uint32_t carry = 0;
+    for (int base = 0; base < nb; base += SCAN2_T) {
+        const int i = base + (int)threadIdx.x;
+        const uint32_t v = i < nb ? bsum[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exscan<SCAN2_T>(v, tot);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+// phase 3 (compaction): selected item g -> out[position] = g (IDX) or src[g]
+template <class T, bool IDX>
+__global__ __launch_bounds__(SCAN_T) void compact_kernel(const uint8_t* __restrict__ flag, int64_t n,
+                                                         const uint32_t* __restrict__ boff, const T* __restrict__ src,
+                                                         T* __restrict__ out) {
+    const int64_t b0 = (int64_t)blockIdx.x * SCAN_B + (int64_t)threadIdx.x * SCAN_I;
+    uint8_t f[SCAN_I];
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_I; ++k) {
+        f[k] = b0 + k < n ? flag[b0 + k] : 0;
+        c += f[k] != 0;
+    }
+    uint32_t tot;
+    uint32_t pos = boff[blockIdx.x] + block_exscan<SCAN_T>(c, tot);
+#pragma unroll
+    for (int k = 0; k < SCAN_I; ++k)
+        if (f[k]) {
+            if constexpr (IDX) out[pos] = (T)(b0 + k);
+            else out[pos] = src[b0 + k];
+            ++pos;
+        }
+}
+
+// phase 3 (scan): out[i] = out2[i] = exclusive prefix sum of in[] at i
+__global__ __launch_bounds__(SCAN_T) void exscan_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                        const uint32_t* __restrict__ boff, uint32_t* __restrict__ out,
+                                                        uint32_t* __restrict__ out2) {
+    const int64_t b0 = (int64_t)blockIdx.x * SCAN_B + (int64_t)threadIdx.x * SCAN_I;
+    uint32_t v[SCAN_I];
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_I; ++k) {
+        v[k] = b0 + k < n ? in[b0 + k] : 0u;
+        c += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = boff[blockIdx.x] + block_exscan<SCAN_T>(c, tot);
+#pragma unroll
+    for (int k = 0; k < SCAN_I; ++k)
+        if (b0 + k < n) {
+            out[b0 + k] = run;
+            out2[b0 + k] = run;
+            run += v[k];
+        }
+}
+
+// --------------------------------------------------------------------------
+// grouping of candidates by hash slot
+// --------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void slot_count_kernel(const uint32_t* __restrict__ slot_of, int n,
+                                                         uint32_t* __restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[slot_of[i]], 1u);
+}
+
+// candidate i into its slot's segment [off[s], off[s] + cnt[s]) (cursor = a
+// copy of off); the order inside a segment is fixed afterwards
+__global__ __launch_bounds__(256) void slot_place_kernel(const uint32_t* __restrict__ slot_of, int n,
+                                                         uint32_t* __restrict__ cursor, uint32_t* __restrict__ member) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) member[atomicAdd(&cursor[slot_of[i]], 1u)] = (uint32_t)i;
+}
+
+// One thread per occupied slot (fuse.rs:185-212): its members ordered by
+// candidate index (insertion sort in place; a voxel holds few candidates),
+// then the f64 sums and wrapping u16 colour sums in that order, the
+// averages and round() as u8, stored at the first candidate's index.
+__global__ __launch_bounds__(256) void reduce_kernel(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off,
+                                                     int64_t nslot, uint32_t* __restrict__ member,
+                                                     const int32_t* __restrict__ cid, const double* __restrict__ xyz,
+                                                     const uchar4* __restrict__ rgb, DevPoint* __restrict__ by_first,
+                                                     uint8_t* __restrict__ first_flag) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslot) return;
+    const uint32_t m = cnt[s];
+    if (m == 0) return;
+    uint32_t* seg = member + off[s];
+    for (uint32_t a = 1; a < m; ++a) {
+        const uint32_t v = seg[a];
+        uint32_t b = a;
+        for (; b > 0 && seg[b - 1] > v; --b) seg[b] = seg[b - 1];
+        seg[b] = v;
+    }
     double sx = 0.0, sy = 0.0, sz = 0.0;
     uint16_t r = 0, g = 0, b = 0;
-    uint32_t cnt = 0;
-    const uint32_t first = sval[p];
-    for (int q = p; q < n && skey[q] == key; ++q) {
-        const int64_t gi = cid[sval[q]];
+    for (uint32_t q = 0; q < m; ++q) {
+        const int64_t gi = cid[seg[q]];
         sx = sx + xyz[3 * gi];
         sy = sy + xyz[3 * gi + 1];
         sz = sz + xyz[3 * gi + 2];
@@ -229,17 +366,17 @@ __global__ __launch_bounds__(256) void reduce_kernel(const uint32_t* __restrict_
         r = (uint16_t)(r + c.x);
         g = (uint16_t)(g + c.y);
         b = (uint16_t)(b + c.z);
-        ++cnt;
     }
-    const double m = (double)cnt;
+    const double md = (double)m;
     DevPoint o;
-    o.x = sx / m;
-    o.y = sy / m;
-    o.z = sz / m;
-    o.r = f64_to_u8_sat(round((double)r / m));
-    o.g = f64_to_u8_sat(round((double)g / m));
-    o.b = f64_to_u8_sat(round((double)b / m));
+    o.x = sx / md;
+    o.y = sy / md;
+    o.z = sz / md;
+    o.r = f64_to_u8_sat(round((double)r / md));
+    o.g = f64_to_u8_sat(round((double)g / md));
+    o.b = f64_to_u8_sat(round((double)b / md));
     for (int k = 0; k < 5; ++k) o.pad[k] = 0;
+    const uint32_t first = seg[0];
     by_first[first] = o;
     first_flag[first] = 1;
 }
@@ -270,12 +407,12 @@ struct Buf {
 };
 
 struct FuseScratch {
-    Buf<uint8_t> flag, first_flag, tmp;
+    Buf<uint8_t> flag, first_flag;
     Buf<double> xyz;
     Buf<uchar4> rgb;
     Buf<int32_t> cid, counts;   // counts: [0] n_sel, [1] n_pts, [2] err
     Buf<long long> key3, kr;    // kr: per-axis key min (3) and max (3)
-    Buf<uint32_t> slot, iota, skey, sval;
+    Buf<uint32_t> slot, cnt, off, cursor, member, bsum;
     Buf<DevPoint> by_first, out;
     Buf<u64> table;
 };
@@ -288,10 +425,10 @@ struct FuseScratch {
 
 void free_fuse_scratch(FuseScratch* s) {
     if (!s) return;
-    s->flag.release(); s->first_flag.release(); s->tmp.release(); s->xyz.release(); s->rgb.release();
+    s->flag.release(); s->first_flag.release(); s->xyz.release(); s->rgb.release();
     s->cid.release(); s->counts.release(); s->key3.release(); s->kr.release(); s->slot.release();
-    s->iota.release(); s->skey.release(); s->sval.release(); s->by_first.release(); s->out.release();
-    s->table.release();
+    s->cnt.release(); s->off.release(); s->cursor.release(); s->member.release(); s->bsum.release();
+    s->by_first.release(); s->out.release(); s->table.release();
     delete s;
 }
 
@@ -319,24 +456,25 @@ hipError_t run_fusion(FuseScratch*& S, const FuseArgs& a, const FuseView* h_view
     hipLaunchKernelGGL(cand_kernel, dim3(nblk(np)), dim3(256), 0, st, a.d_views, a.nviews, a.d_order, (int64_t)np,
                        a.min_consistent, a.depth_diff, S->flag.p, S->xyz.p, S->rgb.p);
     FCHK(hipGetLastError());
-    // compaction of candidates (order preserving)
-    hipcub::CountingInputIterator<int32_t> it0(0);
-    size_t t1 = 0;
-    FCHK(hipcub::DeviceSelect::Flagged(nullptr, t1, it0, S->flag.p, S->cid.p, S->counts.p, (int)np, st));
-    FCHK(S->tmp.need(t1));
-    t1 = S->tmp.cap;
-    FCHK(hipcub::DeviceSelect::Flagged(S->tmp.p, t1, it0, S->flag.p, S->cid.p, S->counts.p, (int)np, st));
-    int nsel = 0;
-    FCHK(hipMemcpyAsync(&nsel, S->counts.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    // compaction of candidates (order preserving): cid[k] = index of the k-th flagged pixel
+    const size_t nb1 = (np + SCAN_B - 1) / SCAN_B;
+    FCHK(S->bsum.need(nb1 + 1));
+    uint32_t* total = S->bsum.p + nb1;
+    hipLaunchKernelGGL((block_sum_kernel<uint8_t>), dim3((unsigned)nb1), dim3(SCAN_T), 0, st, S->flag.p, (int64_t)np,
+                       S->bsum.p);
+    hipLaunchKernelGGL(scan_bsum_kernel, dim3(1), dim3(SCAN2_T), 0, st, S->bsum.p, (int)nb1, total);
+    hipLaunchKernelGGL((compact_kernel<int32_t, true>), dim3((unsigned)nb1), dim3(SCAN_T), 0, st, S->flag.p,
+                       (int64_t)np, S->bsum.p, (const int32_t*)nullptr, S->cid.p);
+    FCHK(hipGetLastError());
+    uint32_t nsel = 0;
+    FCHK(hipMemcpyAsync(&nsel, total, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     FCHK(hipStreamSynchronize(st));
     *n_cand = nsel;
     if (nsel == 0) return hipSuccess;
     const size_t n = (size_t)nsel;
     FCHK(S->key3.need(3 * n));
     FCHK(S->slot.need(n));
-    FCHK(S->iota.need(n));
-    FCHK(S->skey.need(n));
-    FCHK(S->sval.need(n));
+    FCHK(S->member.need(n));
     FCHK(S->by_first.need(n));
     FCHK(S->first_flag.need(n));
     FCHK(S->out.need(n));
@@ -356,44 +494,51 @@ hipError_t run_fusion(FuseScratch*& S, const FuseArgs& a, const FuseView* h_view
         }
     }
     size_t cap = 1024;
-    int bits = 10;
-    while (cap < 2 * n) {
-        cap <<= 1;
-        ++bits;
-    }
+    while (cap < 2 * n) cap <<= 1;
     FCHK(S->table.need(cap));
+    FCHK(S->cnt.need(cap));
+    FCHK(S->off.need(cap));
+    FCHK(S->cursor.need(cap));
     hipLaunchKernelGGL(fill_u64_k, dim3(nblk(cap)), dim3(256), 0, st, S->table.p, cap, EMPTY_KEY);
     hipLaunchKernelGGL(fill_u64_k, dim3(1), dim3(64), 0, st, (u64*)S->counts.p, (size_t)2, 0ULL);
+    hipLaunchKernelGGL(fill_u32_k, dim3(nblk(cap)), dim3(256), 0, st, S->cnt.p, cap);
     hipLaunchKernelGGL(insert_kernel, dim3(nblk(n)), dim3(256), 0, st, S->key3.p, (int)n, S->kr.p, S->table.p,
                        (u64)(cap - 1), S->slot.p, S->counts.p + 2);
-    hipLaunchKernelGGL(iota_kernel, dim3(nblk(n)), dim3(256), 0, st, S->iota.p, (int)n);
-    FCHK(hipGetLastError());
-    // stable sort by slot: each voxel's candidates become contiguous, in reference order
-    size_t t2 = 0;
-    FCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, t2, S->slot.p, S->skey.p, S->iota.p, S->sval.p, (int)n, 0, bits,
-                                            st));
-    FCHK(S->tmp.need(t2));
-    t2 = S->tmp.cap;
-    FCHK(hipcub::DeviceRadixSort::SortPairs(S->tmp.p, t2, S->slot.p, S->skey.p, S->iota.p, S->sval.p, (int)n, 0, bits,
-                                            st));
+    // grouping by slot: counts, their exclusive scan (segment offsets), placement
+    hipLaunchKernelGGL(slot_count_kernel, dim3(nblk(n)), dim3(256), 0, st, S->slot.p, (int)n, S->cnt.p);
+    const size_t nb2 = (cap + SCAN_B - 1) / SCAN_B;
+    FCHK(S->bsum.need(nb1 > nb2 ? nb1 + 1 : nb2 + 1));
+    total = S->bsum.p + nb2;
+    hipLaunchKernelGGL((block_sum_kernel<uint32_t>), dim3((unsigned)nb2), dim3(SCAN_T), 0, st, S->cnt.p,
+                       (int64_t)cap, S->bsum.p);
+    hipLaunchKernelGGL(scan_bsum_kernel, dim3(1), dim3(SCAN2_T), 0, st, S->bsum.p, (int)nb2, total);
+    hipLaunchKernelGGL(exscan_kernel, dim3((unsigned)nb2), dim3(SCAN_T), 0, st, S->cnt.p, (int64_t)cap, S->bsum.p,
+                       S->off.p, S->cursor.p);
+    hipLaunchKernelGGL(slot_place_kernel, dim3(nblk(n)), dim3(256), 0, st, S->slot.p, (int)n, S->cursor.p,
+                       S->member.p);
     hipLaunchKernelGGL(fill_u8_k, dim3(nblk(n)), dim3(256), 0, st, S->first_flag.p, n);
-    hipLaunchKernelGGL(reduce_kernel, dim3(nblk(n)), dim3(256), 0, st, S->skey.p, S->sval.p, (int)n, S->cid.p,
-                       S->xyz.p, S->rgb.p, S->by_first.p, S->first_flag.p);
+    hipLaunchKernelGGL(reduce_kernel, dim3(nblk(cap)), dim3(256), 0, st, S->cnt.p, S->off.p, (int64_t)cap,
+                       S->member.p, S->cid.p, S->xyz.p, S->rgb.p, S->by_first.p, S->first_flag.p);
     FCHK(hipGetLastError());
-    size_t t3 = 0;
-    FCHK(hipcub::DeviceSelect::Flagged(nullptr, t3, S->by_first.p, S->first_flag.p, S->out.p, S->counts.p + 1, (int)n,
-                                       st));
-    FCHK(S->tmp.need(t3));
-    t3 = S->tmp.cap;
-    FCHK(hipcub::DeviceSelect::Flagged(S->tmp.p, t3, S->by_first.p, S->first_flag.p, S->out.p, S->counts.p + 1, (int)n,
-                                       st));
+    // voxel results in first-insertion order
+    const size_t nb3 = (n + SCAN_B - 1) / SCAN_B;
+    total = S->bsum.p + nb3;
+    hipLaunchKernelGGL((block_sum_kernel<uint8_t>), dim3((unsigned)nb3), dim3(SCAN_T), 0, st, S->first_flag.p,
+                       (int64_t)n, S->bsum.p);
+    hipLaunchKernelGGL(scan_bsum_kernel, dim3(1), dim3(SCAN2_T), 0, st, S->bsum.p, (int)nb3, total);
+    hipLaunchKernelGGL((compact_kernel<DevPoint, false>), dim3((unsigned)nb3), dim3(SCAN_T), 0, st, S->first_flag.p,
+                       (int64_t)n, S->bsum.p, S->by_first.p, S->out.p);
+    FCHK(hipGetLastError());
+    uint32_t npts = 0;
     int cnt[3] = {0, 0, 0};
+    FCHK(hipMemcpyAsync(&npts, total, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     FCHK(hipMemcpyAsync(cnt, S->counts.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
     FCHK(hipStreamSynchronize(st));
     if (cnt[2]) {
         *limit_err = 3;
         return hipSuccess;
     }
+    cnt[1] = (int)npts;
     *n_pts = cnt[1];
     *d_points = S->out.p;
     return hipSuccess;
