@@ -353,8 +353,8 @@ def main():
         % (ms_init, ms_pass, avg_pass_ms, ms_gate, n_eval_init, n_eval_pass))
     st = ctx.stats()
     launch = (st.tile_w, st.tile_h, st.persistent, st.evaluator)
-    pass_kernel = "mvs::%s<%d, %d, %d>" % ("pass_persistent_kernel" if launch[2] else "pass_kernel", launch[0],
-                                           launch[1], launch[3])
+    pass_kernel = "mvs::%s<%d, %d, %d>" % (["pass_kernel", "pass_persistent_kernel", "pass_sorted_kernel"][launch[2]],
+                                           launch[0], launch[1], launch[3])
     traffic = profiled_traffic(pass_kernel)
 
     # fusion of the resident depth maps
@@ -374,7 +374,8 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(views, srcs, opts, args.cpu_seconds)
 
-    init_kernel = "mvs::init_kernel<%d, %d, %d>" % (launch[0], launch[1], launch[3])
+    init_kernel = ("mvs::init_sub_kernel<%d, %d>" % launch[:2] if launch[2] == 2 else
+                   "mvs::init_kernel<%d, %d, %d>" % (launch[0], launch[1], launch[3]))
     init_valu = valu_roofline(init_kernel, ms_init)
     floor = pass_unique_bytes(views, jobs)
 

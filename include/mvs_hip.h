@@ -98,7 +98,7 @@ typedef struct mvs_stats {
     double ms_learned_volume; /* learned init: sweep volume (part of ms_init) */
     double ms_learned_net;    /* learned init: network forward (part of ms_init) */
     int32_t tile_w, tile_h;   /* launch shape of the last stereo call */
-    int32_t persistent;       /* 1 = persistent tile queue for the passes */
+    int32_t persistent;       /* 1 = persistent tile queue for the passes, 2 = its sorted form */
     int32_t evaluator;        /* plane_cost evaluator (see mvs_autotune) */
 } mvs_stats_t;
 
@@ -170,7 +170,9 @@ int mvs_estimate_depth_ex(mvs_ctx *ctx, int32_t ref, const int32_t *srcs, int32_
  * stay resident for mvs_fuse and are copied to the jobs' host pointers
  * where those are non-NULL.  `mode` bits 0-1: 0 = tuned schedule,
  * 1 = one launch per job and stage (unbatched), 2 = batched tile grid,
- * 3 = batched persistent tile queue; bit 2 (+4): launch directly instead
+ * 3 = batched persistent tile queue (either form; a tuned tile-grid shape
+ * becomes its persistent form, a tuned sorted shape becomes the default
+ * shape under mode 2); bit 2 (+4): launch directly instead
  * of replaying the graph, with per-stage device timings in mvs_stats. */
 int mvs_estimate_depth_batch(mvs_ctx *ctx, const mvs_job_t *jobs, int32_t njobs, const mvs_options_t *opts,
                              int32_t mode);
@@ -179,7 +181,9 @@ int mvs_estimate_depth_batch(mvs_ctx *ctx, const mvs_job_t *jobs, int32_t njobs,
  * with every candidate (tile 16x16 / 32x8 / 16x8 / 8x8; evaluator 0
  * reference order, 1 hoisted geometry, 2 divide-free abort, 3 / 4 per-tap
  * geometry with the scaling-free exact divide and the strengthened abort,
- * source-0 rows 1-2 batched or not; tile grid or persistent queue) for
+ * source-0 rows 1-2 batched or not; tile grid, persistent queue, or
+ * persistent queue with one wave per tile whose pixels run in rounds
+ * ordered by the previous pass's cost: persistent flag 2) for
  * `passes` passes, and records the fastest
  * for the batch's shape class (longest
  * side bucket x K bucket); later batches of that class use it.  Results

@@ -106,12 +106,13 @@ struct ViewState {
     DBuf<float4> plane[2];
     DBuf<float> cost[2];
     DBuf<float> depth, cost_out, normal;
+    DBuf<uint8_t> work;   // per-pixel pass cost of the sorted pass kernel (scheduling only)
     bool has_depth = false;
     int64_t n_valid = 0;
     void release() {
         gray.release(); rgb.release(); census.release(); rays.release();
         plane[0].release(); plane[1].release(); cost[0].release(); cost[1].release();
-        depth.release(); cost_out.release(); normal.release();
+        depth.release(); cost_out.release(); normal.release(); work.release();
         has_depth = false;
     }
 };
@@ -668,6 +669,7 @@ static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_op
     P.run_passes = (max_passes < 0 || max_passes > P.n_passes) ? P.n_passes : max_passes;
     // launch shape: tuned per shape class unless the mode pins it
     int maxw = 0, maxh = 0, maxk = 0;
+    int64_t total_px = 0;
     for (int j = 0; j < njobs; j++) {
         const mvs_job_t& jb = jobs[j];
         if (jb.ref < 0 || jb.ref >= (int)c->views.size()) return fail(MVS_ERR_ARG, "estimate_depth: bad ref index");
@@ -681,10 +683,14 @@ static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_op
         maxw = std::max(maxw, c->views[jb.ref].w);
         maxh = std::max(maxh, c->views[jb.ref].h);
         maxk = std::max(maxk, jb.k);
+        total_px += (int64_t)c->views[jb.ref].w * c->views[jb.ref].h;
     }
-    P.cfg = c->tuner.pick(maxw, maxh, maxk, njobs);
-    if ((mode & 3) == 2) P.cfg.persistent = 0;
-    if ((mode & 3) == 3) P.cfg.persistent = 1;
+    P.cfg = c->tuner.pick(maxw, maxh, maxk, total_px);
+    if ((mode & 3) == 2) {
+        if (P.cfg.persistent == 2) P.cfg = TileCfg();   // the sorted shapes have no tile-grid form
+        P.cfg.persistent = 0;
+    }
+    if ((mode & 3) == 3 && P.cfg.persistent == 0) P.cfg.persistent = 1;
     const int TW = P.cfg.tw, TH = P.cfg.th;
     int tile_off = 0;
     for (int j = 0; j < njobs; j++) {
@@ -698,6 +704,11 @@ static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_op
         HCHK(rv.depth.need(np));
         HCHK(rv.cost_out.need(np));
         HCHK(rv.normal.need(3 * np));
+        if (P.cfg.persistent == 2 && (!rv.work.p || rv.work.cap < np)) {
+            // any contents give the same results; zeros keep the first pass's order spatial
+            HCHK(rv.work.need(np));
+            HCHK(hipMemsetAsync(rv.work.p, 0, np, c->stream));
+        }
         JobDesc J{};
         J.census = rv.census.p;
         J.rays = rv.rays.p;
@@ -708,6 +719,7 @@ static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_op
         J.depth_out = rv.depth.p;
         J.cost_out = rv.cost_out.p;
         J.normal_out = rv.normal.p;
+        J.work = P.cfg.persistent == 2 ? rv.work.p : nullptr;
         J.w = rv.w;
         J.h = rv.h;
         J.nsrc = jb.k;
@@ -1071,7 +1083,8 @@ int mvs_autotune(mvs_ctx* c, const mvs_job_t* jobs, int32_t njobs, const mvs_opt
         static const int cand[][4] = {{16, 16, 0, 1}, {16, 8, 0, 1}, {8, 8, 0, 1},  {8, 8, 1, 1},  {16, 16, 0, 2},
                                       {16, 8, 0, 2},  {8, 8, 0, 2},  {8, 8, 1, 2},  {16, 16, 0, 0}, {16, 8, 0, 0},
                                       {16, 16, 0, 3}, {16, 16, 0, 4}, {16, 8, 0, 3}, {16, 8, 0, 4}, {8, 8, 0, 3},
-                                      {8, 8, 0, 4},   {8, 8, 1, 3},  {8, 8, 1, 4},  {32, 8, 0, 3}, {16, 16, 1, 4}};
+                                      {8, 8, 0, 4},   {8, 8, 1, 3},  {8, 8, 1, 4},  {32, 8, 0, 3}, {16, 16, 1, 4},
+                                      {24, 8, 2, 3},  {16, 8, 2, 3}, {16, 16, 2, 3}};
         const int ncand = (int)(sizeof(cand) / sizeof(cand[0]));
         const int n_passes = (int)std::max<uint32_t>(opts->sweeps, 1) * 2;
         double best_t = 1e300;

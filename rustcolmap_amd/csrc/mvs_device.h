@@ -41,6 +41,8 @@ struct JobDesc {
     int32_t tile_off;       // first tile of this job in the batch tile space
     int32_t pinhole_ok;     // k == 0 sources may skip the distortion polynomial (finite-r2 bound holds)
     u64* valid_count;       // gate: number of valid pixels (accumulated), or NULL
+    uint8_t* work;          // per pixel: window rows read in the last pass / 2 (capped at 255), or NULL;
+                            // orders the pixels of a tile in the sorted pass kernel, never read for results
 };
 
 // A view as the fusion kernels see it (fuse.rs:99-162).

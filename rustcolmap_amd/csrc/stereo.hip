@@ -519,15 +519,17 @@ __device__ __forceinline__ bool rows12_v4(const u64* __restrict__ s_cen, const f
 template <int PW, bool FRONTO, bool PIN, bool PAIR>
 __device__ __forceinline__ float eval_v4_body(const u64* __restrict__ s_cen, const float2* __restrict__ s_ray, int c,
                                               const SrcDesc* __restrict__ S, int nsrc, float nx, float ny, float nz,
-                                              float d, const AbortThr& A, const float* __restrict__ tab) {
+                                              float d, const AbortThr& A, const float* __restrict__ tab, int& nr) {
     float sum = 0.0f;
     {
         // divisors of source 0's row tests, capped by source 1's first test
         const double cap = nsrc > 1 ? 10.0 : 1e30;
         const SrcP P = load_src(S[0]);
         row_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, 0, nx, ny, nz, d, P, tab, sum);
+        nr = 1;
         if (abort_test(sum, 5.0, A)) return 1e9f;
         if (PAIR) {
+            nr = 3;
             if (rows12_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, nx, ny, nz, d, P, tab, A, fmin(10.0, cap),
                                            fmin(15.0, cap), sum))
                 return 1e9f;
@@ -535,12 +537,14 @@ __device__ __forceinline__ float eval_v4_body(const u64* __restrict__ s_cen, con
 #pragma unroll
             for (int r = 1; r < 3; ++r) {
                 row_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, r, nx, ny, nz, d, P, tab, sum);
+                nr = r + 1;
                 if (abort_test(sum, fmin((double)(r + 1) * 5.0, cap), A)) return 1e9f;
             }
         }
 #pragma unroll
         for (int r = 3; r < 5; ++r) {
             row_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, r, nx, ny, nz, d, P, tab, sum);
+            nr = r + 1;
             if (abort_test(sum, fmin((double)(r + 1) * 5.0, cap), A)) return 1e9f;
         }
     }
@@ -562,30 +566,40 @@ __device__ __forceinline__ float eval_v4_body(const u64* __restrict__ s_cen, con
 #pragma unroll
             for (int r = 0; r < 5; ++r) {
                 row_v4<PW, FRONTO, true>(s_cen, s_ray, cc, r, nx, ny, nz, d, P, tab, sum);
+                nr = 5 * si + r + 1;
                 if (abort_test(sum, fmin((double)(r + 1) * fs, cap), A)) return 1e9f;
             }
         } else {
 #pragma unroll
             for (int r = 0; r < 5; ++r) {
                 row_v4<PW, FRONTO, false>(s_cen, s_ray, cc, r, nx, ny, nz, d, P, tab, sum);
+                nr = 5 * si + r + 1;
                 if (abort_test(sum, fmin((double)(r + 1) * fs, cap), A)) return 1e9f;
             }
         }
     }
+    nr = 5 * nsrc;
     return sum / (25.0f * (float)nsrc);
 }
 
 // Entry of the fourth evaluator; jobs outside the fast-divide domain (bit 1
 // of pinhole_ok clear) take the reference-order evaluator, which holds no
-// hoisted state either.
+// hoisted state either.  `nr` receives the number of window rows the
+// evaluation read (5 per source in full; the reference-order fallback
+// reports its full count), a measure of its cost used for scheduling only.
 template <int PW, bool FRONTO, bool PAIR>
 __device__ __forceinline__ float eval_plane_v4(const u64* __restrict__ s_cen, const float2* __restrict__ s_ray, int c,
                                                const SrcDesc* __restrict__ S, int nsrc, float nx, float ny, float nz,
-                                               float d, float thr, const float* __restrict__ tab, int pin_flags) {
-    if (!(pin_flags & 2)) return eval_plane<false>(LdsRef{s_cen, s_ray, c, PW}, S, nsrc, nx, ny, nz, d, thr, nullptr);
+                                               float d, float thr, const float* __restrict__ tab, int pin_flags,
+                                               int& nr) {
+    if (!(pin_flags & 2)) {
+        nr = 5 * nsrc;
+        return eval_plane<false>(LdsRef{s_cen, s_ray, c, PW}, S, nsrc, nx, ny, nz, d, thr, nullptr);
+    }
     const AbortThr A = make_abort_thr(thr);
-    if (S[0].k == 0.0f) return eval_v4_body<PW, FRONTO, true, PAIR>(s_cen, s_ray, c, S, nsrc, nx, ny, nz, d, A, tab);
-    return eval_v4_body<PW, FRONTO, false, PAIR>(s_cen, s_ray, c, S, nsrc, nx, ny, nz, d, A, tab);
+    if (S[0].k == 0.0f)
+        return eval_v4_body<PW, FRONTO, true, PAIR>(s_cen, s_ray, c, S, nsrc, nx, ny, nz, d, A, tab, nr);
+    return eval_v4_body<PW, FRONTO, false, PAIR>(s_cen, s_ray, c, S, nsrc, nx, ny, nz, d, A, tab, nr);
 }
 
 __device__ __forceinline__ void fill_pc_table(float* tab) {
@@ -598,14 +612,14 @@ __device__ __forceinline__ void fill_pc_table(float* tab) {
 
 // Global-address-space loads (not flat ones, which also count against the
 // LDS wait counter).
-template <int TW, int TH>
+template <int TW, int TH, int NT = TW * TH>
 __device__ __forceinline__ void stage_tile(const JobDesc& J, int x0, int y0, u64* s_cen, float2* s_ray) {
     constexpr int PW = TW + 4, PH = TH + 4;
     gu64* cen = (gu64*)J.census;
     gu64* ray = (gu64*)J.rays;   // a float2 moved as its 8 bytes
     u64* s_ray8 = reinterpret_cast<u64*>(s_ray);
     const int w = J.w, h = J.h;
-    for (int i = threadIdx.x; i < PW * PH; i += TW * TH) {
+    for (int i = threadIdx.x; i < PW * PH; i += NT) {
         const int ay = i / PW, ax = i % PW;
         const int gx = min(max(x0 - 2 + ax, 0), w - 1);
         const int gy = min(max(y0 - 2 + ay, 0), h - 1);
@@ -644,8 +658,9 @@ __device__ __forceinline__ void init_tile(const JobDesc& J, const SrcDesc* __res
         const float inv = inv_min + (inv_max - inv_min) * (float)hi / (float)(nh - 1);
         const float d = 1.0f / inv;
         const float thr = (best_c + 0.02f) + 0.12f;
+        int nr;
         const float c = EV >= 3 ? eval_plane_v4<PW, true, EV == 3>(s_cen, s_ray, R.c, S, J.nsrc, 0.0f, 0.0f, 1.0f, d,
-                                                                  thr, s_tab, J.pinhole_ok)
+                                                                  thr, s_tab, J.pinhole_ok, nr)
                         : EV == 2 ? eval_plane_v3<PW>(s_cen, s_ray, R.c, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, thr, s_tab,
                                                       J.pinhole_ok != 0)
                         : EV == 1 ? eval_plane_fast<PW>(s_cen, s_ray, R.c, S, J.nsrc, 0.0f, 0.0f, 1.0f, d, thr)
@@ -760,18 +775,71 @@ __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* _
 // one pass (stereo.rs:279-369)
 // --------------------------------------------------------------------------
 
-template <int TW, int TH, int EV>
-__device__ __forceinline__ void pass_tile(const JobDesc& J, const SrcDesc* __restrict__ srcs, int x0, int y0, int par,
-                                          u64 salt, float sig_inv, float sig_n, u64* s_cen, float2* s_ray,
+// Candidate q of pixel (x, y) (window centre c of the LDS tile of pitch PW)
+// in the reference's order: q = 0..3 the neighbour planes L, R, U, D
+// carried to this pixel (stereo.rs:293-321), q = 4..6 the perturbations of
+// the current plane (stereo.rs:324-353), which draw from `seed`.  Returns
+// false where the reference skips the candidate; the draws made stay
+// consumed (a failed inv_new / zcand test consumes only g0).
+template <int PW>
+__device__ __forceinline__ bool make_candidate(int q, int x, int y, int w, int h, int c, gplane* prev,
+                                               const float2* __restrict__ s_ray, float2 ri, float4 cur, u64& seed,
+                                               float sig_inv, float sig_n, float range, float zlo, float zhi,
+                                               float& cnx, float& cny, float& cnz, float& d_plane) {
+    if (q < 4) {
+        const int dx = q == 0 ? -1 : (q == 1 ? 1 : 0);
+        const int dy = q == 2 ? -1 : (q == 3 ? 1 : 0);
+        const int xx = x + dx, yy = y + dy;
+        if (xx < 0 || yy < 0 || xx >= w || yy >= h) return false;
+        const float4 nb = as_float4(prev[(size_t)yy * w + xx]);
+        const float2 rj = s_ray[c + dy * PW + dx];
+        const float a_nb = (nb.x * rj.x + nb.y * rj.y) + nb.z;
+        if (a_nb < 0.1f) return false;
+        d_plane = nb.w * a_nb;
+        const float a_here = (nb.x * ri.x + nb.y * ri.y) + nb.z;
+        if (a_here < 0.1f) return false;
+        const float zc = d_plane / a_here;
+        if (!(zlo <= zc && zc <= zhi)) return false;
+        cnx = nb.x;
+        cny = nb.y;
+        cnz = nb.z;
+        return true;
+    }
+    const float inv = 1.0f / fmaxf(cur.w, 1e-6f);
+    const float g0 = gauss3(seed);
+    const float inv_new = inv + sig_inv * range * g0;
+    if (inv_new <= 0.0f) return false;
+    const float zc = 1.0f / inv_new;
+    if (!(zlo <= zc && zc <= zhi)) return false;
+    const float g1 = gauss3(seed);
+    float nnx = cur.x + sig_n * g1;
+    const float g2 = gauss3(seed);
+    float nny = cur.y + sig_n * g2;
+    const float g3 = gauss3(seed);
+    float nnz = cur.z + sig_n * g3;
+    const float len = sqrtf((nnx * nnx + nny * nny) + nnz * nnz);
+    if (len < 1e-6f) return false;
+    nnx = nnx / len;
+    nny = nny / len;
+    nnz = nnz / len;
+    const float a_here = (nnx * ri.x + nny * ri.y) + nnz;
+    if (a_here < 0.1f) return false;
+    d_plane = zc * a_here;
+    cnx = nnx;
+    cny = nny;
+    cnz = nnz;
+    return true;
+}
+
+// One pixel of one pass: pixel (x, y) at window centre c of the LDS tile of
+// pitch PW.  With WORK, the window rows its evaluations read, halved and
+// capped at 255, go to J.work (scheduling information only).
+template <int PW, int EV, bool WORK = false>
+__device__ __forceinline__ void pass_pixel(const JobDesc& J, const SrcDesc* __restrict__ S, int x, int y, int c,
+                                          int par, u64 salt, float sig_inv, float sig_n, u64* s_cen, float2* s_ray,
                                           unsigned& ne, const float* s_tab) {
-    constexpr int PW = TW + 4;
     const int w = J.w, h = J.h;
-    const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
-    const int x = x0 + lx, y = y0 + ly;
-    if (x >= w || y >= h) return;
-    const int c = (ly + 2) * PW + (lx + 2);
     const LdsRef R{s_cen, s_ray, c, PW};
-    const SrcDesc* S = srcs + J.src_off;
     const int nsrc = J.nsrc;
     const size_t i = (size_t)y * w + x;
     // planes and costs of the previous pass through global-address-space
@@ -785,6 +853,7 @@ __device__ __forceinline__ void pass_tile(const JobDesc& J, const SrcDesc* __res
     float b0 = cur.x, b1 = cur.y, b2 = cur.z;
     float bd = cur.w * ((b0 * ri.x + b1 * ri.y) + b2);
     const float zlo = J.zmin * 0.5f, zhi = J.zmax * 2.0f;
+    int rows = 0;
 
     // Seven candidates in the reference's order: the 4 neighbour planes
     // L, R, U, D (stereo.rs:293-321), then 3 perturbations of the current
@@ -794,56 +863,19 @@ __device__ __forceinline__ void pass_tile(const JobDesc& J, const SrcDesc* __res
     const float range = J.inv_max - J.inv_min;
     for (int q = 0; q < 7; ++q) {
         float cnx, cny, cnz, d_plane;
-        if (q < 4) {
-            const int dx = q == 0 ? -1 : (q == 1 ? 1 : 0);
-            const int dy = q == 2 ? -1 : (q == 3 ? 1 : 0);
-            const int xx = x + dx, yy = y + dy;
-            if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
-            const float4 nb = as_float4(prev[(size_t)yy * w + xx]);
-            const float2 rj = s_ray[c + dy * PW + dx];
-            const float a_nb = (nb.x * rj.x + nb.y * rj.y) + nb.z;
-            if (a_nb < 0.1f) continue;
-            d_plane = nb.w * a_nb;
-            const float a_here = (nb.x * ri.x + nb.y * ri.y) + nb.z;
-            if (a_here < 0.1f) continue;
-            const float zc = d_plane / a_here;
-            if (!(zlo <= zc && zc <= zhi)) continue;
-            cnx = nb.x;
-            cny = nb.y;
-            cnz = nb.z;
-        } else {
-            const float inv = 1.0f / fmaxf(cur.w, 1e-6f);
-            const float g0 = gauss3(seed);
-            const float inv_new = inv + sig_inv * range * g0;
-            if (inv_new <= 0.0f) continue;
-            const float zc = 1.0f / inv_new;
-            if (!(zlo <= zc && zc <= zhi)) continue;
-            const float g1 = gauss3(seed);
-            float nnx = cur.x + sig_n * g1;
-            const float g2 = gauss3(seed);
-            float nny = cur.y + sig_n * g2;
-            const float g3 = gauss3(seed);
-            float nnz = cur.z + sig_n * g3;
-            const float len = sqrtf((nnx * nnx + nny * nny) + nnz * nnz);
-            if (len < 1e-6f) continue;
-            nnx = nnx / len;
-            nny = nny / len;
-            nnz = nnz / len;
-            const float a_here = (nnx * ri.x + nny * ri.y) + nnz;
-            if (a_here < 0.1f) continue;
-            d_plane = zc * a_here;
-            cnx = nnx;
-            cny = nny;
-            cnz = nnz;
-        }
+        if (!make_candidate<PW>(q, x, y, w, h, c, prev, s_ray, ri, cur, seed, sig_inv, sig_n, range, zlo, zhi, cnx,
+                                cny, cnz, d_plane))
+            continue;
         const float thr = (best_c + 0.02f) + 0.12f;
+        int nr = 5 * nsrc;
         const float cst = EV >= 3 ? eval_plane_v4<PW, false, EV == 3>(s_cen, s_ray, c, S, nsrc, cnx, cny, cnz, d_plane,
-                                                                      thr, s_tab, J.pinhole_ok)
+                                                                      thr, s_tab, J.pinhole_ok, nr)
                           : EV == 2 ? eval_plane_v3<PW>(s_cen, s_ray, c, S, nsrc, cnx, cny, cnz, d_plane, thr, s_tab,
                                                         J.pinhole_ok != 0)
                           : EV == 1 ? eval_plane_fast<PW>(s_cen, s_ray, c, S, nsrc, cnx, cny, cnz, d_plane, thr)
                                     : eval_plane<false>(R, S, nsrc, cnx, cny, cnz, d_plane, thr, nullptr);
         ++ne;
+        rows += nr;
         if (cst < best_c) { best_c = cst; b0 = cnx; b1 = cny; b2 = cnz; bd = d_plane; }
     }
 
@@ -859,6 +891,19 @@ __device__ __forceinline__ void pass_tile(const JobDesc& J, const SrcDesc* __res
     }
     J.plane[par ^ 1][i] = outp;
     J.cost[par ^ 1][i] = outc;
+    if (WORK && J.work) J.work[i] = (uint8_t)min((rows + 1) >> 1, 255);
+}
+
+template <int TW, int TH, int EV>
+__device__ __forceinline__ void pass_tile(const JobDesc& J, const SrcDesc* __restrict__ srcs, int x0, int y0, int par,
+                                          u64 salt, float sig_inv, float sig_n, u64* s_cen, float2* s_ray,
+                                          unsigned& ne, const float* s_tab) {
+    constexpr int PW = TW + 4;
+    const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= J.w || y >= J.h) return;
+    pass_pixel<PW, EV>(J, srcs + J.src_off, x, y, (ly + 2) * PW + (lx + 2), par, salt, sig_inv, sig_n, s_cen, s_ray,
+                       ne, s_tab);
 }
 
 // --------------------------------------------------------------------------
@@ -952,6 +997,122 @@ __global__ __launch_bounds__(TW* TH) PASS_OCCUPANCY void pass_persistent_kernel(
         __syncthreads();   // LDS tile and s_tile are reused by the next iteration
     }
     block_count<TW, TH>(ne, &s_cnt, counter);
+}
+
+// Sorted form: one wave per TW x TH tile (TW * TH / 64 rounds of 64 pixels).
+// A pass's cost at a pixel is the window rows its seven evaluations read,
+// and a wave's time on a candidate is that of its slowest lane; the rows a
+// pixel read in the previous pass predict the current ones well (window
+// texture and the plane's fit change slowly between passes).  So the tile's
+// pixels are grouped by the previous pass's rows (J.work / 2, i.e. rows / 4,
+// in 16 classes, the last open) and the rounds take them 64 at a time in
+// class order, bringing pixels of similar cost into one wave.  A counting
+// sort by wave ballots keeps each class in raster order.  Results do not
+// depend on the order: every pixel reads only the previous pass's planes and
+// the LDS window, and writes only its own outputs.
+template <int TW, int TH, int EV>
+__global__ __launch_bounds__(64) PASS_OCCUPANCY void pass_sorted_kernel(const JobDesc* __restrict__ jobs,
+                                                                       const SrcDesc* __restrict__ srcs,
+                                                                       const int4* __restrict__ tiles, int ntiles,
+                                                                       int par, u64 salt, float sig_inv, float sig_n,
+                                                                       u64* __restrict__ counter,
+                                                                       int* __restrict__ queue) {
+    constexpr int NPIX = TW * TH, NG = NPIX / 64, PW = TW + 4;
+    constexpr unsigned NCLASS = 16;
+    static_assert(NPIX % 64 == 0 && NPIX <= 1024, "a whole number of 64-pixel rounds");
+    __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
+    __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
+    using perm_t = typename std::conditional<NPIX <= 256, uint8_t, uint16_t>::type;
+    __shared__ perm_t s_perm[NPIX];
+    __shared__ unsigned s_cnt;
+    __shared__ int s_tile;
+    __shared__ float s_tab[64];
+    if (threadIdx.x == 0) s_cnt = 0;
+    fill_pc_table(s_tab);
+    unsigned ne = 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(queue, 1);
+        __syncthreads();
+        const int t = s_tile;
+        if (t >= ntiles) break;
+        const int4 tl = tiles[t];
+        const JobDesc& J = jobs[tl.x];
+        const int w = J.w, h = J.h;
+        stage_tile<TW, TH, 64>(J, tl.y, tl.z, s_cen, s_ray);
+        // class of each of this lane's pixels (p = lane + 64 g); NCLASS = outside the view
+        const uint8_t* wk = J.work;
+        unsigned cls[NG];
+        int n_in = 0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const int p = (int)threadIdx.x + 64 * g;
+            const int x = tl.y + p % TW, y = tl.z + p / TW;
+            cls[g] = NCLASS;
+            if (x < w && y < h) cls[g] = wk ? min((unsigned)wk[(size_t)y * w + x] >> 1, NCLASS - 1) : 0u;
+            n_in += __popcll(__ballot(cls[g] < NCLASS));
+        }
+        // counting sort: position = pixels of lower classes + those of the
+        // same class earlier in (group, lane) order
+        int pos[NG];
+        int base = 0;
+#pragma unroll
+        for (unsigned v = 0; v < NCLASS; ++v) {
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const u64 m = __ballot(cls[g] == v);
+                const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                pos[g] = cls[g] == v ? base + below : pos[g];
+                base += __popcll(m);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            if (cls[g] < NCLASS) s_perm[pos[g]] = (perm_t)(threadIdx.x + 64 * g);
+        __syncthreads();
+        const SrcDesc* S = srcs + J.src_off;
+#pragma unroll 1
+        for (int r = 0; r * 64 < n_in; ++r) {
+            const int k = r * 64 + (int)threadIdx.x;
+            if (k >= n_in) break;
+            const int p = s_perm[k];
+            const int lx = p % TW, ly = p / TW;
+            pass_pixel<PW, EV, true>(J, S, tl.y + lx, tl.z + ly, (ly + 2) * PW + (lx + 2), par, salt, sig_inv, sig_n,
+                                     s_cen, s_ray, ne, s_tab);
+        }
+        __syncthreads();   // LDS tile, order and s_tile are reused by the next iteration
+    }
+    block_count<TW, TH>(ne, &s_cnt, counter);
+}
+
+// Init sweep for the sorted form's tiles: one single-wave block per 8x8
+// sub-tile, running init_box_tile as the 8x8 init launch does (same values
+// as every other init launch).
+template <int TW, int TH>
+__global__ __launch_bounds__(64) void init_sub_kernel(const JobDesc* __restrict__ jobs,
+                                                      const SrcDesc* __restrict__ srcs,
+                                                      const int4* __restrict__ tiles, int ntiles, int nh,
+                                                      u64* __restrict__ counter) {
+    static_assert(TW % 8 == 0 && TH % 8 == 0, "whole 8x8 sub-tiles");
+    constexpr int NSX = TW / 8, NSUB = NSX * (TH / 8);
+    __shared__ u64 s_cen[12 * 12];
+    __shared__ float2 s_ray[12 * 12];
+    __shared__ unsigned char s_code[12 * 12];
+    __shared__ unsigned s_cnt;
+    __shared__ float s_tab[64];
+    const int b = xcd_remap(blockIdx.x, ntiles * NSUB);
+    const int sub = b % NSUB;
+    const int4 tl = tiles[b / NSUB];
+    const JobDesc& J = jobs[tl.x];
+    const int x0 = tl.y + (sub % NSX) * 8, y0 = tl.z + (sub / NSX) * 8;
+    if (x0 >= J.w || y0 >= J.h) return;   // the same for the whole block
+    if (threadIdx.x == 0) s_cnt = 0;
+    fill_pc_table(s_tab);
+    stage_tile<8, 8>(J, x0, y0, s_cen, s_ray);
+    __syncthreads();
+    unsigned ne = 0;
+    init_box_tile<8, 8>(J, srcs, x0, y0, nh, true, s_cen, s_ray, s_code, ne, s_tab);
+    block_count<8, 8>(ne, &s_cnt, counter);
 }
 
 // --------------------------------------------------------------------------
@@ -1332,8 +1493,25 @@ hipError_t launch_fill_u64(u64* p, int n, u64 v, hipStream_t st) {
     X(16, 16, 0) X(16, 16, 1) X(16, 16, 2) X(32, 8, 0) X(32, 8, 1) X(32, 8, 2) X(16, 8, 0) X(16, 8, 1) X(16, 8, 2) \
     X(8, 8, 1) X(8, 8, 2) X(16, 16, 3) X(16, 16, 4) X(16, 8, 3) X(16, 8, 4) X(8, 8, 3) X(8, 8, 4) X(32, 8, 3)
 
+// sorted-form shapes (one wave per tile, evaluator 3)
+#define SORTED_CASES(X) X(16, 8) X(24, 8) X(16, 16)
+
+static hipError_t launch_init_sub(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles,
+                                  int ntiles, int nh, u64* counter, hipStream_t st) {
+#define X(W, H)                                                                                                   \
+    if (cfg.tw == W && cfg.th == H && cfg.eval == 3) {                                                            \
+        hipLaunchKernelGGL((init_sub_kernel<W, H>), dim3(ntiles * (W / 8) * (H / 8)), dim3(64), 0, st, jobs, srcs, \
+                           tiles, ntiles, nh, counter);                                                           \
+        return hipGetLastError();                                                                                 \
+    }
+    SORTED_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles,
                        int nh, u64* counter, hipStream_t st) {
+    if (cfg.persistent == 2) return launch_init_sub(cfg, jobs, srcs, tiles, ntiles, nh, counter, st);
 #define X(W, H, E)                                                                                     \
     if (cfg.tw == W && cfg.th == H && cfg.eval == E) {                                                 \
         hipLaunchKernelGGL((init_kernel<W, H, E>), dim3(ntiles), dim3(W * H), 0, st, jobs, srcs, tiles, ntiles, \
@@ -1348,6 +1526,17 @@ hipError_t launch_init(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* s
 hipError_t launch_pass(const TileCfg& cfg, const JobDesc* jobs, const SrcDesc* srcs, const int4* tiles, int ntiles,
                        int par, u64 salt, float sig_inv, float sig_n, u64* counter, int* queue, int grid_persistent,
                        hipStream_t st) {
+    if (cfg.persistent == 2) {
+#define X(W, H)                                                                                                  \
+        if (cfg.tw == W && cfg.th == H && cfg.eval == 3) {                                                       \
+            hipLaunchKernelGGL((pass_sorted_kernel<W, H, 3>), dim3(grid_persistent), dim3(64), 0, st, jobs, srcs, \
+                               tiles, ntiles, par, salt, sig_inv, sig_n, counter, queue);                        \
+            return hipGetLastError();                                                                            \
+        }
+        SORTED_CASES(X)
+#undef X
+        return hipErrorInvalidValue;
+    }
 #define X(W, H, E)                                                                                                 \
     if (cfg.tw == W && cfg.th == H && cfg.eval == E) {                                                             \
         if (cfg.persistent) {                                                                                      \
@@ -1371,13 +1560,20 @@ hipError_t launch_gate(const TileCfg& cfg, const JobDesc* jobs, const int4* tile
         hipLaunchKernelGGL((gate_kernel<W, H>), dim3(ntiles), dim3(W * H), 0, st, jobs, tiles, par, cost_thr);    \
         return hipGetLastError();                                                                                \
     }
-    X(16, 16) X(32, 8) X(16, 8) X(8, 8)
+    X(16, 16) X(32, 8) X(16, 8) X(8, 8) X(24, 8)
 #undef X
     return hipErrorInvalidValue;
 }
 
 int persistent_blocks_per_cu(const TileCfg& cfg) {
     int nb = 0;
+#define X(W, H)                                                                                          \
+    if (cfg.persistent == 2 && cfg.tw == W && cfg.th == H && cfg.eval == 3) {                            \
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pass_sorted_kernel<W, H, 3>, 64, 0);    \
+        return nb > 0 ? nb : 1;                                                                          \
+    }
+    SORTED_CASES(X)
+#undef X
 #define X(W, H, E)                                                                                           \
     if (cfg.tw == W && cfg.th == H && cfg.eval == E) {                                                       \
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pass_persistent_kernel<W, H, E>, W * H, 0); \
@@ -1402,7 +1598,7 @@ hipError_t launch_volume(const JobDesc* jobs, const SrcDesc* srcs, const int4* t
 
 #define PROPOSAL_CASES(X) \
     X(16, 16, 0) X(16, 16, 1) X(16, 16, 2) X(16, 16, 3) X(16, 16, 4) X(16, 8, 3) X(16, 8, 4) X(8, 8, 3) X(8, 8, 4) \
-    X(32, 8, 3)
+    X(32, 8, 3) X(24, 8, 3)
 
 bool proposal_supported(const TileCfg& cfg) {
 #define X(W, H, E) \

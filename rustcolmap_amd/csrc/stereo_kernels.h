@@ -20,12 +20,12 @@ struct VolOut {
     float* volf;     // [pix][nplanes] fp32 plane costs
 };
 
-// Launch shape of the stereo stages (picked per shape class by the tuner).
-// Launch shape of the stereo kernels.  The default is the shape the tuner
+// Launch shape of the stereo kernels (picked per shape class by the tuner).  The default is the shape the tuner
 // picks at 1080p (8x8 tiles, persistent passes, evaluator 3).
 struct TileCfg {
     int tw = 8, th = 8;
-    int persistent = 1;   // 1: persistent grid over an atomic tile queue
+    int persistent = 1;   // 1: persistent grid over an atomic tile queue; 2: the same with one wave per tile,
+                          // its pixels in rounds of 64 ordered by the previous pass's cost (evaluator 3)
     int eval = 3;         // plane_cost evaluator: 0 = reference loop order, 1 = hoisted geometry, 2 = + no-divide
                           // abort, 3/4 = per-tap geometry, exact fast divide, strengthened abort, shared-cost init
 };

@@ -288,7 +288,7 @@ class MvsContext:
         oc = opts.to_c()
         check(self._lib.mvs_autotune(self._h, arr, len(jobs), C.byref(oc), int(passes), ptr(best), ptr(ms), ptr(cfg),
                                      ptr(nc)), "mvs_autotune")
-        self.last_tune = [("%dx%d%s%s" % (c[0], c[1], "p" if c[2] else "", "r.vwx"[c[3]] if c[3] != 1 else ""),
+        self.last_tune = [("%dx%d%s%s" % (c[0], c[1], ["", "p", "s"][c[2]], "r.vwx"[c[3]] if c[3] != 1 else ""),
                            float(t)) for c, t in zip(cfg[:nc[0]], ms[:nc[0]])]
         return tuple(int(b) for b in best)
 

@@ -46,7 +46,7 @@ def _states():
     return out
 
 
-@pytest.mark.parametrize("shape", ["8x8pw", "16x16", "16x8r", "8x8v", "32x8w", "16x16px"])
+@pytest.mark.parametrize("shape", ["8x8pw", "16x16", "16x8r", "8x8v", "32x8w", "16x16px", "16x16sw", "24x8sw"])
 def test_constructed_pass_states(ctx, shape, monkeypatch):
     monkeypatch.setenv("MVS_TILE", shape)
     ov = pins._scene()
@@ -77,7 +77,7 @@ def _k12_views():
     return [dataclasses.replace(v, cam=dataclasses.replace(v.cam, k=KS[i])) for i, v in enumerate(views)]
 
 
-@pytest.mark.parametrize("shape", ["8x8pw", "16x16", "16x8v", "32x8w"])
+@pytest.mark.parametrize("shape", ["8x8pw", "16x16", "16x8v", "32x8w", "16x8sw"])
 def test_k12_every_pass(ctx, shape, monkeypatch):
     import oracle
     from rustcolmap_amd.mvs import MvsOptions

@@ -307,7 +307,7 @@ def test_fusion_resident_matches_host_maps(ctx):
 
 @pytest.mark.parametrize("shape", ["16x16", "16x16r", "32x8", "32x8p", "16x8r", "16x8p", "8x8", "16x16pr", "8x8p",
                                    "16x16v", "8x8v", "32x8pv", "16x8v", "16x16w", "16x16x", "16x8w", "16x8px",
-                                   "8x8w", "8x8px", "8x8pw", "32x8w", "16x16pw"])
+                                   "8x8w", "8x8px", "8x8pw", "32x8w", "16x16pw", "16x8sw", "16x16sw", "24x8sw"])
 def test_every_launch_shape_is_exact(ctx, shape, monkeypatch):
     """Tile shape, evaluator and schedule change only speed, never values."""
     import oracle
@@ -446,7 +446,7 @@ def _uneven_case():
     return _UNEVEN
 
 
-@pytest.mark.parametrize("shape", ["8x8pw", "16x16w", "32x8w", "16x8r"])
+@pytest.mark.parametrize("shape", ["8x8pw", "16x16w", "32x8w", "16x8r", "16x16sw", "24x8sw"])
 def test_uneven_sizes_batch_is_exact(ctx, shape, monkeypatch):
     """One batch over views of different, odd sizes in several launch
     families, single-pass and persistent schedules alike."""

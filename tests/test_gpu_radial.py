@@ -54,7 +54,7 @@ def test_radial_every_pass(ctx):
     assert (bits(c0) != bits(c1)).mean() > 0.5
 
 
-@pytest.mark.parametrize("shape", ["16x16", "16x8r", "8x8v", "16x16w", "8x8pw", "8x8px", "32x8w"])
+@pytest.mark.parametrize("shape", ["16x16", "16x8r", "8x8v", "16x16w", "8x8pw", "8x8px", "32x8w", "16x16sw", "24x8sw"])
 def test_radial_launch_shapes(ctx, shape, monkeypatch):
     import oracle
     from rustcolmap_amd.mvs import MvsOptions
