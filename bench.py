@@ -49,6 +49,15 @@ VALU_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12
 MFMA_BF16_PEAK_TFLOPS = 2500.0             # dense bf16 MFMA
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "pass_kernel_room1080_traffic.txt")
 KERNEL_SUMMARY = os.path.join(ROOT, "profiles", "stereo_room1080_kernel_trace.txt")
+FUSION_SUMMARY = os.path.join(ROOT, "profiles", "fusion_room1080_counters.txt")
+LEARNED_SUMMARY = os.path.join(ROOT, "profiles", "learned_room1080_kernel_trace.txt")
+FP64_PEAK_TFLOPS = 78.6                    # FP64 vector
+HBM_PEAK_GBS = 8000.0
+# every kernel a fusion run launches (csrc/fuse.hip)
+FUSION_KERNELS = ("mvs::cand_kernel", "mvs::key_init_kernel", "mvs::key_kernel", "mvs::insert_kernel",
+                  "mvs::reduce_kernel", "mvs::compact_kernel", "mvs::exscan_kernel", "mvs::block_sum_kernel",
+                  "mvs::scan_bsum_kernel", "mvs::slot_count_kernel", "mvs::slot_place_kernel", "mvs::fill_u32_k",
+                  "mvs::fill_u64_k", "mvs::fill_u8_k")
 
 
 def profiled_traffic(kernel, path=TRAFFIC_SUMMARY):
@@ -84,6 +93,80 @@ def profiled_counter(kernel, counter, path):
     return None
 
 
+def profiled_stage_total(kernels, counter, path):
+    """Sum over the kernels named in `kernels` (template arguments aside) of
+    dispatches x per-dispatch value of `counter` (the copy-calibrated bytes
+    where the summary gives them) in a committed --rocprof-summary file;
+    None when the file or the counter is absent."""
+    if not os.path.exists(path):
+        return None
+    total, seen = 0.0, 0
+    tag = " %s " % counter
+    with open(path) as f:
+        for line in f:
+            if tag not in line or "dispatches" not in line:
+                continue
+            if line.split(tag)[0].strip().split("<")[0] not in kernels:
+                continue
+            n = int(line.split("dispatches")[1].split()[0])
+            val = float(line.split("->")[1].split()[0]) if "->" in line else float(line.split(" avg ")[1].split()[0])
+            total += n * val
+            seen += 1
+    return (total, os.path.relpath(path, ROOT)) if seen else None
+
+
+def profiled_kernel_avg_ns(kernel, path):
+    """Average duration (ns) of `kernel` from the kernel table of a committed
+    --rocprof-summary file (kernel, calls, total_ns, avg_ns, pct)."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith(kernel + " ") and "dispatches" not in line:
+                parts = line[len(kernel):].split()
+                if len(parts) >= 4:
+                    return float(parts[2])
+    return None
+
+
+def fusion_fractions(ms_fuse, path=FUSION_SUMMARY):
+    """HBM and FP64 rates of one fusion run: rocprof totals over every fusion
+    kernel (FETCH_SIZE + WRITE_SIZE copy-calibrated bytes, and
+    SQ_INSTS_VALU_FLOPS_FP64) of the committed profile of the same maps, over
+    the measured fusion stage time."""
+    fetch = profiled_stage_total(FUSION_KERNELS, "FETCH_SIZE", path)
+    write = profiled_stage_total(FUSION_KERNELS, "WRITE_SIZE", path)
+    fp64 = profiled_stage_total(FUSION_KERNELS, "SQ_INSTS_VALU_FLOPS_FP64", path)
+    if ms_fuse <= 0 or fetch is None or write is None:
+        return None
+    b = fetch[0] + write[0]
+    out = {"hbm": {"bytes_per_run": int(b), "GBs": round(b / (ms_fuse / 1000.0) / 1e9, 1), "peak_GBs": HBM_PEAK_GBS,
+                   "frac": round(b / (ms_fuse / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)},
+           "fp64": None, "source": fetch[1],
+           "definition": "rocprof totals over every fusion kernel of one fusion of the same maps, over the "
+                         "measured fusion stage time"}
+    if fp64 is not None:
+        out["fp64"] = {"flops_per_run": int(fp64[0]),
+                       "TFLOPs": round(fp64[0] / (ms_fuse / 1000.0) / 1e12, 3), "peak_TFLOPs": FP64_PEAK_TFLOPS,
+                       "frac": round(fp64[0] / (ms_fuse / 1000.0) / 1e12 / FP64_PEAK_TFLOPS, 5)}
+    return out
+
+
+def conv_kernel_mfma(path=LEARNED_SUMMARY):
+    """MFMA rate of the network's convolution kernel from the committed
+    learned-path profile: SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512 flops per
+    dispatch over its average dispatch time, against the dense bf16 peak."""
+    kernel = "mvs::net::conv3x3_lds_kernel"
+    mops = profiled_counter(kernel, "SQ_INSTS_VALU_MFMA_MOPS_BF16", path)
+    ns = profiled_kernel_avg_ns(kernel, path)
+    if mops is None or not ns:
+        return None
+    flops = mops[0] * 512.0
+    t = flops / (ns * 1e-9) / 1e12
+    return {"kernel": kernel, "mfma_flops_per_dispatch": int(flops), "avg_ns": ns, "achieved_TFLOPs": round(t, 1),
+            "peak_TFLOPs": MFMA_BF16_PEAK_TFLOPS, "frac": round(t / MFMA_BF16_PEAK_TFLOPS, 4), "source": mops[1]}
+
+
 def valu_roofline(kernel, launch_ms, path=KERNEL_SUMMARY):
     """The bound that actually limits the stereo kernels: wave-level VALU
     instructions per launch (SQ_INSTS_VALU from the committed profile of the
@@ -95,6 +178,20 @@ def valu_roofline(kernel, launch_ms, path=KERNEL_SUMMARY):
     achieved = c[0] * 64.0 / (launch_ms / 1000.0) / 1e12
     return {"instructions_per_launch": c[0], "achieved": round(achieved, 2), "peak": round(VALU_PEAK_TOPS, 2),
             "unit": "T lane-ops/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "source": c[1]}
+
+
+STALLS_SUMMARY = os.path.join(ROOT, "profiles", "pass_kernel_room1080_stalls.txt")
+
+
+def lane_utilisation(kernel, path=STALLS_SUMMARY):
+    """Active lanes per issued VALU instruction / 64 (SQ_THREAD_CYCLES_VALU
+    over SQ_ACTIVE_INST_VALU x 64) from the committed profile; None when
+    absent."""
+    t = profiled_counter(kernel, "SQ_THREAD_CYCLES_VALU", path)
+    a = profiled_counter(kernel, "SQ_ACTIVE_INST_VALU", path)
+    if t is None or a is None or a[0] <= 0:
+        return None
+    return round(t[0] / (a[0] * 64.0), 4)
 
 
 def stereo_roofline(pass_kernel, avg_pass_ms, traffic, floor, alg_gbs, alg_bytes_per_launch, time_source):
@@ -112,6 +209,7 @@ def stereo_roofline(pass_kernel, avg_pass_ms, traffic, floor, alg_gbs, alg_bytes
             "avg_launch_ms": round(avg_pass_ms, 4),
             "valu_instructions_per_launch": None if valu is None else valu["instructions_per_launch"],
             "valu_source": None if valu is None else valu["source"],
+            "lane_utilisation": lane_utilisation(pass_kernel),
             "peak_basis": "256 CUs x 128 lanes/clock x 2.4 GHz: the published 157.3 TFLOP/s FP32 vector rate with "
                           "an FMA counted as two operations",
             "hbm": {"measured_bytes": None if t is None else int(t),
@@ -299,6 +397,8 @@ def main():
         ctx.estimate_depth_batch(jobs, opts, mode=(args.mode & 3) | 4, download=False)
         st = ctx.stats()
         log("profile run: passes %.1f ms, evals pass %d" % (st.ms_pass, st.n_eval_pass))
+        pts = ctx.fuse_resident([True] * len(views), select_sources(sc.views, len(views) - 1), args.cell, opts)
+        log("profile run: one fusion, %d points" % len(pts))
         from rustcolmap_amd import _lib as L
         L.check(L.load().mvs_debug_copy(ctx._h, 1 << 30), "mvs_debug_copy")   # mvs::copy_kernel, 1 GiB
         log("copy of 1073741824 bytes done")
@@ -410,6 +510,7 @@ def main():
         "views_per_s": round(views_per_s, 3),
         "fused_points_per_s": round(fused_pps, 1),
         "fused_points": int(len(pts)),
+        "fusion": fusion_fractions(sf.ms_fuse),
         "init": {"kernel": init_kernel, "ms": round(ms_init, 3),
                  "valu_frac": None if init_valu is None else init_valu["frac"],
                  "valu_achieved": None if init_valu is None else init_valu["achieved"],
@@ -507,7 +608,8 @@ def learned_block(ctx, sc, views, jobs, K, opts, args, ms_prepare, ms_plain, ms_
                        "mfma_frac": round(flops / (netm / 1000.0) / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4)
                        if netm > 0 else None,
                        "definition": "2 x 9 x cin x cout per output pixel of the 18 3x3 convolutions, over the "
-                                     "whole network stage time (convolutions, pooling, upsampling, head)"}}
+                                     "whole network stage time (convolutions, pooling, upsampling, head)",
+                       "conv_kernel": conv_kernel_mfma()}}
     log("learned: %s" % out)
     return out
 

@@ -680,10 +680,15 @@ __device__ __forceinline__ void init_tile(const JobDesc& J, const SrcDesc* __res
 // the tap's own pixel only.  Per hypothesis and source the block computes
 // the code of every pixel of its tile and apron once into LDS; each pixel
 // then adds its 25 terms in plane_cost's order (source, row, column) with
-// the same (strengthened) abort tests.  A source is costed only while some
-// pixel of the block still needs it (block vote).  `want` selects the pixels
-// that run the sweep (all of them, or those the learned initialisation did
-// not settle); every thread of the block must call this.
+// the same (strengthened) abort tests.  Every hypothesis needs source 0 at
+// every pixel, so the source-0 codes of INIT_NB hypotheses are computed
+// together (their census gathers in flight at once, one barrier for all);
+// a later source is costed one hypothesis at a time, while some pixel of
+// the block still needs it (block vote).  `want` selects the pixels that
+// run the sweep (all of them, or those the learned initialisation did not
+// settle); every thread of the block must call this.
+constexpr int INIT_NB = 2;
+
 template <bool PIN, bool FD>
 __device__ __forceinline__ unsigned fronto_code(u64 cref, float2 ray, float d, const SrcP& P) {
     int idx;
@@ -703,6 +708,39 @@ __device__ __forceinline__ void box_codes(const u64* s_cen, const float2* s_ray,
     }
 }
 
+// Source-0 codes of hypotheses d[0 .. nb) into s_code[b * NP ..]: every
+// projection first, then every gather, then the popcounts.
+template <int NP, int NT, bool PIN, bool FD>
+__device__ __forceinline__ void box_codes_batch(const u64* s_cen, const float2* s_ray, const float* d, int nb,
+                                                const SrcP& P, unsigned char* s_code) {
+    constexpr int K = (NP + NT - 1) / NT;
+    int idx[INIT_NB][K];
+    bool ok[INIT_NB][K];
+#pragma unroll
+    for (int b = 0; b < INIT_NB; ++b)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = (int)threadIdx.x + k * NT;
+            ok[b][k] = false;
+            idx[b][k] = 0;
+            if (b < nb && i < NP) ok[b][k] = tap_v4<true, PIN, FD>(s_ray[i], 0.0f, 0.0f, 1.0f, d[b], P, idx[b][k]);
+        }
+    u64 cs[INIT_NB][K];
+#pragma unroll
+    for (int b = 0; b < INIT_NB; ++b)
+#pragma unroll
+        for (int k = 0; k < K; ++k) cs[b][k] = ok[b][k] ? P.cen[idx[b][k]] : 0ull;
+#pragma unroll
+    for (int b = 0; b < INIT_NB; ++b)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = (int)threadIdx.x + k * NT;
+            if (b < nb && i < NP)
+                s_code[b * NP + i] = (unsigned char)(ok[b][k] ? (unsigned)__popcll(s_cen[i] ^ cs[b][k]) : 48u);
+        }
+}
+
+// s_code: room for (INIT_NB + 1) * (TW + 4) * (TH + 4) codes.
 template <int TW, int TH>
 __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* __restrict__ srcs, int x0, int y0,
                                               int nh, bool want, const u64* s_cen, const float2* s_ray,
@@ -711,41 +749,49 @@ __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* _
     const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
     const int x = x0 + lx, y = y0 + ly;
     const bool mine = want && x < J.w && y < J.h;
+    if (!__syncthreads_or(mine)) return;
     const int c = (ly + 2) * PW + (lx + 2);
     const SrcDesc* S = srcs + J.src_off;
     const int nsrc = J.nsrc;
     const bool fast = (J.pinhole_ok & 2) != 0, pin_ok = (J.pinhole_ok & 1) != 0;
     const float inv_min = J.inv_min, inv_max = J.inv_max;
+    const SrcP P0 = load_src(S[0]);
+    const bool pin0 = pin_ok && P0.k == 0.0f;
+    unsigned char* s_later = s_code + INIT_NB * NP;   // codes of sources >= 1
     float best_c = __builtin_inff(), best_d = 0.0f;
-    for (int hi = 0; hi < nh; ++hi) {
-        const float inv = inv_min + (inv_max - inv_min) * (float)hi / (float)(nh - 1);
-        const float d = 1.0f / inv;
-        const float thr = (best_c + 0.02f) + 0.12f;
-        const AbortThr A = make_abort_thr(thr);
-        float sum = 0.0f;
-        bool live = mine, aborted = false;
-#pragma nounroll
-        for (int si = 0; si < nsrc; ++si) {
-            // the vote is also the barrier after the previous source's sums
-            if (!__syncthreads_or(live)) break;
-            const SrcP P = load_src(S[si]);
-            const bool pin = pin_ok && P.k == 0.0f;
-            if (fast && pin)
-                box_codes<NP, NT, true, true>(s_cen, s_ray, d, P, s_code);
-            else if (fast)
-                box_codes<NP, NT, false, true>(s_cen, s_ray, d, P, s_code);
-            else if (pin)
-                box_codes<NP, NT, true, false>(s_cen, s_ray, d, P, s_code);
-            else
-                box_codes<NP, NT, false, false>(s_cen, s_ray, d, P, s_code);
-            __syncthreads();
-            if (live) {
+    for (int h0 = 0; h0 < nh; h0 += INIT_NB) {
+        const int nb = min(INIT_NB, nh - h0);
+        float d[INIT_NB];
+#pragma unroll
+        for (int b = 0; b < INIT_NB; ++b) {
+            const int hi = min(h0 + b, nh - 1);
+            const float inv = inv_min + (inv_max - inv_min) * (float)hi / (float)(nh - 1);
+            d[b] = 1.0f / inv;
+        }
+        __syncthreads();   // the previous batch's sums are done with s_code
+        if (fast && pin0)
+            box_codes_batch<NP, NT, true, true>(s_cen, s_ray, d, nb, P0, s_code);
+        else if (fast)
+            box_codes_batch<NP, NT, false, true>(s_cen, s_ray, d, nb, P0, s_code);
+        else if (pin0)
+            box_codes_batch<NP, NT, true, false>(s_cen, s_ray, d, nb, P0, s_code);
+        else
+            box_codes_batch<NP, NT, false, false>(s_cen, s_ray, d, nb, P0, s_code);
+        __syncthreads();
+        for (int b = 0; b < nb; ++b) {
+            const float dd = d[b];
+            const float thr = (best_c + 0.02f) + 0.12f;
+            const AbortThr A = make_abort_thr(thr);
+            float sum = 0.0f;
+            bool live = mine, aborted = false;
+            // the 25 terms of source si from `codes`, row tests as in eval_v4_body
+            auto add_source = [&](const unsigned char* codes, int si) {
                 const double fs = (double)(si + 1) * 5.0;
                 const double cap = si + 1 < nsrc ? (double)(si + 2) * 5.0 : 1e30;
 #pragma unroll
                 for (int r = 0; r < 5; ++r) {
 #pragma unroll
-                    for (int q = 0; q < 5; ++q) sum += s_tab[s_code[c + (r - 2) * PW + (q - 2)]];
+                    for (int q = 0; q < 5; ++q) sum += s_tab[codes[c + (r - 2) * PW + (q - 2)]];
                     if (abort_test(sum, fmin((double)(r + 1) * fs, cap), A)) {
                         aborted = true;
                         live = false;
@@ -753,14 +799,32 @@ __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* _
                     }
                 }
                 if (si + 1 == nsrc) live = false;
+            };
+            if (live) add_source(s_code + b * NP, 0);
+#pragma nounroll
+            for (int si = 1; si < nsrc; ++si) {
+                // the vote is also the barrier after the previous source's sums
+                if (!__syncthreads_or(live)) break;
+                const SrcP P = load_src(S[si]);
+                const bool pin = pin_ok && P.k == 0.0f;
+                if (fast && pin)
+                    box_codes<NP, NT, true, true>(s_cen, s_ray, dd, P, s_later);
+                else if (fast)
+                    box_codes<NP, NT, false, true>(s_cen, s_ray, dd, P, s_later);
+                else if (pin)
+                    box_codes<NP, NT, true, false>(s_cen, s_ray, dd, P, s_later);
+                else
+                    box_codes<NP, NT, false, false>(s_cen, s_ray, dd, P, s_later);
+                __syncthreads();
+                if (live) add_source(s_later, si);
             }
-        }
-        if (mine) {
-            ++ne;
-            const float cst = aborted ? 1e9f : sum / (25.0f * (float)nsrc);
-            if (cst < best_c) {
-                best_c = cst;
-                best_d = d;
+            if (mine) {
+                ++ne;
+                const float cst = aborted ? 1e9f : sum / (25.0f * (float)nsrc);
+                if (cst < best_c) {
+                    best_c = cst;
+                    best_d = dd;
+                }
             }
         }
     }
@@ -923,7 +987,7 @@ __global__ __launch_bounds__(TW* TH) void init_kernel(const JobDesc* __restrict_
                                                       u64* __restrict__ counter) {
     __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
-    __shared__ unsigned char s_code[EV >= 3 ? (TW + 4) * (TH + 4) : 1];
+    __shared__ unsigned char s_code[EV >= 3 ? (INIT_NB + 1) * (TW + 4) * (TH + 4) : 1];
     __shared__ unsigned s_cnt;
     __shared__ float s_tab[64];
     const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
@@ -1097,7 +1161,7 @@ __global__ __launch_bounds__(64) void init_sub_kernel(const JobDesc* __restrict_
     constexpr int NSX = TW / 8, NSUB = NSX * (TH / 8);
     __shared__ u64 s_cen[12 * 12];
     __shared__ float2 s_ray[12 * 12];
-    __shared__ unsigned char s_code[12 * 12];
+    __shared__ unsigned char s_code[(INIT_NB + 1) * 12 * 12];
     __shared__ unsigned s_cnt;
     __shared__ float s_tab[64];
     const int b = xcd_remap(blockIdx.x, ntiles * NSUB);
@@ -1278,7 +1342,7 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
     constexpr int PW = TW + 4;
     __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
-    __shared__ unsigned char s_code[EV >= 3 ? (TW + 4) * (TH + 4) : 1];
+    __shared__ unsigned char s_code[EV >= 3 ? (INIT_NB + 1) * (TW + 4) * (TH + 4) : 1];
     __shared__ unsigned s_cnt, s_acc;
     __shared__ u64 s_base;
     __shared__ float s_tab[64];

@@ -12,7 +12,7 @@ sys.path.insert(0, ROOT)
 
 import bench  # noqa: E402
 
-PASS = "mvs::pass_persistent_kernel<8, 8, 3>"
+PASS = "mvs::pass_sorted_kernel<24, 8, 3>"
 TILE = "mt::filt_tile_kernel<1>"
 
 
@@ -82,8 +82,9 @@ def test_roofline_object_is_a_bound():
     assert 0.0 < r["hbm"]["frac"] <= 1.0
     assert r["hbm"]["traffic_over_floor"] >= 1.0
     assert r["algorithmic"]["GBs"] == 49917.9
+    assert 0.5 < r["lane_utilisation"] <= 1.0
     # the init kernel on the same basis
-    init = "mvs::init_kernel<8, 8, 3>"
+    init = "mvs::init_sub_kernel<24, 8>"
     ns_i = _avg_ns(os.path.join(ROOT, "profiles", "stereo_room1080_kernel_trace.txt"), init)
     v = bench.valu_roofline(init, ns_i / 1e6)
     assert v is not None and 0.0 < v["frac"] <= 1.0
@@ -98,3 +99,29 @@ def test_network_flops_from_the_layer_table():
     lvl0 = sum(2 * 9 * ci * co for ci, co, lev in bench.NET_LAYERS if lev == 0)
     assert bench.net_conv_flops(tiny) == lvl0
     assert bench.net_conv_flops([SimpleNamespace(w=1920, h=1080)]) > 1e11
+
+
+def test_fusion_fractions_parse():
+    """Fusion's HBM and FP64 fractions: rocprof totals over every fusion
+    kernel of one run (the stereo fill kernel, mvs::fill_u64_kernel, is not
+    one of them), over a stage time; both are fractions of their peaks."""
+    fe = bench.profiled_stage_total(bench.FUSION_KERNELS, "FETCH_SIZE", bench.FUSION_SUMMARY)
+    wr = bench.profiled_stage_total(bench.FUSION_KERNELS, "WRITE_SIZE", bench.FUSION_SUMMARY)
+    assert fe is not None and wr is not None
+    # more than the candidate kernel's own reads, less than a pass launch
+    assert 7e8 < fe[0] + wr[0] < 5e9
+    f = bench.fusion_fractions(2.8)
+    assert f["hbm"]["bytes_per_run"] == int(fe[0] + wr[0])
+    assert 0.0 < f["hbm"]["frac"] < 1.0 and 0.0 < f["fp64"]["frac"] < 1.0
+    assert f["hbm"]["GBs"] == pytest.approx((fe[0] + wr[0]) / 2.8e-3 / 1e9, rel=1e-3)
+
+
+def test_conv_kernel_mfma_from_counters():
+    """The network's convolution kernel: MFMA flops per dispatch from
+    SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512, which equals its MFMA instruction
+    count x 16384 (v_mfma_f32_16x16x32_bf16) in the same profile."""
+    c = bench.conv_kernel_mfma()
+    assert c is not None and 0.0 < c["frac"] < 1.0
+    n = bench.profiled_counter("mvs::net::conv3x3_lds_kernel", "SQ_INSTS_VALU_MFMA_BF16", bench.LEARNED_SUMMARY)
+    assert n is not None
+    assert c["mfma_flops_per_dispatch"] == pytest.approx(n[0] * 16384, rel=2e-3)
