@@ -680,7 +680,10 @@ __device__ __forceinline__ void init_tile(const JobDesc& J, const SrcDesc* __res
 // the tap's own pixel only.  Per hypothesis and source the block computes
 // the code of every pixel of its tile and apron once into LDS; each pixel
 // then adds its 25 terms in plane_cost's order (source, row, column) with
-// the same (strengthened) abort tests.  Every hypothesis needs source 0 at
+// the same (strengthened) abort tests; LDS holds each tap's term (the /48
+// table entry of its popcount, 1 when invalid), looked up once when the
+// code is made instead of at each of the up to 25 windows that read it.
+// Every hypothesis needs source 0 at
 // every pixel, so the source-0 codes of INIT_NB hypotheses are computed
 // together (their census gathers in flight at once, one barrier for all);
 // a later source is costed one hypothesis at a time, while some pixel of
@@ -696,23 +699,23 @@ __device__ __forceinline__ unsigned fronto_code(u64 cref, float2 ray, float d, c
     return ok ? (unsigned)__popcll(cref ^ P.cen[idx]) : 48u;
 }
 
-// One source's codes of the tile + apron, this thread's pixels unrolled so
+// One source's terms of the tile + apron, this thread's pixels unrolled so
 // that their census gathers are in flight together.
 template <int NP, int NT, bool PIN, bool FD>
 __device__ __forceinline__ void box_codes(const u64* s_cen, const float2* s_ray, float d, const SrcP& P,
-                                          unsigned char* s_code) {
+                                          const float* tab, float* s_term) {
 #pragma unroll
     for (int k = 0; k < (NP + NT - 1) / NT; ++k) {
         const int i = (int)threadIdx.x + k * NT;
-        if (i < NP) s_code[i] = (unsigned char)fronto_code<PIN, FD>(s_cen[i], s_ray[i], d, P);
+        if (i < NP) s_term[i] = tab[fronto_code<PIN, FD>(s_cen[i], s_ray[i], d, P)];
     }
 }
 
-// Source-0 codes of hypotheses d[0 .. nb) into s_code[b * NP ..]: every
+// Source-0 terms of hypotheses d[0 .. nb) into s_term[b * NP ..]: every
 // projection first, then every gather, then the popcounts.
 template <int NP, int NT, bool PIN, bool FD>
 __device__ __forceinline__ void box_codes_batch(const u64* s_cen, const float2* s_ray, const float* d, int nb,
-                                                const SrcP& P, unsigned char* s_code) {
+                                                const SrcP& P, const float* tab, float* s_term) {
     constexpr int K = (NP + NT - 1) / NT;
     int idx[INIT_NB][K];
     bool ok[INIT_NB][K];
@@ -736,15 +739,15 @@ __device__ __forceinline__ void box_codes_batch(const u64* s_cen, const float2* 
         for (int k = 0; k < K; ++k) {
             const int i = (int)threadIdx.x + k * NT;
             if (b < nb && i < NP)
-                s_code[b * NP + i] = (unsigned char)(ok[b][k] ? (unsigned)__popcll(s_cen[i] ^ cs[b][k]) : 48u);
+                s_term[b * NP + i] = tab[ok[b][k] ? (unsigned)__popcll(s_cen[i] ^ cs[b][k]) : 48u];
         }
 }
 
-// s_code: room for (INIT_NB + 1) * (TW + 4) * (TH + 4) codes.
+// s_term: room for (INIT_NB + 1) * (TW + 4) * (TH + 4) terms.
 template <int TW, int TH>
 __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* __restrict__ srcs, int x0, int y0,
                                               int nh, bool want, const u64* s_cen, const float2* s_ray,
-                                              unsigned char* s_code, unsigned& ne, const float* s_tab) {
+                                              float* s_term, unsigned& ne, const float* s_tab) {
     constexpr int PW = TW + 4, NP = (TW + 4) * (TH + 4), NT = TW * TH;
     const int lx = threadIdx.x % TW, ly = threadIdx.x / TW;
     const int x = x0 + lx, y = y0 + ly;
@@ -757,7 +760,7 @@ __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* _
     const float inv_min = J.inv_min, inv_max = J.inv_max;
     const SrcP P0 = load_src(S[0]);
     const bool pin0 = pin_ok && P0.k == 0.0f;
-    unsigned char* s_later = s_code + INIT_NB * NP;   // codes of sources >= 1
+    float* s_later = s_term + INIT_NB * NP;   // terms of sources >= 1
     float best_c = __builtin_inff(), best_d = 0.0f;
     for (int h0 = 0; h0 < nh; h0 += INIT_NB) {
         const int nb = min(INIT_NB, nh - h0);
@@ -768,15 +771,15 @@ __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* _
             const float inv = inv_min + (inv_max - inv_min) * (float)hi / (float)(nh - 1);
             d[b] = 1.0f / inv;
         }
-        __syncthreads();   // the previous batch's sums are done with s_code
+        __syncthreads();   // the previous batch's sums are done with s_term
         if (fast && pin0)
-            box_codes_batch<NP, NT, true, true>(s_cen, s_ray, d, nb, P0, s_code);
+            box_codes_batch<NP, NT, true, true>(s_cen, s_ray, d, nb, P0, s_tab, s_term);
         else if (fast)
-            box_codes_batch<NP, NT, false, true>(s_cen, s_ray, d, nb, P0, s_code);
+            box_codes_batch<NP, NT, false, true>(s_cen, s_ray, d, nb, P0, s_tab, s_term);
         else if (pin0)
-            box_codes_batch<NP, NT, true, false>(s_cen, s_ray, d, nb, P0, s_code);
+            box_codes_batch<NP, NT, true, false>(s_cen, s_ray, d, nb, P0, s_tab, s_term);
         else
-            box_codes_batch<NP, NT, false, false>(s_cen, s_ray, d, nb, P0, s_code);
+            box_codes_batch<NP, NT, false, false>(s_cen, s_ray, d, nb, P0, s_tab, s_term);
         __syncthreads();
         for (int b = 0; b < nb; ++b) {
             const float dd = d[b];
@@ -785,13 +788,13 @@ __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* _
             float sum = 0.0f;
             bool live = mine, aborted = false;
             // the 25 terms of source si from `codes`, row tests as in eval_v4_body
-            auto add_source = [&](const unsigned char* codes, int si) {
+            auto add_source = [&](const float* terms, int si) {
                 const double fs = (double)(si + 1) * 5.0;
                 const double cap = si + 1 < nsrc ? (double)(si + 2) * 5.0 : 1e30;
 #pragma unroll
                 for (int r = 0; r < 5; ++r) {
 #pragma unroll
-                    for (int q = 0; q < 5; ++q) sum += s_tab[codes[c + (r - 2) * PW + (q - 2)]];
+                    for (int q = 0; q < 5; ++q) sum += terms[c + (r - 2) * PW + (q - 2)];
                     if (abort_test(sum, fmin((double)(r + 1) * fs, cap), A)) {
                         aborted = true;
                         live = false;
@@ -800,7 +803,7 @@ __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* _
                 }
                 if (si + 1 == nsrc) live = false;
             };
-            if (live) add_source(s_code + b * NP, 0);
+            if (live) add_source(s_term + b * NP, 0);
 #pragma nounroll
             for (int si = 1; si < nsrc; ++si) {
                 // the vote is also the barrier after the previous source's sums
@@ -808,13 +811,13 @@ __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* _
                 const SrcP P = load_src(S[si]);
                 const bool pin = pin_ok && P.k == 0.0f;
                 if (fast && pin)
-                    box_codes<NP, NT, true, true>(s_cen, s_ray, dd, P, s_later);
+                    box_codes<NP, NT, true, true>(s_cen, s_ray, dd, P, s_tab, s_later);
                 else if (fast)
-                    box_codes<NP, NT, false, true>(s_cen, s_ray, dd, P, s_later);
+                    box_codes<NP, NT, false, true>(s_cen, s_ray, dd, P, s_tab, s_later);
                 else if (pin)
-                    box_codes<NP, NT, true, false>(s_cen, s_ray, dd, P, s_later);
+                    box_codes<NP, NT, true, false>(s_cen, s_ray, dd, P, s_tab, s_later);
                 else
-                    box_codes<NP, NT, false, false>(s_cen, s_ray, dd, P, s_later);
+                    box_codes<NP, NT, false, false>(s_cen, s_ray, dd, P, s_tab, s_later);
                 __syncthreads();
                 if (live) add_source(s_later, si);
             }
@@ -987,7 +990,7 @@ __global__ __launch_bounds__(TW* TH) void init_kernel(const JobDesc* __restrict_
                                                       u64* __restrict__ counter) {
     __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
-    __shared__ unsigned char s_code[EV >= 3 ? (INIT_NB + 1) * (TW + 4) * (TH + 4) : 1];
+    __shared__ float s_term[EV >= 3 ? (INIT_NB + 1) * (TW + 4) * (TH + 4) : 1];
     __shared__ unsigned s_cnt;
     __shared__ float s_tab[64];
     const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
@@ -998,7 +1001,7 @@ __global__ __launch_bounds__(TW* TH) void init_kernel(const JobDesc* __restrict_
     __syncthreads();
     unsigned ne = 0;
     if (EV >= 3)
-        init_box_tile<TW, TH>(J, srcs, tl.y, tl.z, nh, true, s_cen, s_ray, s_code, ne, s_tab);
+        init_box_tile<TW, TH>(J, srcs, tl.y, tl.z, nh, true, s_cen, s_ray, s_term, ne, s_tab);
     else
         init_tile<TW, TH, EV>(J, srcs, tl.y, tl.z, nh, s_cen, s_ray, ne, s_tab);
     block_count<TW, TH>(ne, &s_cnt, counter);
@@ -1161,7 +1164,7 @@ __global__ __launch_bounds__(64) void init_sub_kernel(const JobDesc* __restrict_
     constexpr int NSX = TW / 8, NSUB = NSX * (TH / 8);
     __shared__ u64 s_cen[12 * 12];
     __shared__ float2 s_ray[12 * 12];
-    __shared__ unsigned char s_code[(INIT_NB + 1) * 12 * 12];
+    __shared__ float s_term[(INIT_NB + 1) * 12 * 12];
     __shared__ unsigned s_cnt;
     __shared__ float s_tab[64];
     const int b = xcd_remap(blockIdx.x, ntiles * NSUB);
@@ -1175,7 +1178,7 @@ __global__ __launch_bounds__(64) void init_sub_kernel(const JobDesc* __restrict_
     stage_tile<8, 8>(J, x0, y0, s_cen, s_ray);
     __syncthreads();
     unsigned ne = 0;
-    init_box_tile<8, 8>(J, srcs, x0, y0, nh, true, s_cen, s_ray, s_code, ne, s_tab);
+    init_box_tile<8, 8>(J, srcs, x0, y0, nh, true, s_cen, s_ray, s_term, ne, s_tab);
     block_count<8, 8>(ne, &s_cnt, counter);
 }
 
@@ -1342,7 +1345,7 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
     constexpr int PW = TW + 4;
     __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
-    __shared__ unsigned char s_code[EV >= 3 ? (INIT_NB + 1) * (TW + 4) * (TH + 4) : 1];
+    __shared__ float s_term[EV >= 3 ? (INIT_NB + 1) * (TW + 4) * (TH + 4) : 1];
     __shared__ unsigned s_cnt, s_acc;
     __shared__ u64 s_base;
     __shared__ float s_tab[64];
@@ -1407,7 +1410,7 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
     if (EV >= 3) {
         // the pixels without an accepted proposal (all, with `prop`) sweep together
         unsigned ne2 = 0;
-        init_box_tile<TW, TH>(J, srcs, tl.y, tl.z, nh, !accept || prop != nullptr, s_cen, s_ray, s_code, ne2, s_tab);
+        init_box_tile<TW, TH>(J, srcs, tl.y, tl.z, nh, !accept || prop != nullptr, s_cen, s_ray, s_term, ne2, s_tab);
         ne += ne2;
     }
     (void)na;
