@@ -143,7 +143,9 @@ int mvs_census_noalloc(mvs_ctx *ctx, const uint8_t *gray, int32_t w, int32_t h, 
 
 /* load_views (mvs/src/lib.rs:144-218) after decode/resize: uploads n views,
  * replacing any previous set, and computes each view's census and rays on
- * the device (one launch for all views). */
+ * the device (one launch for all views).  A view holds at most 2^29 pixels
+ * and 2^24 - 1 on a side (MVS_ERR_LIMIT beyond: census offsets are 32-bit
+ * byte offsets). */
 int mvs_upload_views(mvs_ctx *ctx, const mvs_view_t *views, int32_t n);
 
 /* ViewData.census of an uploaded view (w*h u64). */

@@ -545,8 +545,10 @@ int mvs_upload_views(mvs_ctx* c, const mvs_view_t* views, int32_t n) {
     for (int i = 0; i < n; i++) {
         if (views[i].w <= 0 || views[i].h <= 0 || !views[i].gray)
             return fail(MVS_ERR_ARG, "mvs_upload_views: view " + std::to_string(i) + " has no image");
-        if ((int64_t)views[i].w * views[i].h > (int64_t)1 << 30)
-            return fail(MVS_ERR_LIMIT, "mvs_upload_views: view larger than 2^30 pixels");
+        // the stereo kernels address a census by 32-bit byte offsets (8 B per
+        // pixel) computed with 24-bit multiplies
+        if ((int64_t)views[i].w * views[i].h > (int64_t)1 << 29 || views[i].w >= (1 << 24) || views[i].h >= (1 << 24))
+            return fail(MVS_ERR_LIMIT, "mvs_upload_views: view larger than 2^29 pixels or 2^24 - 1 on a side");
         if (!(views[i].f != 0.0)) return fail(MVS_ERR_ARG, "mvs_upload_views: focal length is 0");
     }
     std::lock_guard<std::mutex> lk(c->mu);

@@ -176,13 +176,21 @@ __device__ __forceinline__ float4 as_float4(f32x4v v) { return make_float4(v.x, 
 
 struct SrcP {
     gu64* cen;
+    __amdgpu_buffer_rsrc_t rcen;   // the same census as a raw buffer (scalar registers)
     int sw, sh;
     float r0, r1, r2, r3, r4, r5, r6, r7, r8, t0, t1, t2, f, cx, cy, k;
 };
 
+// gfx9 raw-buffer descriptor word 3 (32-bit data, no swizzle)
+constexpr int RAW_BUFFER_FLAGS = 0x00020000;
+
 __device__ __forceinline__ SrcP load_src(const SrcDesc& s) {
     SrcP p;
     p.cen = (gu64*)s.census;
+    // record count in bytes: a view holds at most 2^29 pixels (host check)
+    const unsigned long long bytes = (unsigned long long)s.w * (unsigned long long)s.h * 8ull;
+    p.rcen = __builtin_amdgcn_make_buffer_rsrc((void*)s.census, (short)0,
+                                               (int)(bytes > 0xffffffffull ? 0xffffffffull : bytes), RAW_BUFFER_FLAGS);
     p.sw = s.w;
     p.sh = s.h;
     p.r0 = s.rs[0]; p.r1 = s.rs[1]; p.r2 = s.rs[2]; p.r3 = s.rs[3]; p.r4 = s.rs[4];
@@ -193,6 +201,14 @@ __device__ __forceinline__ SrcP load_src(const SrcDesc& s) {
 }
 
 // `(x) as i32` lands in [0, n): NaN -> 0 (valid for n > 0), else -1 < x < n.
+// The census word at byte offset `off` (< the census size: callers pass 0
+// for an invalid tap) through a raw-buffer load: the view's base stays in
+// scalar registers and the offset is a 32-bit vector register, so a tap has
+// no 64-bit address arithmetic.
+__device__ __forceinline__ u64 cen_at(const SrcP& P, unsigned off) {
+    return __builtin_bit_cast(u64, __builtin_amdgcn_raw_buffer_load_b64(P.rcen, (int)off, 0, 0));
+}
+
 __device__ __forceinline__ bool cast_ok(float x, int n, int& out) {
     const bool in = (x > -1.0f) && (x < (float)n);
     out = (int)(in ? x : 0.0f);
@@ -435,7 +451,7 @@ __device__ __forceinline__ float div_rn_fast(float a, float b) {
 // smallest divisor of any test still ahead.  With PAIR, rows 1 and 2 of
 // source 0 issue their ten gathers together.
 template <bool FRONTO, bool PIN, bool FD = true>
-__device__ __forceinline__ bool tap_v4(float2 ray, float nx, float ny, float nz, float d, const SrcP& P, int& idx) {
+__device__ __forceinline__ bool tap_v4(float2 ray, float nx, float ny, float nz, float d, const SrcP& P, unsigned& off) {
     float z, xr, yr;
     bool vd;
     if (FRONTO) {
@@ -463,7 +479,8 @@ __device__ __forceinline__ bool tap_v4(float2 ray, float nx, float ny, float nz,
     }
     const int u = cvt_i32_rust(fu), v = cvt_i32_rust(fv);
     const bool ok = vd && !(zs < 1e-3f) && ((unsigned)u < (unsigned)P.sw) && ((unsigned)v < (unsigned)P.sh);
-    idx = ok ? v * P.sw + u : 0;
+    // u < sw < 2^24 and v < sh < 2^24 (host check), so the 24-bit multiply is exact
+    off = ok ? (__umul24((unsigned)v, (unsigned)P.sw) + (unsigned)u) * 8u : 0u;   // byte offset of the census word
     return ok;
 }
 
@@ -471,14 +488,14 @@ template <int PW, bool FRONTO, bool PIN>
 __device__ __forceinline__ void row_v4(const u64* __restrict__ s_cen, const float2* __restrict__ s_ray, int c, int r,
                                        float nx, float ny, float nz, float d, const SrcP& P,
                                        const float* __restrict__ tab, float& sum) {
-    int idx[5];
+    unsigned off[5];
     bool ok[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q)
-        ok[q] = tap_v4<FRONTO, PIN>(s_ray[c + (r - 2) * PW + (q - 2)], nx, ny, nz, d, P, idx[q]);
+        ok[q] = tap_v4<FRONTO, PIN>(s_ray[c + (r - 2) * PW + (q - 2)], nx, ny, nz, d, P, off[q]);
     u64 cs[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) cs[q] = P.cen[idx[q]];
+    for (int q = 0; q < 5; ++q) cs[q] = cen_at(P, off[q]);
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
         const u64 rc = s_cen[c + (r - 2) * PW + (q - 2)];
@@ -493,17 +510,17 @@ __device__ __forceinline__ bool rows12_v4(const u64* __restrict__ s_cen, const f
                                           float nx, float ny, float nz, float d, const SrcP& P,
                                           const float* __restrict__ tab, const AbortThr& A, double D1, double D2,
                                           float& sum) {
-    int idx[10];
+    unsigned off[10];
     unsigned okm = 0;
 #pragma unroll
     for (int t = 0; t < 10; ++t) {
         const int r = 1 + t / 5, q = t % 5;
-        const bool ok = tap_v4<FRONTO, PIN>(s_ray[c + (r - 2) * PW + (q - 2)], nx, ny, nz, d, P, idx[t]);
+        const bool ok = tap_v4<FRONTO, PIN>(s_ray[c + (r - 2) * PW + (q - 2)], nx, ny, nz, d, P, off[t]);
         okm |= (ok ? 1u : 0u) << t;
     }
     u64 cs[10];
 #pragma unroll
-    for (int t = 0; t < 10; ++t) cs[t] = P.cen[idx[t]];
+    for (int t = 0; t < 10; ++t) cs[t] = cen_at(P, off[t]);
     bool ab = false;
 #pragma unroll
     for (int t = 0; t < 10; ++t) {
@@ -694,9 +711,9 @@ constexpr int INIT_NB = 2;
 
 template <bool PIN, bool FD>
 __device__ __forceinline__ unsigned fronto_code(u64 cref, float2 ray, float d, const SrcP& P) {
-    int idx;
-    const bool ok = tap_v4<true, PIN, FD>(ray, 0.0f, 0.0f, 1.0f, d, P, idx);
-    return ok ? (unsigned)__popcll(cref ^ P.cen[idx]) : 48u;
+    unsigned off;
+    const bool ok = tap_v4<true, PIN, FD>(ray, 0.0f, 0.0f, 1.0f, d, P, off);
+    return ok ? (unsigned)__popcll(cref ^ cen_at(P, off)) : 48u;
 }
 
 // One source's terms of the tile + apron, this thread's pixels unrolled so
@@ -717,7 +734,7 @@ template <int NP, int NT, bool PIN, bool FD>
 __device__ __forceinline__ void box_codes_batch(const u64* s_cen, const float2* s_ray, const float* d, int nb,
                                                 const SrcP& P, const float* tab, float* s_term) {
     constexpr int K = (NP + NT - 1) / NT;
-    int idx[INIT_NB][K];
+    unsigned off[INIT_NB][K];
     bool ok[INIT_NB][K];
 #pragma unroll
     for (int b = 0; b < INIT_NB; ++b)
@@ -725,14 +742,14 @@ __device__ __forceinline__ void box_codes_batch(const u64* s_cen, const float2* 
         for (int k = 0; k < K; ++k) {
             const int i = (int)threadIdx.x + k * NT;
             ok[b][k] = false;
-            idx[b][k] = 0;
-            if (b < nb && i < NP) ok[b][k] = tap_v4<true, PIN, FD>(s_ray[i], 0.0f, 0.0f, 1.0f, d[b], P, idx[b][k]);
+            off[b][k] = 0;
+            if (b < nb && i < NP) ok[b][k] = tap_v4<true, PIN, FD>(s_ray[i], 0.0f, 0.0f, 1.0f, d[b], P, off[b][k]);
         }
     u64 cs[INIT_NB][K];
 #pragma unroll
     for (int b = 0; b < INIT_NB; ++b)
 #pragma unroll
-        for (int k = 0; k < K; ++k) cs[b][k] = ok[b][k] ? P.cen[idx[b][k]] : 0ull;
+        for (int k = 0; k < K; ++k) cs[b][k] = ok[b][k] ? cen_at(P, off[b][k]) : 0ull;
 #pragma unroll
     for (int b = 0; b < INIT_NB; ++b)
 #pragma unroll
@@ -808,7 +825,9 @@ __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* _
             for (int si = 1; si < nsrc; ++si) {
                 // the vote is also the barrier after the previous source's sums
                 if (!__syncthreads_or(live)) break;
-                const SrcP P = load_src(S[si]);
+                // si is the same in every lane (the vote is block-wide); saying
+                // so keeps the source parameters and census descriptor scalar
+                const SrcP P = load_src(S[__builtin_amdgcn_readfirstlane(si)]);
                 const bool pin = pin_ok && P.k == 0.0f;
                 if (fast && pin)
                     box_codes<NP, NT, true, true>(s_cen, s_ray, dd, P, s_tab, s_later);
@@ -1054,7 +1073,9 @@ __global__ __launch_bounds__(TW* TH) PASS_OCCUPANCY void pass_persistent_kernel(
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(queue, 1);
         __syncthreads();
-        const int t = s_tile;
+        // the same in every lane: saying so keeps the tile, job and source
+        // descriptors in scalar registers
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);
         if (t >= ntiles) break;
         const int4 tl = tiles[t];
         const JobDesc& J = jobs[tl.x];
@@ -1100,7 +1121,9 @@ __global__ __launch_bounds__(64) PASS_OCCUPANCY void pass_sorted_kernel(const Jo
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(queue, 1);
         __syncthreads();
-        const int t = s_tile;
+        // the same in every lane: saying so keeps the tile, job and source
+        // descriptors in scalar registers
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);
         if (t >= ntiles) break;
         const int4 tl = tiles[t];
         const JobDesc& J = jobs[tl.x];
@@ -1242,9 +1265,9 @@ __device__ __forceinline__ uint16_t f2bf16(float f) {
 // bf16 network features (value - 0.5) * 2 are written per coarse pixel.
 template <bool PIN, bool FD>
 __device__ __forceinline__ unsigned fronto_tap_cost(u64 cref, float2 ray, float d, const SrcP& P) {
-    int idx;
-    const bool ok = tap_v4<true, PIN, FD>(ray, 0.0f, 0.0f, 1.0f, d, P, idx);
-    return ok ? (unsigned)__popcll(cref ^ P.cen[idx]) : 48u;
+    unsigned off;
+    const bool ok = tap_v4<true, PIN, FD>(ray, 0.0f, 0.0f, 1.0f, d, P, off);
+    return ok ? (unsigned)__popcll(cref ^ cen_at(P, off)) : 48u;
 }
 
 // One source's tap costs of this thread's pixels of the tile + apron.
