@@ -356,6 +356,8 @@ def main():
                     help="default run: skip the descriptor-matching block (sfm_matching)")
     ap.add_argument("--no-learned", action="store_true",
                     help="default run: skip the learned-initialisation block (learned)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="default run: skip the uneven-batch block (uneven_batch)")
     ap.add_argument("--profile", action="store_true",
                     help="one directly launched batch plus a 1 GiB device copy (for counter calibration), no JSON")
     args = ap.parse_args()
@@ -518,6 +520,14 @@ def main():
         "learned": learned,
         "cpu_baseline": cpu,
     }
+    if not args.no_batched:
+        # the uneven 64-view batch (mixed sizes up to 2560x1440, K in [4, 12]), batched graph schedule against
+        # per-view launches, in the same run
+        ctx.close()
+        bargs = argparse.Namespace(**{**vars(args), "steps": 2, "warmup": 1, "views": 0})
+        b = batched_bench(bargs, emit=False)
+        out["uneven_batch"] = {k: b[k] for k in ("metric", "value", "unit", "ms_per_step", "config", "schedules",
+                                                 "speedup_batched_vs_per_view")}
     if not args.no_match:
         # the second hot path, SfM descriptor matching, measured in the same run
         margs = argparse.Namespace(**{**vars(args), "steps": 30, "warmup": 3, "match_exact": False,
@@ -747,11 +757,11 @@ def learned_bench(args):
     print(json.dumps(out), flush=True)
 
 
-def batched_bench(args):
+def batched_bench(args, emit=True):
     """64 views of uneven size (seeded draw over 640x480 .. 2560x1440,
     f = 0.86 W), K uniform in [4, 12] per view, 8 sweeps: one batched
     schedule (graph replay of one launch per stage) against one launch per
-    view and stage."""
+    view and stage.  With emit=False the result is returned, not printed."""
     import torch
     from rustcolmap_amd.mvs import MvsContext, MvsOptions, ViewData
     from rustcolmap_amd.scene import config_spec, render_scene, uneven_jobs
@@ -792,7 +802,10 @@ def batched_bench(args):
            "schedules": res,
            "speedup_batched_vs_per_view": round(res["per_view_launches"]["ms_per_step"]
                                                 / res["batched_graph"]["ms_per_step"], 3)}
-    print(json.dumps(out), flush=True)
+    ctx.close()
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 # f32 VALU operations/s without packed math: 256 CUs x 128 lanes/clock x 2.4 GHz (the published 157.3 TFLOP/s

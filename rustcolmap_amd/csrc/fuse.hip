@@ -3,11 +3,14 @@
 // Reference: mvs/src/fuse.rs:32-215 (sample_depth, fuse_depth_maps).
 //
 // Stages (all on the context's stream):
-//   1. candidates: one thread per pixel of every view that has a depth map
-//      (global index = view-major raster order, the reference's merge
-//      order); f64 back-projection, projection into every view of its
-//      fusion order, f32 bilinear depth sample, relative agreement test,
-//      average.  Writes a flag, the f64 point and the colour per pixel.
+//   1. candidates: the pixels with a depth of every view that has a depth
+//      map (global index = view-major raster order, the reference's merge
+//      order) are listed in order by a flag pass and a compaction; one
+//      thread per listed pixel: f64 back-projection, projection into every
+//      view of its fusion order, f32 bilinear depth sample, relative
+//      agreement test, average.  Writes a flag, the f64 point and the
+//      colour per pixel.  (Only a few percent of pixels carry a depth: a
+//      thread per pixel left most lanes of a wave idle in the per-view loop.)
 //   2. compaction of flagged pixels (order preserving).
 //   3. voxel keys floor(p / cell) as i64 and their per-axis range.
 //   4. integer voxel hash: 63-bit packed keys (21 bits per axis, relative to
@@ -24,6 +27,8 @@
 //      in first-insertion order.
 // The compactions and scans are this file's own three-phase kernels (block
 // sums, a one-block scan of the block sums, block-local scan + scatter).
+#include <algorithm>
+
 #include "stereo_kernels.h"
 
 namespace mvs {
@@ -53,13 +58,30 @@ __device__ __forceinline__ double sample_depth(const float* __restrict__ depth, 
     return wsum > 1e-6f ? (double)(acc / wsum) : 0.0;
 }
 
-// fuse.rs:98-163 for one pixel.
+// Pixels with a depth (fuse.rs:98-100 skips the others): vflag[g] = depth > 0
+// for every pixel g of every view with a depth map (grid row y = view y),
+// and the candidate flag cleared.
+__global__ __launch_bounds__(256) void valid_kernel(const FuseView* __restrict__ V, uint8_t* __restrict__ vflag,
+                                                    uint8_t* __restrict__ flag) {
+    const FuseView& v = V[blockIdx.y];
+    if (!v.depth) return;
+    const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= (int64_t)v.w * v.h) return;
+    const int64_t g = v.pix_off + li;
+    vflag[g] = v.depth[li] > 0.0f ? 1 : 0;
+    flag[g] = 0;
+}
+
+// fuse.rs:98-163 for one pixel with a depth: thread i takes the i-th such
+// pixel (vid[i], ascending), so every lane of a wave runs the per-view loop.
 __global__ __launch_bounds__(256) void cand_kernel(const FuseView* __restrict__ V, int nv,
-                                                   const int32_t* __restrict__ order, int64_t n_pix, int min_cons,
-                                                   double depth_diff, uint8_t* __restrict__ flag,
-                                                   double* __restrict__ xyz, uchar4* __restrict__ rgb) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n_pix) return;
+                                                   const int32_t* __restrict__ order, const int32_t* __restrict__ vid,
+                                                   int64_t n_valid, int min_cons, double depth_diff,
+                                                   uint8_t* __restrict__ flag, double* __restrict__ xyz,
+                                                   uchar4* __restrict__ rgb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_valid) return;
+    const int64_t g = vid[i];
     int lo = 0, hi = nv - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -68,9 +90,8 @@ __global__ __launch_bounds__(256) void cand_kernel(const FuseView* __restrict__ 
     const FuseView& v = V[lo];
     const int64_t li = g - v.pix_off;
     const int px = (int)(li % v.w), py = (int)(li / v.w);
-    uint8_t ok = 0;
     const double zr = (double)v.depth[li];
-    if (zr > 0.0) {
+    {
         double nx, ny;
         cam_normalize(v.f, v.cx, v.cy, v.k, (double)px, (double)py, nx, ny);
         const double dr[3] = {nx * zr - v.t[0], ny * zr - v.t[1], 1.0 * zr - v.t[2]};
@@ -122,10 +143,9 @@ __global__ __launch_bounds__(256) void cand_kernel(const FuseView* __restrict__ 
             }
             c.w = 0;
             rgb[g] = c;
-            ok = 1;
+            flag[g] = 1;
         }
     }
-    flag[g] = ok;
 }
 
 __global__ void key_init_kernel(long long* kr) {
@@ -407,7 +427,8 @@ struct Buf {
 };
 
 struct FuseScratch {
-    Buf<uint8_t> flag, first_flag;
+    Buf<uint8_t> flag, first_flag, vflag;
+    Buf<int32_t> vid;
     Buf<double> xyz;
     Buf<uchar4> rgb;
     Buf<int32_t> cid, counts;   // counts: [0] n_sel, [1] n_pts, [2] err
@@ -425,7 +446,8 @@ struct FuseScratch {
 
 void free_fuse_scratch(FuseScratch* s) {
     if (!s) return;
-    s->flag.release(); s->first_flag.release(); s->xyz.release(); s->rgb.release();
+    s->flag.release(); s->first_flag.release(); s->vflag.release(); s->vid.release();
+    s->xyz.release(); s->rgb.release();
     s->cid.release(); s->counts.release(); s->key3.release(); s->kr.release(); s->slot.release();
     s->cnt.release(); s->off.release(); s->cursor.release(); s->member.release(); s->bsum.release();
     s->by_first.release(); s->out.release(); s->table.release();
@@ -436,7 +458,6 @@ static inline unsigned nblk(size_t n, unsigned b = 256) { return (unsigned)((n +
 
 hipError_t run_fusion(FuseScratch*& S, const FuseArgs& a, const FuseView* h_views, hipStream_t st, int64_t* n_cand,
                       int64_t* n_pts, const void** d_points, int* limit_err) {
-    (void)h_views;
     *limit_err = 0;
     *n_cand = 0;
     *n_pts = 0;
@@ -450,16 +471,34 @@ hipError_t run_fusion(FuseScratch*& S, const FuseArgs& a, const FuseView* h_view
         return hipSuccess;
     }
     FCHK(S->flag.need(np));
+    FCHK(S->vflag.need(np));
+    FCHK(S->vid.need(np));
     FCHK(S->xyz.need(3 * np));
     FCHK(S->rgb.need(np));
     FCHK(S->cid.need(np));
-    hipLaunchKernelGGL(cand_kernel, dim3(nblk(np)), dim3(256), 0, st, a.d_views, a.nviews, a.d_order, (int64_t)np,
-                       a.min_consistent, a.depth_diff, S->flag.p, S->xyz.p, S->rgb.p);
-    FCHK(hipGetLastError());
-    // compaction of candidates (order preserving): cid[k] = index of the k-th flagged pixel
     const size_t nb1 = (np + SCAN_B - 1) / SCAN_B;
     FCHK(S->bsum.need(nb1 + 1));
     uint32_t* total = S->bsum.p + nb1;
+    // pixels with a depth, then their list in global order (vid)
+    int64_t max_px = 0;
+    for (int i = 0; i < a.nviews; ++i)
+        if (h_views[i].depth) max_px = std::max(max_px, (int64_t)h_views[i].w * h_views[i].h);
+    hipLaunchKernelGGL(valid_kernel, dim3(nblk((size_t)max_px), (unsigned)a.nviews), dim3(256), 0, st, a.d_views,
+                       S->vflag.p, S->flag.p);
+    hipLaunchKernelGGL((block_sum_kernel<uint8_t>), dim3((unsigned)nb1), dim3(SCAN_T), 0, st, S->vflag.p,
+                       (int64_t)np, S->bsum.p);
+    hipLaunchKernelGGL(scan_bsum_kernel, dim3(1), dim3(SCAN2_T), 0, st, S->bsum.p, (int)nb1, total);
+    hipLaunchKernelGGL((compact_kernel<int32_t, true>), dim3((unsigned)nb1), dim3(SCAN_T), 0, st, S->vflag.p,
+                       (int64_t)np, S->bsum.p, (const int32_t*)nullptr, S->vid.p);
+    FCHK(hipGetLastError());
+    uint32_t nvalid = 0;
+    FCHK(hipMemcpyAsync(&nvalid, total, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    FCHK(hipStreamSynchronize(st));
+    if (nvalid > 0)
+        hipLaunchKernelGGL(cand_kernel, dim3(nblk(nvalid)), dim3(256), 0, st, a.d_views, a.nviews, a.d_order,
+                           S->vid.p, (int64_t)nvalid, a.min_consistent, a.depth_diff, S->flag.p, S->xyz.p, S->rgb.p);
+    FCHK(hipGetLastError());
+    // compaction of candidates (order preserving): cid[k] = index of the k-th flagged pixel
     hipLaunchKernelGGL((block_sum_kernel<uint8_t>), dim3((unsigned)nb1), dim3(SCAN_T), 0, st, S->flag.p, (int64_t)np,
                        S->bsum.p);
     hipLaunchKernelGGL(scan_bsum_kernel, dim3(1), dim3(SCAN2_T), 0, st, S->bsum.p, (int)nb1, total);

@@ -1281,6 +1281,40 @@ __device__ __forceinline__ void vol_costs(const u64* s_cen, const float2* s_ray,
     }
 }
 
+// Four pinhole sources on the scaling-free divide (the usual case): the tap
+// offsets of all four sources and the thread's NIT pixels first, then all
+// their census gathers (4 NIT in flight at once), then the popcounts.
+template <int NP, int NT>
+__device__ __forceinline__ void vol_costs4(const u64* s_cen, const float2* s_ray, float d, const SrcP* P,
+                                           unsigned* tot) {
+    constexpr int K = (NP + NT - 1) / NT;
+    unsigned off[4][K];
+    bool ok[4][K];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = (int)threadIdx.x + k * NT;
+            ok[s][k] = false;
+            off[s][k] = 0;
+            if (i < NP) ok[s][k] = tap_v4<true, true, true>(s_ray[i], 0.0f, 0.0f, 1.0f, d, P[s], off[s][k]);
+        }
+    u64 cs[4][K];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int k = 0; k < K; ++k) cs[s][k] = cen_at(P[s], off[s][k]);   // offset 0 when invalid: in bounds
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = (int)threadIdx.x + k * NT;
+        if (i < NP) {
+            const u64 cr = s_cen[i];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) tot[k] += ok[s][k] ? (unsigned)__popcll(cr ^ cs[s][k]) : 48u;
+        }
+    }
+}
+
 template <int TW, int TH, int S>
 __global__ __launch_bounds__(TW* TH) void volume_kernel(const JobDesc* __restrict__ jobs,
                                                         const SrcDesc* __restrict__ srcs,
@@ -1312,6 +1346,8 @@ __global__ __launch_bounds__(TW* TH) void volume_kernel(const JobDesc* __restric
     const size_t oi = (size_t)y * cw + x;
     const bool fast = (J.pinhole_ok & 2) != 0, pin_ok = (J.pinhole_ok & 1) != 0;
     const float denom = (float)(48 * 25 * J.nsrc);
+    bool four = J.nsrc == 4 && fast && pin_ok;
+    for (int si = 0; si < 4 && four; ++si) four = Sd[si].k == 0.0f;
     for (int hi = 0; hi < nplanes; ++hi) {
         const float inv = J.inv_min + (J.inv_max - J.inv_min) * (float)hi / (float)(nplanes - 1);
         const float d = 1.0f / inv;
@@ -1322,7 +1358,11 @@ __global__ __launch_bounds__(TW* TH) void volume_kernel(const JobDesc* __restric
         unsigned tot[NIT];
 #pragma unroll
         for (int k = 0; k < NIT; ++k) tot[k] = 0;
-        for (int si = 0; si < J.nsrc; ++si) {
+        if (four) {
+            const SrcP P4[4] = {load_src(Sd[0]), load_src(Sd[1]), load_src(Sd[2]), load_src(Sd[3])};
+            vol_costs4<NP, NT>(s_cen, s_ray, d, P4, tot);
+        }
+        for (int si = 0; si < (four ? 0 : J.nsrc); ++si) {
             const SrcP P = load_src(Sd[si]);
             const bool pin = pin_ok && P.k == 0.0f;
             if (fast && pin)
