@@ -604,8 +604,42 @@ def learned_block(ctx, sc, views, jobs, K, opts, args, ms_prepare, ms_plain, ms_
     vps_learned = len(views) / ((ms_prepare + ms_learned + ms_fuse) / 1000.0)
     flops = net_conv_flops(views)
     net.close()
+    # the plain method's own speed / accuracy curve on the same workload (1-4 sweeps, each timed with its
+    # fusion), and its views/s interpolated at the learned run's agreement: the equal-accuracy comparison
+    curve = {}
+    mode = (args.mode & 3) | 4
+    for ns in (1, 2, 3, 4):
+        o = MvsOptions(num_sources=K, sweeps=ns)
+        ctx.estimate_depth_batch(jobs, o, mode=mode, download=False)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            ctx.estimate_depth_batch(jobs, o, mode=mode, download=False)
+        torch.cuda.synchronize()
+        ms_s = 1000.0 * (time.perf_counter() - t) / steps
+        d_s = {r: ctx.get_depth(r).depth for r in range(len(views))}
+        ctx.fuse_resident([True] * len(views), order, args.cell, o)
+        ms_f = ctx.stats().ms_fuse
+        curve[str(ns)] = {"views_per_s": round(len(views) / ((ms_prepare + ms_s + ms_f) / 1000.0), 3),
+                          "agreement_with_plain_8": round(float(np.mean([_agree(ref8[r], d_s[r]) for r in ref8])), 4),
+                          "ms_stereo": round(ms_s, 2), "ms_fusion": round(ms_f, 2)}
+    curve["8"] = {"views_per_s": round(vps_plain, 3), "agreement_with_plain_8": 1.0, "ms_stereo": round(ms_plain, 2),
+                  "ms_fusion": round(ms_fuse_plain, 2)}
+    pts = sorted((c["agreement_with_plain_8"], c["views_per_s"]) for c in curve.values())
+    vps_eq = None
+    for (a0, v0), (a1, v1) in zip(pts, pts[1:]):
+        if a0 <= agree <= a1 and a1 > a0:
+            vps_eq = v0 + (v1 - v0) * (agree - a0) / (a1 - a0)
+            break
+    if vps_eq is None and agree < pts[0][0]:
+        vps_eq = pts[0][1]   # below the 1-sweep agreement: the fastest plain run is already more accurate
     out = {"views_per_s": round(vps_learned, 3), "plain_8_views_per_s": round(vps_plain, 3),
            "speedup": round(vps_learned / vps_plain, 3),
+           "plain_views_per_s_at_equal_agreement": None if vps_eq is None else round(vps_eq, 3),
+           "speedup_at_equal_agreement": None if vps_eq is None else round(vps_learned / vps_eq, 3),
+           "equal_agreement_definition": "plain views/s interpolated linearly in agreement between the plain runs "
+                                         "of 1, 2, 3, 4 and 8 sweeps (plain_by_sweeps) at the learned run's agreement",
+           "plain_by_sweeps": curve,
            "agreement_with_plain_8": round(agree, 4),
            "agreement_definition": "pixels valid in the plain 8-sweep map that are valid with inverse depth "
                                    "within 1 % of it",

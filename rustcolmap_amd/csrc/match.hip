@@ -60,6 +60,22 @@ __device__ __forceinline__ void merge_lanes(float& b, uint32_t& bi, float& s, in
 }
 
 // Flattened tile t -> pair: the last pair whose tile0 <= t.
+// The same search by one wave: tile0 ascends, so the pairs with tile0 <= t
+// are a prefix; each round tests 64 of them at once (a handful of
+// independent loads instead of log2(npairs) dependent ones).  Every lane of
+// the wave must call it.
+__device__ __forceinline__ int find_pair_wave(const PairDesc* pairs, int npairs, int64_t t) {
+    const int lane = threadIdx.x & 63;
+    int best = 0;
+    for (int base = 0; base < npairs; base += 64) {
+        const int p = base + lane;
+        const uint64_t m = __ballot(p < npairs && pairs[p].tile0 <= t);
+        if (m) best = base + 63 - __builtin_clzll(m);
+        if (m != ~0ull) break;
+    }
+    return best;
+}
+
 __device__ __forceinline__ int find_pair(const PairDesc* pairs, int npairs, int64_t t) {
     int lo = 0, hi = npairs - 1;
     while (lo < hi) {
@@ -443,7 +459,10 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
     const int64_t t = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
     if (t >= ntiles) return;
     __shared__ int s_pair;
-    if (threadIdx.x == 0) s_pair = find_pair(pairs, npairs, t);
+    if (threadIdx.x < 64) {
+        const int p = find_pair_wave(pairs, npairs, t);
+        if (threadIdx.x == 0) s_pair = p;
+    }
     if (PASS == 1 && t == 0 && threadIdx.x == 0) *fb.ovf_n = 0u;   // read by the collection after this launch
     __syncthreads();
     const PairDesc P = pairs[s_pair];
@@ -451,6 +470,15 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
     const int rt = (int)(lt / P.tb), ct = (int)(lt - (int64_t)rt * P.tb);
     const int i0 = rt * FTILE, j0 = ct * FTILE;
     const int nra = min(FTILE, P.na - i0), nrb = min(FTILE, P.nb - j0);
+    // squared norms of the tile's rows (threads 0..127) and columns (128..255),
+    // loaded now and put in LDS after the products (0 past a set)
+    __shared__ float s_norm[2 * FTILE];
+    float my_norm = 0.0f;
+    {
+        const int e = threadIdx.x & (FTILE - 1);
+        const bool is_row = threadIdx.x < FTILE;
+        if (e < (is_row ? nra : nrb)) my_norm = norm[is_row ? P.a + i0 + e : P.b + j0 + e];
+    }
 
     __shared__ __align__(16) uint16_t sAh[FTILE * LROW], sAl[FTILE * LROW], sBh[FTILE * LROW], sBl[FTILE * LROW];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
@@ -534,18 +562,20 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
     // L = NaN fails every compare.
     const float scale = PASS == 1 ? 1.0f + EPS_REL : 1.0f;
     const float pad = PASS == 1 ? INFINITY : 0.0f;
+    s_norm[threadIdx.x] = my_norm;
+    __syncthreads();
     float nrow[4][4], ncol[4];
 #pragma unroll
     for (int mf = 0; mf < 4; ++mf)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int R = wr * 64 + mf * 16 + kq * 4 + i;
-            nrow[mf][i] = R < nra ? scale * norm[P.a + i0 + R] : pad;
+            nrow[mf][i] = R < nra ? scale * s_norm[R] : pad;
         }
 #pragma unroll
     for (int nf = 0; nf < 4; ++nf) {
         const int C = wc * 64 + nf * 16 + r16;
-        ncol[nf] = C < nrb ? scale * norm[P.b + j0 + C] : pad;
+        ncol[nf] = C < nrb ? scale * s_norm[FTILE + C] : pad;
     }
 
     if constexpr (PASS == 1) {
@@ -612,8 +642,7 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
             (is_row ? sur : suc)[e] = a;
             // scaled norms are >= 0, so their bit patterns order as the values
             if (e < (is_row ? nra : nrb))
-                atomicMax(&snmax[is_row ? 0 : 1],
-                          __float_as_uint((1.0f + EPS_REL) * norm[is_row ? P.a + i0 + e : P.b + j0 + e]));
+                atomicMax(&snmax[is_row ? 0 : 1], __float_as_uint((1.0f + EPS_REL) * s_norm[q]));
         }
         __syncthreads();
         // Second sweep over the accumulators: bit masks of the elements with
@@ -691,7 +720,7 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
         const bool is_row = q < FTILE;
         if (e < (is_row ? nra : nrb)) {
             const float2 u = is_row ? sur[e] : suc[e];
-            const float own = (1.0f + EPS_REL) * norm[is_row ? P.a + i0 + e : P.b + j0 + e];
+            const float own = (1.0f + EPS_REL) * s_norm[q];
             const float other = __uint_as_float(snmax[is_row ? 1 : 0]);
             // every element's L >= u1 - kb * max tU; the factor and the
             // constant cover the rounding of L
