@@ -539,30 +539,32 @@ __device__ __forceinline__ float eval_v4_body(const u64* __restrict__ s_cen, con
                                               float d, const AbortThr& A, const float* __restrict__ tab, int& nr) {
     float sum = 0.0f;
     {
-        // divisors of source 0's row tests, capped by source 1's first test
-        const double cap = nsrc > 1 ? 10.0 : 1e30;
+        // divisors of source 0's row tests 5 (r + 1), capped at 10 by source
+        // 1's first test; selected as constants (wave-uniform, scalar) rather
+        // than formed with a vector min, whose results were spilled
+        const bool capped = nsrc > 1;
         const SrcP P = load_src(S[0]);
         row_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, 0, nx, ny, nz, d, P, tab, sum);
         nr = 1;
         if (abort_test(sum, 5.0, A)) return 1e9f;
         if (PAIR) {
             nr = 3;
-            if (rows12_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, nx, ny, nz, d, P, tab, A, fmin(10.0, cap),
-                                           fmin(15.0, cap), sum))
+            if (rows12_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, nx, ny, nz, d, P, tab, A, 10.0, capped ? 10.0 : 15.0,
+                                           sum))
                 return 1e9f;
         } else {
 #pragma unroll
             for (int r = 1; r < 3; ++r) {
                 row_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, r, nx, ny, nz, d, P, tab, sum);
                 nr = r + 1;
-                if (abort_test(sum, fmin((double)(r + 1) * 5.0, cap), A)) return 1e9f;
+                if (abort_test(sum, capped ? 10.0 : (double)(5 * (r + 1)), A)) return 1e9f;
             }
         }
 #pragma unroll
         for (int r = 3; r < 5; ++r) {
             row_v4<PW, FRONTO, PIN>(s_cen, s_ray, c, r, nx, ny, nz, d, P, tab, sum);
             nr = r + 1;
-            if (abort_test(sum, fmin((double)(r + 1) * 5.0, cap), A)) return 1e9f;
+            if (abort_test(sum, capped ? 10.0 : (double)(5 * (r + 1)), A)) return 1e9f;
         }
     }
     // not unrolled: evaluations rarely reach a second source, and unrolling by
@@ -572,9 +574,11 @@ __device__ __forceinline__ float eval_v4_body(const u64* __restrict__ s_cen, con
         // si is the same in every lane still here; saying so keeps the
         // source parameters in scalar registers (a loop with lane-dependent
         // exits is otherwise treated as divergent and loads them per lane)
-        const SrcP P = load_src(S[__builtin_amdgcn_readfirstlane(si)]);
-        const double fs = (double)(si + 1) * 5.0;
-        const double cap = si + 1 < nsrc ? (double)(si + 2) * 5.0 : 1e30;
+        const int su = __builtin_amdgcn_readfirstlane(si);
+        const SrcP P = load_src(S[su]);
+        // row r's divisor 5 (r + 1) (si + 1), capped at the next source's
+        // first divisor 5 (si + 2) (exact integers, as fmin of the products)
+        const bool capped = su + 1 < nsrc;
         // an opaque copy of the window index keeps the compiler from hoisting
         // the 25 ray and 25 census LDS reads out of this loop (100 registers)
         int cc = c;
@@ -584,14 +588,16 @@ __device__ __forceinline__ float eval_v4_body(const u64* __restrict__ s_cen, con
             for (int r = 0; r < 5; ++r) {
                 row_v4<PW, FRONTO, true>(s_cen, s_ray, cc, r, nx, ny, nz, d, P, tab, sum);
                 nr = 5 * si + r + 1;
-                if (abort_test(sum, fmin((double)(r + 1) * fs, cap), A)) return 1e9f;
+                const int dv = (r == 0 || !capped) ? (r + 1) * (su + 1) : su + 2;
+                if (abort_test(sum, (double)(5 * dv), A)) return 1e9f;
             }
         } else {
 #pragma unroll
             for (int r = 0; r < 5; ++r) {
                 row_v4<PW, FRONTO, false>(s_cen, s_ray, cc, r, nx, ny, nz, d, P, tab, sum);
                 nr = 5 * si + r + 1;
-                if (abort_test(sum, fmin((double)(r + 1) * fs, cap), A)) return 1e9f;
+                const int dv = (r == 0 || !capped) ? (r + 1) * (su + 1) : su + 2;
+                if (abort_test(sum, (double)(5 * dv), A)) return 1e9f;
             }
         }
     }
