@@ -54,7 +54,7 @@ LEARNED_SUMMARY = os.path.join(ROOT, "profiles", "learned_room1080_kernel_trace.
 FP64_PEAK_TFLOPS = 78.6                    # FP64 vector
 HBM_PEAK_GBS = 8000.0
 # every kernel a fusion run launches (csrc/fuse.hip)
-FUSION_KERNELS = ("mvs::cand_kernel", "mvs::key_init_kernel", "mvs::key_kernel", "mvs::insert_kernel",
+FUSION_KERNELS = ("mvs::valid_kernel", "mvs::cand_kernel", "mvs::key_init_kernel", "mvs::key_kernel", "mvs::insert_kernel",
                   "mvs::reduce_kernel", "mvs::compact_kernel", "mvs::exscan_kernel", "mvs::block_sum_kernel",
                   "mvs::scan_bsum_kernel", "mvs::slot_count_kernel", "mvs::slot_place_kernel", "mvs::fill_u32_k",
                   "mvs::fill_u64_k", "mvs::fill_u8_k")
