@@ -499,7 +499,12 @@ __device__ __forceinline__ void row_v4(const u64* __restrict__ s_cen, const floa
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
         const u64 rc = s_cen[c + (r - 2) * PW + (q - 2)];
-        sum += ok[q] ? tab[__popcll(rc ^ cs[q])] : 1.0f;
+        // tab[48] = 48 / 48 = 1 exactly: an invalid tap's 1.0 from the same
+        // table read; the popcount is pinned (empty asm) so that it is
+        // computed for every lane and selected, with no branch around it
+        unsigned pc = (unsigned)__popcll(rc ^ cs[q]);
+        asm volatile("" : "+v"(pc));
+        sum += tab[ok[q] ? pc : 48u];
     }
 }
 
@@ -526,7 +531,9 @@ __device__ __forceinline__ bool rows12_v4(const u64* __restrict__ s_cen, const f
     for (int t = 0; t < 10; ++t) {
         const int r = 1 + t / 5, q = t % 5;
         const u64 rc = s_cen[c + (r - 2) * PW + (q - 2)];
-        sum += ((okm >> t) & 1u) ? tab[__popcll(rc ^ cs[t])] : 1.0f;
+        unsigned pc = (unsigned)__popcll(rc ^ cs[t]);
+        asm volatile("" : "+v"(pc));
+        sum += tab[((okm >> t) & 1u) ? pc : 48u];
         if (t == 4) ab = abort_test(sum, D1, A);
         if (t == 9) ab = ab || abort_test(sum, D2, A);
     }
