@@ -640,9 +640,13 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
             const float2 b = is_row ? srow[FTILE + e] : scol[FTILE + e];
             two_min_merge(a.x, a.y, b.x, b.y);
             (is_row ? sur : suc)[e] = a;
-            // scaled norms are >= 0, so their bit patterns order as the values
-            if (e < (is_row ? nra : nrb))
-                atomicMax(&snmax[is_row ? 0 : 1], __float_as_uint((1.0f + EPS_REL) * s_norm[q]));
+            // scaled norms are >= 0, so their bit patterns order as the values;
+            // a wave holds 64 rows or 64 columns: its maximum, then one atomic
+            // per wave (256 atomics on two words serialised in the LDS)
+            unsigned nm = e < (is_row ? nra : nrb) ? __float_as_uint((1.0f + EPS_REL) * s_norm[q]) : 0u;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) nm = max(nm, (unsigned)__shfl_xor((int)nm, off));
+            if ((threadIdx.x & 63) == 0) atomicMax(&snmax[is_row ? 0 : 1], nm);
         }
         __syncthreads();
         // Second sweep over the accumulators: bit masks of the elements with
