@@ -125,3 +125,21 @@ def test_conv_kernel_mfma_from_counters():
     n = bench.profiled_counter("mvs::net::conv3x3_lds_kernel", "SQ_INSTS_VALU_MFMA_BF16", bench.LEARNED_SUMMARY)
     assert n is not None
     assert c["mfma_flops_per_dispatch"] == pytest.approx(n[0] * 16384, rel=2e-3)
+
+
+@pytest.mark.parametrize("shape", ["24, 8", "16, 8", "16, 16"])
+def test_every_sorted_pass_shape_is_profiled(shape):
+    """The tuner picks among the sorted pass shapes by timing; whichever it
+    picks, the default line's roofline fields come from the committed
+    profiles: HBM traffic, lane utilisation and VALU instructions, and a
+    kernel-trace launch time within a few percent of the other shapes'."""
+    k = "mvs::pass_sorted_kernel<%s, 3>" % shape
+    t = bench.profiled_traffic(k)
+    assert t is not None and 4.98e9 <= t[0] < 5e10
+    lu = bench.lane_utilisation(k)
+    assert lu is not None and 0.5 < lu <= 1.0
+    v = bench.valu_roofline(k, 10.8)
+    assert v is not None and 0.0 < v["frac"] < 1.0
+    ns = _avg_ns(os.path.join(ROOT, "profiles", "stereo_room1080_kernel_trace.txt"), k)
+    ref = _avg_ns(os.path.join(ROOT, "profiles", "stereo_room1080_kernel_trace.txt"), PASS)
+    assert ns is not None and ns == pytest.approx(ref, rel=0.05)
