@@ -1410,14 +1410,14 @@ __global__ __launch_bounds__(TW* TH) void volume_kernel(const JobDesc* __restric
 // conf_thr and the proposed normal faces the ray (n . r >= 0.1), the pixel
 // starts from the proposed plane with its plane_cost over all sources;
 // elsewhere it runs the plain plane-sweep initialisation.
+// One TW x TH tile (tl: job, x0, y0) of the learned initialisation; every
+// thread of the block calls it.
 template <int TW, int TH, int EV>
-__global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __restrict__ jobs,
-                                                               const SrcDesc* __restrict__ srcs,
-                                                               const int4* __restrict__ tiles, int ntiles, int nh,
-                                                               const float* const* __restrict__ preds, int stride,
-                                                               float conf_thr, u64* __restrict__ counter,
-                                                               u64* __restrict__ accepted, u64* __restrict__ list,
-                                                               float4* __restrict__ prop) {
+__device__ __forceinline__ void proposal_init_tile(const JobDesc* __restrict__ jobs, const SrcDesc* __restrict__ srcs,
+                                                   const int4 tl, int nh, const float* const* __restrict__ preds,
+                                                   int stride, float conf_thr, u64* __restrict__ counter,
+                                                   u64* __restrict__ accepted, u64* __restrict__ list,
+                                                   float4* __restrict__ prop) {
     constexpr int PW = TW + 4;
     __shared__ u64 s_cen[(TW + 4) * (TH + 4)];
     __shared__ float2 s_ray[(TW + 4) * (TH + 4)];
@@ -1425,7 +1425,6 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
     __shared__ unsigned s_cnt, s_acc;
     __shared__ u64 s_base;
     __shared__ float s_tab[64];
-    const int4 tl = tiles[xcd_remap(blockIdx.x, ntiles)];
     const JobDesc& J = jobs[tl.x];
     if (threadIdx.x == 0) {
         s_cnt = 0;
@@ -1493,6 +1492,41 @@ __global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __
     atomicAdd(&s_cnt, ne);
     __syncthreads();
     if (threadIdx.x == 0 && counter) atomicAdd(counter, (u64)s_cnt);
+}
+
+template <int TW, int TH, int EV>
+__global__ __launch_bounds__(TW* TH) void proposal_init_kernel(const JobDesc* __restrict__ jobs,
+                                                               const SrcDesc* __restrict__ srcs,
+                                                               const int4* __restrict__ tiles, int ntiles, int nh,
+                                                               const float* const* __restrict__ preds, int stride,
+                                                               float conf_thr, u64* __restrict__ counter,
+                                                               u64* __restrict__ accepted, u64* __restrict__ list,
+                                                               float4* __restrict__ prop) {
+    proposal_init_tile<TW, TH, EV>(jobs, srcs, tiles[xcd_remap(blockIdx.x, ntiles)], nh, preds, stride, conf_thr,
+                                   counter, accepted, list, prop);
+}
+
+// The same over the 8 x 8 sub-tiles of a sorted launch's TW x TH tiles (one
+// wave each, as init_sub_kernel): a smaller block stops costing a later
+// source as soon as its own 64 pixels are done with it.  Sub-tile origins
+// are multiples of 8, so a pixel's coarse pixel lies in its sub-tile.
+template <int TW, int TH>
+__global__ __launch_bounds__(64) void proposal_init_sub_kernel(const JobDesc* __restrict__ jobs,
+                                                               const SrcDesc* __restrict__ srcs,
+                                                               const int4* __restrict__ tiles, int ntiles, int nh,
+                                                               const float* const* __restrict__ preds, int stride,
+                                                               float conf_thr, u64* __restrict__ counter,
+                                                               u64* __restrict__ accepted, u64* __restrict__ list,
+                                                               float4* __restrict__ prop) {
+    static_assert(TW % 8 == 0 && TH % 8 == 0, "whole 8x8 sub-tiles");
+    constexpr int NSX = TW / 8, NSUB = NSX * (TH / 8);
+    const int b = xcd_remap(blockIdx.x, ntiles * NSUB);
+    const int sub = b % NSUB;
+    const int4 tl = tiles[b / NSUB];
+    const int x0 = tl.y + (sub % NSX) * 8, y0 = tl.z + (sub / NSX) * 8;
+    if (x0 >= jobs[tl.x].w || y0 >= jobs[tl.x].h) return;   // the same for the whole block
+    proposal_init_tile<8, 8, 3>(jobs, srcs, make_int4(tl.x, x0, y0, 0), nh, preds, stride, conf_thr, counter,
+                                accepted, list, prop);
 }
 
 // Costs of the accepted proposals: one thread per entry of the compacted
@@ -1755,6 +1789,17 @@ hipError_t launch_proposal_init(const TileCfg& cfg, const JobDesc* jobs, const S
                                 int ntiles, int nh, const float* const* preds, int stride, float conf_thr,
                                 u64* counter, u64* accepted, u64* list, float4* prop, hipStream_t st) {
     bool done = false;
+    if (cfg.persistent == 2) {
+#define X(W, H)                                                                                                   \
+        if (!done && cfg.tw == W && cfg.th == H && cfg.eval == 3) {                                               \
+            hipLaunchKernelGGL((proposal_init_sub_kernel<W, H>), dim3(ntiles * (W / 8) * (H / 8)), dim3(64), 0, st, \
+                               jobs, srcs, tiles, ntiles, nh, preds, stride, conf_thr, counter, accepted, list,     \
+                               prop);                                                                           \
+            done = true;                                                                                        \
+        }
+        SORTED_CASES(X)
+#undef X
+    }
 #define X(W, H, E)                                                                                              \
     if (!done && cfg.tw == W && cfg.th == H && cfg.eval == E) {                                                 \
         hipLaunchKernelGGL((proposal_init_kernel<W, H, E>), dim3(ntiles), dim3(W * H), 0, st, jobs, srcs, tiles, \
