@@ -599,6 +599,8 @@ def learned_block(ctx, sc, views, jobs, K, opts, args, ms_prepare, ms_plain, ms_
     netm /= steps
     learned_d = {r: ctx.get_depth(r).depth for r in range(len(views))}
     order = select_sources(sc.views, len(views) - 1)
+    # the first fusion of the learned maps also grows the candidate buffers (more valid pixels): time the second
+    ctx.fuse_resident([True] * len(views), order, args.cell, short)
     ctx.fuse_resident([True] * len(views), order, args.cell, short)
     ms_fuse = ctx.stats().ms_fuse
     agree = float(np.mean([_agree(ref8[r], learned_d[r]) for r in ref8]))

@@ -153,33 +153,52 @@ __global__ void key_init_kernel(long long* kr) {
     else if (threadIdx.x < 6) kr[threadIdx.x] = (-9223372036854775807LL - 1);
 }
 
-// fuse.rs:180-184: floor(p / cell) as i64
+// fuse.rs:180-184: floor(p / cell) as i64, and the per-axis key range: each
+// thread takes KEY_PER_THREAD candidates (grid stride), then the range is
+// reduced over the wave and the block and one atomic per block and axis goes
+// to memory (one per wave and axis made ~20 k waves serialise on six words).
+constexpr int KEY_PER_THREAD = 8;
+
 __global__ __launch_bounds__(256) void key_kernel(const int32_t* __restrict__ cid, int n, const double* __restrict__ xyz,
                                                   double cell, long long* __restrict__ key3, long long* __restrict__ kr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    long long k[3] = {0, 0, 0};
-    const bool act = i < n;
-    if (act) {
+    __shared__ long long s_mn[3][4], s_mx[3][4];
+    long long mn[3], mx[3];
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = 9223372036854775807LL;
+        mx[a] = (-9223372036854775807LL - 1);
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int64_t g = cid[i];
         for (int a = 0; a < 3; ++a) {
-            k[a] = f64_to_i64_sat(floor(xyz[3 * g + a] / cell));
-            key3[3 * (size_t)i + a] = k[a];
+            const long long k = f64_to_i64_sat(floor(xyz[3 * g + a] / cell));
+            key3[3 * (size_t)i + a] = k;
+            mn[a] = k < mn[a] ? k : mn[a];
+            mx[a] = k > mx[a] ? k : mx[a];
         }
     }
-    // wave-level min/max, then one atomic per wave and axis
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int a = 0; a < 3; ++a) {
-        long long mn = act ? k[a] : 9223372036854775807LL;
-        long long mx = act ? k[a] : (-9223372036854775807LL - 1);
         for (int off = 32; off > 0; off >>= 1) {
-            const long long omn = __shfl_xor(mn, off, 64);
-            const long long omx = __shfl_xor(mx, off, 64);
-            mn = omn < mn ? omn : mn;
-            mx = omx > mx ? omx : mx;
+            const long long omn = __shfl_xor(mn[a], off, 64);
+            const long long omx = __shfl_xor(mx[a], off, 64);
+            mn[a] = omn < mn[a] ? omn : mn[a];
+            mx[a] = omx > mx[a] ? omx : mx[a];
         }
-        if ((threadIdx.x & 63) == 0) {
-            atomicMin(&kr[a], mn);
-            atomicMax(&kr[3 + a], mx);
+        if (lane == 0) {
+            s_mn[a][wave] = mn[a];
+            s_mx[a][wave] = mx[a];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int a = threadIdx.x;
+        long long bmn = s_mn[a][0], bmx = s_mx[a][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            bmn = s_mn[a][w] < bmn ? s_mn[a][w] : bmn;
+            bmx = s_mx[a][w] > bmx ? s_mx[a][w] : bmx;
+        }
+        atomicMin(&kr[a], bmn);
+        atomicMax(&kr[3 + a], bmx);
     }
 }
 
@@ -518,8 +537,8 @@ hipError_t run_fusion(FuseScratch*& S, const FuseArgs& a, const FuseView* h_view
     FCHK(S->first_flag.need(n));
     FCHK(S->out.need(n));
     hipLaunchKernelGGL(key_init_kernel, dim3(1), dim3(64), 0, st, S->kr.p);
-    hipLaunchKernelGGL(key_kernel, dim3(nblk(n)), dim3(256), 0, st, S->cid.p, (int)n, S->xyz.p, a.cell, S->key3.p,
-                       S->kr.p);
+    hipLaunchKernelGGL(key_kernel, dim3(nblk((n + KEY_PER_THREAD - 1) / KEY_PER_THREAD)), dim3(256), 0, st, S->cid.p,
+                       (int)n, S->xyz.p, a.cell, S->key3.p, S->kr.p);
     FCHK(hipGetLastError());
     long long kr[6];
     FCHK(hipMemcpyAsync(kr, S->kr.p, sizeof(kr), hipMemcpyDeviceToHost, st));
