@@ -47,18 +47,20 @@ struct Tuner {
     }
 
     // Built-in default, used until mvs_autotune has timed the shape class:
-    // 24x8 tiles in the sorted form once the batch gives every wave slot of
-    // the machine a few dozen such tiles (at 6 waves x 4 SIMDs x 256 CUs,
-    // 16 Mpx is about 14 tiles per slot), else 8x8 tiles on the persistent
-    // queue, whose shorter tiles keep the end of each launch short.
-    // Measured pass time, sorted 24x8 against 8x8: -5 % on 20 x 1920x1080,
-    // -6.6 % on 100 x 1920x1080, -9.3 % on 64 views of 640x480 .. 2560x1440,
-    // +29 % on 10 x 320x240.
+    // 16x16 tiles in the sorted form once the batch gives every wave slot of
+    // the machine a few such tiles (at 6 waves x 4 SIMDs x 256 CUs, 16 Mpx is
+    // about 10 tiles per slot), else 8x8 tiles on the persistent queue, whose
+    // shorter tiles keep the end of each launch short.  Measured pass time,
+    // sorted 24x8 against 8x8: -5 % on 20 x 1920x1080, -6.6 % on 100 x
+    // 1920x1080, -9.3 % on 64 views of 640x480 .. 2560x1440, +29 % on 10 x
+    // 320x240; on 20 x 1920x1080 the three sorted shapes are within 3 %,
+    // 16x16 the fastest (16 launches: 168 ms against 172 at 16x8 and 173 at
+    // 24x8), as it also is in the tuner's one-pass estimates run after run.
     static TileCfg default_cfg(int64_t total_px) {
         TileCfg c;   // 8x8, persistent, evaluator 3
         if (total_px >= ((int64_t)16 << 20)) {
-            c.tw = 24;
-            c.th = 8;
+            c.tw = 16;
+            c.th = 16;
             c.persistent = 2;
         }
         return c;
