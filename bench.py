@@ -552,6 +552,23 @@ def _agree(ref, other, tol=0.01):
     return float(ok.mean())
 
 
+EQUAL_AGREEMENT_DEFINITION = ("plain views/s interpolated linearly in agreement between the plain runs of 1, 2, 3, "
+                              "4 and 8 sweeps (plain_by_sweeps) at the learned run's agreement")
+
+
+def _plain_vps_at(points, agree):
+    """Views/s of the plain method at agreement `agree`, interpolated linearly
+    between the (agreement, views/s) points of its runs at several sweep
+    counts; the fastest run's rate when `agree` lies below all of them."""
+    pts = sorted(points)
+    for (a0, v0), (a1, v1) in zip(pts, pts[1:]):
+        if a0 <= agree <= a1 and a1 > a0:
+            return v0 + (v1 - v0) * (agree - a0) / (a1 - a0)
+    if pts and agree < pts[0][0]:
+        return max(v for _, v in pts)
+    return None
+
+
 def _gt_within(depth_maps, gt_maps, tol=0.01):
     """Against the renderer's depth: the share of valid pixels within `tol`
     (relative depth) of the truth, and the share of all foreground pixels
@@ -630,20 +647,12 @@ def learned_block(ctx, sc, views, jobs, K, opts, args, ms_prepare, ms_plain, ms_
                           "ms_stereo": round(ms_s, 2), "ms_fusion": round(ms_f, 2)}
     curve["8"] = {"views_per_s": round(vps_plain, 3), "agreement_with_plain_8": 1.0, "ms_stereo": round(ms_plain, 2),
                   "ms_fusion": round(ms_fuse_plain, 2)}
-    pts = sorted((c["agreement_with_plain_8"], c["views_per_s"]) for c in curve.values())
-    vps_eq = None
-    for (a0, v0), (a1, v1) in zip(pts, pts[1:]):
-        if a0 <= agree <= a1 and a1 > a0:
-            vps_eq = v0 + (v1 - v0) * (agree - a0) / (a1 - a0)
-            break
-    if vps_eq is None and agree < pts[0][0]:
-        vps_eq = pts[0][1]   # below the 1-sweep agreement: the fastest plain run is already more accurate
+    vps_eq = _plain_vps_at([(c["agreement_with_plain_8"], c["views_per_s"]) for c in curve.values()], agree)
     out = {"views_per_s": round(vps_learned, 3), "plain_8_views_per_s": round(vps_plain, 3),
            "speedup": round(vps_learned / vps_plain, 3),
            "plain_views_per_s_at_equal_agreement": None if vps_eq is None else round(vps_eq, 3),
            "speedup_at_equal_agreement": None if vps_eq is None else round(vps_learned / vps_eq, 3),
-           "equal_agreement_definition": "plain views/s interpolated linearly in agreement between the plain runs "
-                                         "of 1, 2, 3, 4 and 8 sweeps (plain_by_sweeps) at the learned run's agreement",
+           "equal_agreement_definition": EQUAL_AGREEMENT_DEFINITION,
            "plain_by_sweeps": curve,
            "agreement_with_plain_8": round(agree, 4),
            "agreement_definition": "pixels valid in the plain 8-sweep map that are valid with inverse depth "
@@ -771,12 +780,18 @@ def learned_bench(args):
         (sweep if extra else replace_sweep)["%.2f" % conf] = row
         if first is None:
             first = row
+    curve = [(c["agreement_with_plain_8"], c["views_per_s"]) for c in by_sweeps.values()]
+    curve += [(agree_s, vps_short), (1.0, vps_plain)]
+    vps_eq = _plain_vps_at(curve, first["agreement_with_plain_8"])
     out = {"metric": "views/s, learned init + 2 sweeps vs plain 8 sweeps (%s)" % name,
            "value": first["views_per_s"], "unit": "views/s", "n_gpus": 1, "steps": steps, "warmup": 1,
            "higher_is_better": True, "data": "synthetic",
            "config": {"workload": "%s: %d views %dx%d K=%d vol_k=4 conf>=%.2f zrange [1, %g]"
                       % (name, len(views), views[0].w, views[0].h, K, args.conf, zmax)},
            "plain_views_per_s": round(vps_plain, 3), "speedup_views_per_s": first["speedup_vs_plain_8"],
+           "plain_views_per_s_at_equal_agreement": None if vps_eq is None else round(vps_eq, 3),
+           "speedup_at_equal_agreement": None if vps_eq is None else round(first["views_per_s"] / vps_eq, 3),
+           "equal_agreement_definition": EQUAL_AGREEMENT_DEFINITION,
            "agreement_with_plain_8": {"learned_2_sweeps": first["agreement_with_plain_8"],
                                       "plain_2_sweeps": round(agree_s, 4),
                                       "definition": "pixels valid in the 8-sweep plain map that are valid with "

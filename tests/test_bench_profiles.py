@@ -143,3 +143,15 @@ def test_every_sorted_pass_shape_is_profiled(shape):
     ns = _avg_ns(os.path.join(ROOT, "profiles", "stereo_room1080_kernel_trace.txt"), k)
     ref = _avg_ns(os.path.join(ROOT, "profiles", "stereo_room1080_kernel_trace.txt"), PASS)
     assert ns is not None and ns == pytest.approx(ref, rel=0.05)
+
+
+def test_plain_rate_at_equal_agreement_interpolates_between_sweep_counts():
+    # (agreement, views/s) of plain runs at 1, 2, 4 and 8 sweeps
+    curve = [(0.80, 500.0), (0.82, 300.0), (0.86, 200.0), (1.0, 100.0)]
+    assert bench._plain_vps_at(curve, 0.84) == pytest.approx(250.0)
+    assert bench._plain_vps_at(curve, 0.86) == pytest.approx(200.0)
+    # order of the points does not matter
+    assert bench._plain_vps_at(curve[::-1], 0.93) == pytest.approx(150.0)
+    # below every plain run: the fastest plain run already agrees better
+    assert bench._plain_vps_at(curve, 0.5) == 500.0
+    assert bench._plain_vps_at([], 0.9) is None
