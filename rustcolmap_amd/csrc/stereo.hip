@@ -299,7 +299,8 @@ __device__ __forceinline__ float eval_plane_fast(const u64* __restrict__ s_cen, 
 //     with m the midpoint of thr and its successor (exact in f64, 25
 //     significant bits) the quotient rounds above thr iff sum > m * D, or
 //     sum == m * D and thr's last mantissa bit is odd (ties to even); m * D
-//     is exact in f64 because D <= 5 * 5 * K has few bits;
+//     is exact in f64 because D <= 5 * 5 * K has few bits, and the tie
+//     cannot occur (make_abort_thr), so a row test is one f64 compare;
 //   * sources with k == 0 skip the distortion polynomial: dist = 1 + 0 * r2
 //     is exactly 1 whenever r2 is finite, which the host guarantees through
 //     a bound on zmax and the translation (JobDesc.pinhole_ok);
@@ -314,23 +315,24 @@ __device__ __forceinline__ int cvt_i32_rust(float x) {
 
 struct AbortThr {
     double m;
-    bool odd, never;
 };
 
+// RN(sum / D) > thr  <=>  sum > m D, or sum == m D with thr's last mantissa
+// bit odd, for m the f64 midpoint of thr and its f32 successor.  The tie
+// cannot happen: thr >= 0.02 is a normal float, thr = a 2^e with a 24-bit a,
+// so m = (2a + 1) 2^(e-1) has an odd 25-bit significand, and m D (exact in
+// f64: D = 5 dv < 2^28) has an odd part of at least 25 bits -- never an f32
+// value, which has at most 24.  So the row test is the single compare
+// sum > m D.  thr = inf gives m = NaN, and no test fires.
 __device__ __forceinline__ AbortThr make_abort_thr(float thr) {
     AbortThr a;
-    a.never = !(thr < __builtin_inff());
-    const uint32_t b = __float_as_uint(thr);
-    const float succ = __uint_as_float(b + 1u);
-    a.m = 0.5 * ((double)thr + (double)succ);
-    a.odd = (b & 1u) != 0;
+    a.m = 0.5 * ((double)thr + (double)__uint_as_float(__float_as_uint(thr) + 1u));
     return a;
 }
 
-// RN(sum / D) > thr for sum >= 0, thr > 0 finite (or never when thr = inf)
+// RN(sum / D) > thr for sum >= 0, thr >= 0.02 finite (or never when thr = inf)
 __device__ __forceinline__ bool abort_test(float sum, double D, const AbortThr& a) {
-    const double x = (double)sum, r = a.m * D;
-    return !a.never && (x > r || (x == r && a.odd));
+    return (double)sum > a.m * D;
 }
 
 template <bool PIN>
@@ -742,35 +744,37 @@ __device__ __forceinline__ void box_codes(const u64* s_cen, const float2* s_ray,
 }
 
 // Source-0 terms of hypotheses d[0 .. nb) into s_term[b * NP ..]: every
-// projection first, then every gather, then the popcounts.
+// projection first, then every gather, then the popcounts.  The nb * NP
+// (hypothesis, tap) items are dealt to the threads as one list, so an 8x8
+// tile's 2 x 144 items take 5 rounds of 64 rather than 2 x 3.
 template <int NP, int NT, bool PIN, bool FD>
 __device__ __forceinline__ void box_codes_batch(const u64* s_cen, const float2* s_ray, const float* d, int nb,
                                                 const SrcP& P, const float* tab, float* s_term) {
-    constexpr int K = (NP + NT - 1) / NT;
-    unsigned off[INIT_NB][K];
-    bool ok[INIT_NB][K];
+    constexpr int K = (INIT_NB * NP + NT - 1) / NT;
+    const int n = nb * NP;
+    unsigned off[K];
+    bool ok[K];
 #pragma unroll
-    for (int b = 0; b < INIT_NB; ++b)
+    for (int k = 0; k < K; ++k) {
+        const int j = (int)threadIdx.x + k * NT;
+        const int b = j / NP, i = j - b * NP;
+        float db = d[0];   // d[b] as selects (no indexed register array)
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i = (int)threadIdx.x + k * NT;
-            ok[b][k] = false;
-            off[b][k] = 0;
-            if (b < nb && i < NP) ok[b][k] = tap_v4<true, PIN, FD>(s_ray[i], 0.0f, 0.0f, 1.0f, d[b], P, off[b][k]);
-        }
-    u64 cs[INIT_NB][K];
+        for (int t = 1; t < INIT_NB; ++t)
+            if (b == t) db = d[t];
+        ok[k] = false;
+        off[k] = 0;
+        if (j < n) ok[k] = tap_v4<true, PIN, FD>(s_ray[i], 0.0f, 0.0f, 1.0f, db, P, off[k]);
+    }
+    u64 cs[K];
 #pragma unroll
-    for (int b = 0; b < INIT_NB; ++b)
+    for (int k = 0; k < K; ++k) cs[k] = ok[k] ? cen_at(P, off[k]) : 0ull;
 #pragma unroll
-        for (int k = 0; k < K; ++k) cs[b][k] = ok[b][k] ? cen_at(P, off[b][k]) : 0ull;
-#pragma unroll
-    for (int b = 0; b < INIT_NB; ++b)
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i = (int)threadIdx.x + k * NT;
-            if (b < nb && i < NP)
-                s_term[b * NP + i] = tab[ok[b][k] ? (unsigned)__popcll(s_cen[i] ^ cs[b][k]) : 48u];
-        }
+    for (int k = 0; k < K; ++k) {
+        const int j = (int)threadIdx.x + k * NT;
+        const int i = j - (j / NP) * NP;
+        if (j < n) s_term[j] = tab[ok[k] ? (unsigned)__popcll(s_cen[i] ^ cs[k]) : 48u];
+    }
 }
 
 // s_term: room for (INIT_NB + 1) * (TW + 4) * (TH + 4) terms.
@@ -817,15 +821,17 @@ __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* _
             const AbortThr A = make_abort_thr(thr);
             float sum = 0.0f;
             bool live = mine, aborted = false;
-            // the 25 terms of source si from `codes`, row tests as in eval_v4_body
+            // the 25 terms of source si from `terms`, row tests as in
+            // eval_v4_body: row r's divisor 5 (r + 1) (si + 1), capped at the
+            // next source's first divisor 5 (si + 2), selected as integers
             auto add_source = [&](const float* terms, int si) {
-                const double fs = (double)(si + 1) * 5.0;
-                const double cap = si + 1 < nsrc ? (double)(si + 2) * 5.0 : 1e30;
+                const bool capped = si + 1 < nsrc;
 #pragma unroll
                 for (int r = 0; r < 5; ++r) {
 #pragma unroll
                     for (int q = 0; q < 5; ++q) sum += terms[c + (r - 2) * PW + (q - 2)];
-                    if (abort_test(sum, fmin((double)(r + 1) * fs, cap), A)) {
+                    const int dv = (r == 0 || !capped) ? (r + 1) * (si + 1) : si + 2;
+                    if (abort_test(sum, (double)(5 * dv), A)) {
                         aborted = true;
                         live = false;
                         break;
