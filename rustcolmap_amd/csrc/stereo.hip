@@ -965,12 +965,40 @@ __device__ __forceinline__ void pass_pixel(const JobDesc& J, const SrcDesc* __re
     // plane (stereo.rs:324-353).  Candidate geometry never depends on
     // costs, so one evaluation site serves all seven; each candidate's
     // hint is the best cost found before it.
+    //
+    // A neighbour candidate whose (n, d_plane) is bit for bit that of the
+    // first neighbour evaluated here cannot change the result, so it is
+    // counted but not evaluated: plane_cost is a function of (n, d_plane)
+    // and the hint; the best cost only falls, so the repeat's threshold is
+    // no higher and it aborts wherever the first did (1e9 > any stored cost,
+    // which is at most 1), or returns the first's cost C, and best <= C
+    // already holds (C was accepted, or was not below a best that has not
+    // risen).  No draws are involved.  Neighbours mostly repeat one another:
+    // the planes of a region are copies of one plane, and a fronto-parallel
+    // plane carries d_plane = z * 1 exactly.
+    uint32_t f0 = 0, f1 = 0, f2 = 0, f3 = 0;
+    bool have_first = false;
     const float range = J.inv_max - J.inv_min;
     for (int q = 0; q < 7; ++q) {
         float cnx, cny, cnz, d_plane;
         if (!make_candidate<PW>(q, x, y, w, h, c, prev, s_ray, ri, cur, seed, sig_inv, sig_n, range, zlo, zhi, cnx,
                                 cny, cnz, d_plane))
             continue;
+        if (q < 4) {
+            const uint32_t u0 = __float_as_uint(cnx), u1 = __float_as_uint(cny), u2 = __float_as_uint(cnz),
+                           u3 = __float_as_uint(d_plane);
+            if (have_first && u0 == f0 && u1 == f1 && u2 == f2 && u3 == f3) {
+                ++ne;
+                continue;
+            }
+            if (!have_first) {
+                f0 = u0;
+                f1 = u1;
+                f2 = u2;
+                f3 = u3;
+                have_first = true;
+            }
+        }
         const float thr = (best_c + 0.02f) + 0.12f;
         int nr = 5 * nsrc;
         const float cst = EV >= 3 ? eval_plane_v4<PW, false, EV == 3>(s_cen, s_ray, c, S, nsrc, cnx, cny, cnz, d_plane,
