@@ -752,6 +752,7 @@ static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_op
             // the projected coordinates above)
             J.pinhole_ok = ok ? (J.zmin >= 1e-6f ? 3 : 1) : 0;
         }
+        J.costs_exact = 1;   // cleared by mvs_debug_run_passes, whose planes and costs are the caller's
         J.tiles_x = (rv.w + TW - 1) / TW;
         J.tiles_y = (rv.h + TH - 1) / TH;
         J.tile_off = tile_off;
@@ -1028,6 +1029,8 @@ int mvs_debug_run_passes(mvs_ctx* c, int32_t ref, const int32_t* srcs, int32_t k
         HCHK(c->tiles.need(P.tiles.size()));
         HCHK(c->counters.need(counter_words(P)));
         P.jobs[0].valid_count = c->counters.p + valid_base(P);
+        // planted planes carry whatever costs the caller chose
+        P.jobs[0].costs_exact = 0;
         HCHK(hipMemcpyAsync(c->jobs.p, P.jobs.data(), sizeof(JobDesc), hipMemcpyHostToDevice, c->stream));
         HCHK(hipMemcpyAsync(c->srcs.p, P.srcs.data(), P.srcs.size() * sizeof(SrcDesc), hipMemcpyHostToDevice,
                             c->stream));

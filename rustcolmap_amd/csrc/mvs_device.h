@@ -40,6 +40,9 @@ struct JobDesc {
     int32_t tiles_x, tiles_y;
     int32_t tile_off;       // first tile of this job in the batch tile space
     int32_t pinhole_ok;     // k == 0 sources may skip the distortion polynomial (finite-r2 bound holds)
+    int32_t costs_exact;    // every stored cost is at most its plane's cost here (states made by the init
+                            // sweep and the passes, not planted ones): the pass may then skip a neighbour
+                            // candidate equal to a fronto-parallel current plane
     u64* valid_count;       // gate: number of valid pixels (accumulated), or NULL
     uint8_t* work;          // per pixel: window rows read in the last pass / 2 (capped at 255), or NULL;
                             // orders the pixels of a tile in the sorted pass kernel, never read for results

@@ -966,18 +966,27 @@ __device__ __forceinline__ void pass_pixel(const JobDesc& J, const SrcDesc* __re
     // costs, so one evaluation site serves all seven; each candidate's
     // hint is the best cost found before it.
     //
-    // A neighbour candidate whose (n, d_plane) is bit for bit that of the
-    // first neighbour evaluated here cannot change the result, so it is
-    // counted but not evaluated: plane_cost is a function of (n, d_plane)
-    // and the hint; the best cost only falls, so the repeat's threshold is
-    // no higher and it aborts wherever the first did (1e9 > any stored cost,
-    // which is at most 1), or returns the first's cost C, and best <= C
-    // already holds (C was accepted, or was not below a best that has not
-    // risen).  No draws are involved.  Neighbours mostly repeat one another:
-    // the planes of a region are copies of one plane, and a fronto-parallel
-    // plane carries d_plane = z * 1 exactly.
-    uint32_t f0 = 0, f1 = 0, f2 = 0, f3 = 0;
-    bool have_first = false;
+    // A neighbour candidate that repeats a plane whose cost here is already
+    // accounted for cannot change the result, so it is counted but not
+    // evaluated.  plane_cost is a function of (n, d_plane) and the hint, and
+    // the best cost only falls: a repeat's threshold is no higher, so it
+    // aborts wherever the earlier evaluation did (1e9 exceeds every stored
+    // cost, which is at most 1) or returns the same cost C, and best <= C
+    // already holds.  The plane compared against is
+    //   * the current plane when it is fronto-parallel, (+0, +0, 1, z) bit
+    //     for bit: such a plane is stored with z = d_plane / 1 = d_plane
+    //     exactly (by the init sweep or an accepted candidate, stereo.rs:
+    //     263-266, 355-366) together with min(C, 1) for its cost C here, so
+    //     best <= cur_cost <= C from the start (J.costs_exact: not for
+    //     planted states, whose costs are arbitrary);
+    //   * otherwise the first neighbour evaluated here (best <= C after it).
+    // No draws are involved.  Neighbours mostly repeat the current plane:
+    // the init's fronto-parallel planes persist wherever no candidate wins,
+    // and a fronto-parallel neighbour carries d_plane = z * 1 exactly.
+    const bool fronto = J.costs_exact && __float_as_uint(cur.x) == 0u && __float_as_uint(cur.y) == 0u &&
+                        __float_as_uint(cur.z) == 0x3f800000u;
+    uint32_t f0 = 0u, f1 = 0u, f2 = 0x3f800000u, f3 = __float_as_uint(cur.w);
+    bool have_first = fronto;
     const float range = J.inv_max - J.inv_min;
     for (int q = 0; q < 7; ++q) {
         float cnx, cny, cnz, d_plane;
