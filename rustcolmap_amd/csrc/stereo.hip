@@ -880,36 +880,94 @@ __device__ __forceinline__ void init_box_tile(const JobDesc& J, const SrcDesc* _
 // one pass (stereo.rs:279-369)
 // --------------------------------------------------------------------------
 
-// Candidate q of pixel (x, y) (window centre c of the LDS tile of pitch PW)
-// in the reference's order: q = 0..3 the neighbour planes L, R, U, D
-// carried to this pixel (stereo.rs:293-321), q = 4..6 the perturbations of
-// the current plane (stereo.rs:324-353), which draw from `seed`.  Returns
-// false where the reference skips the candidate; the draws made stay
-// consumed (a failed inv_new / zcand test consumes only g0).
-template <int PW>
-__device__ __forceinline__ bool make_candidate(int q, int x, int y, int w, int h, int c, gplane* prev,
-                                               const float2* __restrict__ s_ray, float2 ri, float4 cur, u64& seed,
-                                               float sig_inv, float sig_n, float range, float zlo, float zhi,
-                                               float& cnx, float& cny, float& cnz, float& d_plane) {
-    if (q < 4) {
-        const int dx = q == 0 ? -1 : (q == 1 ? 1 : 0);
-        const int dy = q == 2 ? -1 : (q == 3 ? 1 : 0);
-        const int xx = x + dx, yy = y + dy;
-        if (xx < 0 || yy < 0 || xx >= w || yy >= h) return false;
-        const float4 nb = as_float4(prev[(size_t)yy * w + xx]);
-        const float2 rj = s_ray[c + dy * PW + dx];
-        const float a_nb = (nb.x * rj.x + nb.y * rj.y) + nb.z;
-        if (a_nb < 0.1f) return false;
-        d_plane = nb.w * a_nb;
-        const float a_here = (nb.x * ri.x + nb.y * ri.y) + nb.z;
-        if (a_here < 0.1f) return false;
-        const float zc = d_plane / a_here;
-        if (!(zlo <= zc && zc <= zhi)) return false;
-        cnx = nb.x;
-        cny = nb.y;
-        cnz = nb.z;
-        return true;
+// Neighbour candidate q = 0..3 (L, R, U, D) of pixel (x, y), stereo.rs:
+// 293-311: the neighbour's plane carried to this pixel.  ray_at(dx, dy)
+// gives the ray of the pixel at offset (dx, dy).  Returns false where the
+// reference skips it.
+template <class RayAt>
+__device__ __forceinline__ bool nb_candidate(int q, int x, int y, int w, int h, const gplane* prev, RayAt ray_at,
+                                             float2 ri, float zlo, float zhi, float& cnx, float& cny, float& cnz,
+                                             float& d_plane) {
+    const int dx = q == 0 ? -1 : (q == 1 ? 1 : 0);
+    const int dy = q == 2 ? -1 : (q == 3 ? 1 : 0);
+    const int xx = x + dx, yy = y + dy;
+    if (xx < 0 || yy < 0 || xx >= w || yy >= h) return false;
+    const float4 nb = as_float4(prev[(size_t)yy * w + xx]);
+    const float2 rj = ray_at(dx, dy);
+    const float a_nb = (nb.x * rj.x + nb.y * rj.y) + nb.z;
+    if (a_nb < 0.1f) return false;
+    d_plane = nb.w * a_nb;
+    const float a_here = (nb.x * ri.x + nb.y * ri.y) + nb.z;
+    if (a_here < 0.1f) return false;
+    const float zc = d_plane / a_here;
+    if (!(zlo <= zc && zc <= zhi)) return false;
+    cnx = nb.x;
+    cny = nb.y;
+    cnz = nb.z;
+    return true;
+}
+
+// The neighbour candidates of a pixel, 2 bits each (candidate q at bits
+// 2q): 0 = skipped by the reference, 1 = a repeat (counted, not
+// evaluated), 2 = evaluated.
+//
+// A repeat is a candidate whose plane's cost here is already accounted
+// for; it cannot change the result.  plane_cost is a function of
+// (n, d_plane) and the hint, and the best cost only falls: a repeat's
+// threshold is no higher, so it aborts wherever the earlier evaluation did
+// (1e9 exceeds every stored cost, which is at most 1) or returns the same
+// cost C, and best <= C already holds.  The plane compared against is
+//   * the current plane when it is fronto-parallel, (+0, +0, 1, z) bit for
+//     bit: such a plane is stored with z = d_plane / 1 = d_plane exactly (by
+//     the init sweep or an accepted candidate, stereo.rs:263-266, 355-366)
+//     together with min(C, 1) for its cost C here, so best <= cur_cost <= C
+//     from the start (costs_exact: not for planted states, whose costs are
+//     arbitrary);
+//   * otherwise the first neighbour evaluated here (best <= C after it).
+// No draws are involved.  Neighbours mostly repeat the current plane: the
+// init's fronto-parallel planes persist wherever no candidate wins, and a
+// fronto-parallel neighbour carries d_plane = z * 1 exactly.
+template <class RayAt>
+__device__ __forceinline__ unsigned nb_codes(int x, int y, int w, int h, const gplane* prev, RayAt ray_at, float2 ri,
+                                             float4 cur, float zlo, float zhi, bool costs_exact) {
+    const bool fronto = costs_exact && __float_as_uint(cur.x) == 0u && __float_as_uint(cur.y) == 0u &&
+                        __float_as_uint(cur.z) == 0x3f800000u;
+    uint32_t f0 = 0u, f1 = 0u, f2 = 0x3f800000u, f3 = __float_as_uint(cur.w);
+    bool have_first = fronto;
+    unsigned codes = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float cnx, cny, cnz, d_plane;
+        if (!nb_candidate(q, x, y, w, h, prev, ray_at, ri, zlo, zhi, cnx, cny, cnz, d_plane)) continue;
+        const uint32_t u0 = __float_as_uint(cnx), u1 = __float_as_uint(cny), u2 = __float_as_uint(cnz),
+                       u3 = __float_as_uint(d_plane);
+        if (have_first && u0 == f0 && u1 == f1 && u2 == f2 && u3 == f3) {
+            codes |= 1u << (2 * q);
+            continue;
+        }
+        codes |= 2u << (2 * q);
+        if (!have_first) {
+            f0 = u0;
+            f1 = u1;
+            f2 = u2;
+            f3 = u3;
+            have_first = true;
+        }
     }
+    return codes;
+}
+
+// Candidate q of pixel (x, y) in the reference's order: q = 0..3 the
+// neighbour planes L, R, U, D carried to this pixel (nb_candidate), q = 4..6
+// the perturbations of the current plane (stereo.rs:324-353), which draw
+// from `seed`.  Returns false where the reference skips the candidate; the
+// draws made stay consumed (a failed inv_new / zcand test consumes only g0).
+template <class RayAt>
+__device__ __forceinline__ bool make_candidate(int q, int x, int y, int w, int h, const gplane* prev, RayAt ray_at,
+                                               float2 ri, float4 cur, u64& seed, float sig_inv, float sig_n,
+                                               float range, float zlo, float zhi, float& cnx, float& cny, float& cnz,
+                                               float& d_plane) {
+    if (q < 4) return nb_candidate(q, x, y, w, h, prev, ray_at, ri, zlo, zhi, cnx, cny, cnz, d_plane);
     const float inv = 1.0f / fmaxf(cur.w, 1e-6f);
     const float g0 = gauss3(seed);
     const float inv_new = inv + sig_inv * range * g0;
@@ -937,12 +995,13 @@ __device__ __forceinline__ bool make_candidate(int q, int x, int y, int w, int h
 }
 
 // One pixel of one pass: pixel (x, y) at window centre c of the LDS tile of
-// pitch PW.  With WORK, the window rows its evaluations read, halved and
-// capped at 255, go to J.work (scheduling information only).
+// pitch PW.  `codes` are its neighbour candidates' codes (nb_codes), or -1
+// to have them made here.  With WORK, the window rows its evaluations read,
+// halved and capped at 255, go to J.work (scheduling information only).
 template <int PW, int EV, bool WORK = false>
 __device__ __forceinline__ void pass_pixel(const JobDesc& J, const SrcDesc* __restrict__ S, int x, int y, int c,
                                           int par, u64 salt, float sig_inv, float sig_n, u64* s_cen, float2* s_ray,
-                                          unsigned& ne, const float* s_tab) {
+                                          unsigned& ne, const float* s_tab, int codes_in = -1) {
     const int w = J.w, h = J.h;
     const LdsRef R{s_cen, s_ray, c, PW};
     const int nsrc = J.nsrc;
@@ -959,55 +1018,32 @@ __device__ __forceinline__ void pass_pixel(const JobDesc& J, const SrcDesc* __re
     float bd = cur.w * ((b0 * ri.x + b1 * ri.y) + b2);
     const float zlo = J.zmin * 0.5f, zhi = J.zmax * 2.0f;
     int rows = 0;
+    auto ray_at = [&](int dx, int dy) { return s_ray[c + dy * PW + dx]; };
+    const unsigned codes =
+        codes_in >= 0 ? (unsigned)codes_in : nb_codes(x, y, w, h, prev, ray_at, ri, cur, zlo, zhi, J.costs_exact != 0);
 
     // Seven candidates in the reference's order: the 4 neighbour planes
     // L, R, U, D (stereo.rs:293-321), then 3 perturbations of the current
     // plane (stereo.rs:324-353).  Candidate geometry never depends on
     // costs, so one evaluation site serves all seven; each candidate's
-    // hint is the best cost found before it.
-    //
-    // A neighbour candidate that repeats a plane whose cost here is already
-    // accounted for cannot change the result, so it is counted but not
-    // evaluated.  plane_cost is a function of (n, d_plane) and the hint, and
-    // the best cost only falls: a repeat's threshold is no higher, so it
-    // aborts wherever the earlier evaluation did (1e9 exceeds every stored
-    // cost, which is at most 1) or returns the same cost C, and best <= C
-    // already holds.  The plane compared against is
-    //   * the current plane when it is fronto-parallel, (+0, +0, 1, z) bit
-    //     for bit: such a plane is stored with z = d_plane / 1 = d_plane
-    //     exactly (by the init sweep or an accepted candidate, stereo.rs:
-    //     263-266, 355-366) together with min(C, 1) for its cost C here, so
-    //     best <= cur_cost <= C from the start (J.costs_exact: not for
-    //     planted states, whose costs are arbitrary);
-    //   * otherwise the first neighbour evaluated here (best <= C after it).
-    // No draws are involved.  Neighbours mostly repeat the current plane:
-    // the init's fronto-parallel planes persist wherever no candidate wins,
-    // and a fronto-parallel neighbour carries d_plane = z * 1 exactly.
-    const bool fronto = J.costs_exact && __float_as_uint(cur.x) == 0u && __float_as_uint(cur.y) == 0u &&
-                        __float_as_uint(cur.z) == 0x3f800000u;
-    uint32_t f0 = 0u, f1 = 0u, f2 = 0x3f800000u, f3 = __float_as_uint(cur.w);
-    bool have_first = fronto;
+    // hint is the best cost found before it.  Repeated neighbour planes are
+    // counted without an evaluation (nb_codes), and each lane walks only its
+    // pending candidates (the neighbours it evaluates, then the three
+    // perturbations) in order: a round's lanes stay in lock step over the
+    // i-th pending candidate of each, so a neighbour evaluation of one pixel
+    // costs the round one step, not one per neighbour direction.
+    ne += (unsigned)__popc(codes & 0x55u);   // repeats (code 1)
+    unsigned pend = 0x70u;                   // q = 4, 5, 6
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pend |= ((codes >> (2 * q + 1)) & 1u) << q;   // code 2
     const float range = J.inv_max - J.inv_min;
-    for (int q = 0; q < 7; ++q) {
+    while (pend) {
+        const int q = __builtin_ctz(pend);
+        pend &= pend - 1u;
         float cnx, cny, cnz, d_plane;
-        if (!make_candidate<PW>(q, x, y, w, h, c, prev, s_ray, ri, cur, seed, sig_inv, sig_n, range, zlo, zhi, cnx,
-                                cny, cnz, d_plane))
+        if (!make_candidate(q, x, y, w, h, prev, ray_at, ri, cur, seed, sig_inv, sig_n, range, zlo, zhi, cnx, cny,
+                            cnz, d_plane))
             continue;
-        if (q < 4) {
-            const uint32_t u0 = __float_as_uint(cnx), u1 = __float_as_uint(cny), u2 = __float_as_uint(cnz),
-                           u3 = __float_as_uint(d_plane);
-            if (have_first && u0 == f0 && u1 == f1 && u2 == f2 && u3 == f3) {
-                ++ne;
-                continue;
-            }
-            if (!have_first) {
-                f0 = u0;
-                f1 = u1;
-                f2 = u2;
-                f3 = u3;
-                have_first = true;
-            }
-        }
         const float thr = (best_c + 0.02f) + 0.12f;
         int nr = 5 * nsrc;
         const float cst = EV >= 3 ? eval_plane_v4<PW, false, EV == 3>(s_cen, s_ray, c, S, nsrc, cnx, cny, cnz, d_plane,
