@@ -436,15 +436,28 @@ typedef struct {
     int variant;       /* 0 = the reference; >0 = a planted mistake (orc_run_passes) */
 } est_ctx;
 
-/* Planted mistakes of orc_run_passes, used only by the tests that show the
- * pass pins reject them. */
+/* Variants of orc_run_passes, tests only: the planted mistakes (1-4) that
+ * the pass pins reject, and the device's skipping of repeated neighbour
+ * planes (5, 6), which must change no plane.  A neighbour candidate whose
+ * (n, d_plane) bits equal those of the current plane, when that plane is
+ * (+0, +0, 1, z), or else of the first neighbour evaluated at the pixel, is
+ * not evaluated; 5 counts it as an evaluation (as the device does), 6 does
+ * not (so the tests can see how many were skipped). */
 enum {
     ORC_PASS_REFERENCE = 0,
     ORC_PASS_IN_PLACE = 1,        /* neighbours and cur read from the buffer being written */
     ORC_PASS_DRAW_ALL = 2,        /* all 12 uniforms drawn before the inv_new / zcand test */
     ORC_PASS_PERTURB_BEST = 3,    /* perturbations centred on the best plane so far */
     ORC_PASS_STORE_ZCAND = 4,     /* the accepted candidate's zcand stored instead of bd / a_here */
+    ORC_PASS_SKIP_REPEATS = 5,    /* repeated neighbour planes counted, not evaluated */
+    ORC_PASS_SKIP_UNCOUNTED = 6,  /* the same, not counted */
 };
+
+static uint32_t f32_bits(float v) {
+    uint32_t u;
+    memcpy(&u, &v, 4);
+    return u;
+}
 
 static void init_row(void *c, int item) {
     est_ctx *e = (est_ctx *)c;
@@ -486,6 +499,10 @@ static void pass_row(void *c, int item) {
         float bn0 = cur.nx, bn1 = cur.ny, bn2 = cur.nz;
         float bd = cur.z * ((bn0 * rx[i] + bn1 * ry[i]) + bn2);
         float bz = cur.z;   /* zcand of the best candidate (planted mistake 4 only) */
+        /* repeat skipping (variants 5, 6): the plane a repeat is compared with */
+        const int skip = var == ORC_PASS_SKIP_REPEATS || var == ORC_PASS_SKIP_UNCOUNTED;
+        uint32_t f0 = 0u, f1 = 0u, f2 = f32_bits(1.0f), f3 = f32_bits(cur.z);
+        int have = f32_bits(cur.nx) == 0u && f32_bits(cur.ny) == 0u && f32_bits(cur.nz) == f32_bits(1.0f);
         for (int q = 0; q < 4; q++) {
             const int xx = x + NB[q][0], yy = y + NB[q][1];
             if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
@@ -498,6 +515,14 @@ static void pass_row(void *c, int item) {
             if (a_here < 0.1f) continue;
             const float zcand = d_plane / a_here;
             if (!(zlo <= zcand && zcand <= zhi)) continue;
+            if (skip) {
+                const uint32_t u0 = f32_bits(nb.nx), u1 = f32_bits(nb.ny), u2 = f32_bits(nb.nz), u3 = f32_bits(d_plane);
+                if (have && u0 == f0 && u1 == f1 && u2 == f2 && u3 == f3) {
+                    if (var == ORC_PASS_SKIP_REPEATS) ne++;
+                    continue;
+                }
+                if (!have) { f0 = u0; f1 = u1; f2 = u2; f3 = u3; have = 1; }
+            }
             const float cst = orc_plane_cost(ws, x, y, nb.nx, nb.ny, nb.nz, d_plane, best_c + 0.02f, NULL);
             ne++;
             if (cst < best_c) { best_c = cst; bn0 = nb.nx; bn1 = nb.ny; bn2 = nb.nz; bd = d_plane; bz = zcand; }
