@@ -434,15 +434,18 @@ typedef struct {
     int64_t n_eval;
     int y_lo;          /* first row of the computed band */
     int variant;       /* 0 = the reference; >0 = a planted mistake (orc_run_passes) */
+    float *dpl;        /* variants 5, 6: the d_plane each pixel's plane was costed with */
+    uint8_t *dok;      /* ... where known (fronto-parallel or accepted in this run) */
 } est_ctx;
 
 /* Variants of orc_run_passes, tests only: the planted mistakes (1-4) that
  * the pass pins reject, and the device's skipping of repeated neighbour
  * planes (5, 6), which must change no plane.  A neighbour candidate whose
- * (n, d_plane) bits equal those of the current plane, when that plane is
- * (+0, +0, 1, z), or else of the first neighbour evaluated at the pixel, is
- * not evaluated; 5 counts it as an evaluation (as the device does), 6 does
- * not (so the tests can see how many were skipped). */
+ * (n, d_plane) bits equal those of the current plane with the d_plane it was
+ * costed with (z for (+0, +0, 1, z); for other planes known only once
+ * accepted in this run), or of the first neighbour evaluated at the pixel,
+ * is not evaluated; 5 counts it as an evaluation (as the device does), 6
+ * does not (so the tests can see how many were skipped). */
 enum {
     ORC_PASS_REFERENCE = 0,
     ORC_PASS_IN_PLACE = 1,        /* neighbours and cur read from the buffer being written */
@@ -499,10 +502,13 @@ static void pass_row(void *c, int item) {
         float bn0 = cur.nx, bn1 = cur.ny, bn2 = cur.nz;
         float bd = cur.z * ((bn0 * rx[i] + bn1 * ry[i]) + bn2);
         float bz = cur.z;   /* zcand of the best candidate (planted mistake 4 only) */
-        /* repeat skipping (variants 5, 6): the plane a repeat is compared with */
+        /* repeat skipping (variants 5, 6): the planes a repeat is compared with */
         const int skip = var == ORC_PASS_SKIP_REPEATS || var == ORC_PASS_SKIP_UNCOUNTED;
-        uint32_t f0 = 0u, f1 = 0u, f2 = f32_bits(1.0f), f3 = f32_bits(cur.z);
-        int have = f32_bits(cur.nx) == 0u && f32_bits(cur.ny) == 0u && f32_bits(cur.nz) == f32_bits(1.0f);
+        const int cur_known = skip && e->dok[i];
+        const uint32_t c0 = f32_bits(cur.nx), c1 = f32_bits(cur.ny), c2 = f32_bits(cur.nz),
+                       c3 = skip ? f32_bits(e->dpl[i]) : 0u;
+        uint32_t f0 = 0u, f1 = 0u, f2 = 0u, f3 = 0u;
+        int have = 0;
         for (int q = 0; q < 4; q++) {
             const int xx = x + NB[q][0], yy = y + NB[q][1];
             if (xx < 0 || yy < 0 || xx >= w || yy >= h) continue;
@@ -517,7 +523,8 @@ static void pass_row(void *c, int item) {
             if (!(zlo <= zcand && zcand <= zhi)) continue;
             if (skip) {
                 const uint32_t u0 = f32_bits(nb.nx), u1 = f32_bits(nb.ny), u2 = f32_bits(nb.nz), u3 = f32_bits(d_plane);
-                if (have && u0 == f0 && u1 == f1 && u2 == f2 && u3 == f3) {
+                if ((cur_known && u0 == c0 && u1 == c1 && u2 == c2 && u3 == c3) ||
+                    (have && u0 == f0 && u1 == f1 && u2 == f2 && u3 == f3)) {
                     if (var == ORC_PASS_SKIP_REPEATS) ne++;
                     continue;
                 }
@@ -567,6 +574,7 @@ static void pass_row(void *c, int item) {
             if (a_here > 0.05f) {
                 orc_plane p = {bn0, bn1, bn2, var == ORC_PASS_STORE_ZCAND ? bz : bd / a_here, fminf(best_c, 1.0f)};
                 e->next[i] = p;
+                if (skip) { e->dpl[i] = bd; e->dok[i] = 1; }   /* this pixel's own entry */
             }
         }
     }
@@ -678,6 +686,15 @@ ORC_API int orc_run_passes(const orc_view *views, int nviews, int ref, const int
     e.inv_min = 1.0f / ws->zmax;
     e.inv_max = 1.0f / ws->zmin;
     e.variant = variant;
+    if (variant == ORC_PASS_SKIP_REPEATS || variant == ORC_PASS_SKIP_UNCOUNTED) {
+        /* known d_plane: z for the fronto-parallel planes given; none for others */
+        e.dpl = (float *)malloc(n * sizeof(float));
+        e.dok = (uint8_t *)malloc(n);
+        for (size_t i = 0; i < n; i++) {
+            e.dpl[i] = prev[i].z;
+            e.dok[i] = f32_bits(prev[i].nx) == 0u && f32_bits(prev[i].ny) == 0u && f32_bits(prev[i].nz) == f32_bits(1.0f);
+        }
+    }
     for (int pass = pass_lo; pass < pass_hi; pass++) {
         orc_schedule((uint32_t)pass, (uint32_t)n_passes, &e.sig_inv, &e.sig_n);
         memcpy(next, prev, n * sizeof(orc_plane));
@@ -695,6 +712,8 @@ ORC_API int orc_run_passes(const orc_view *views, int nviews, int ref, const int
     }
     free(prev);
     free(next);
+    free(e.dpl);
+    free(e.dok);
     orc_ws_free(ws);
     return 0;
 }

@@ -104,6 +104,7 @@ struct ViewState {
     DBuf<float2> rays;
     // stereo state (allocated when the view is a reference view)
     DBuf<float4> plane[2];
+    DBuf<float> dplane;     // d_plane of the stored plane where it is not fronto-parallel (JobDesc.dplane)
     DBuf<float> cost[2];
     DBuf<float> depth, cost_out, normal;
     DBuf<uint8_t> work;   // per-pixel pass cost of the sorted pass kernel (scheduling only)
@@ -111,7 +112,7 @@ struct ViewState {
     int64_t n_valid = 0;
     void release() {
         gray.release(); rgb.release(); census.release(); rays.release();
-        plane[0].release(); plane[1].release(); cost[0].release(); cost[1].release();
+        plane[0].release(); plane[1].release(); cost[0].release(); cost[1].release(); dplane.release();
         depth.release(); cost_out.release(); normal.release(); work.release();
         has_depth = false;
     }
@@ -703,6 +704,7 @@ static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_op
             HCHK(rv.plane[b].need(np));
             HCHK(rv.cost[b].need(np));
         }
+        HCHK(rv.dplane.need(np));
         HCHK(rv.depth.need(np));
         HCHK(rv.cost_out.need(np));
         HCHK(rv.normal.need(3 * np));
@@ -718,6 +720,7 @@ static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_op
         J.plane[1] = rv.plane[1].p;
         J.cost[0] = rv.cost[0].p;
         J.cost[1] = rv.cost[1].p;
+        J.dplane = rv.dplane.p;
         J.depth_out = rv.depth.p;
         J.cost_out = rv.cost_out.p;
         J.normal_out = rv.normal.p;

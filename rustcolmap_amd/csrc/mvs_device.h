@@ -42,7 +42,10 @@ struct JobDesc {
     int32_t pinhole_ok;     // k == 0 sources may skip the distortion polynomial (finite-r2 bound holds)
     int32_t costs_exact;    // every stored cost is at most its plane's cost here (states made by the init
                             // sweep and the passes, not planted ones): the pass may then skip a neighbour
-                            // candidate equal to a fronto-parallel current plane
+                            // candidate equal to the current plane
+    float* dplane;          // per pixel: the d_plane the stored plane was costed with, written wherever a
+                            // plane other than (+0, +0, 1, z) is stored (for those d_plane = z); one buffer,
+                            // each pixel reading and writing only its own entry
     u64* valid_count;       // gate: number of valid pixels (accumulated), or NULL
     uint8_t* work;          // per pixel: window rows read in the last pass / 2 (capped at 255), or NULL;
                             // orders the pixels of a tile in the sorted pass kernel, never read for results

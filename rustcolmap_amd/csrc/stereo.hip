@@ -916,24 +916,24 @@ __device__ __forceinline__ bool nb_candidate(int q, int x, int y, int w, int h, 
 // (n, d_plane) and the hint, and the best cost only falls: a repeat's
 // threshold is no higher, so it aborts wherever the earlier evaluation did
 // (1e9 exceeds every stored cost, which is at most 1) or returns the same
-// cost C, and best <= C already holds.  The plane compared against is
-//   * the current plane when it is fronto-parallel, (+0, +0, 1, z) bit for
-//     bit: such a plane is stored with z = d_plane / 1 = d_plane exactly (by
-//     the init sweep or an accepted candidate, stereo.rs:263-266, 355-366)
-//     together with min(C, 1) for its cost C here, so best <= cur_cost <= C
-//     from the start (costs_exact: not for planted states, whose costs are
-//     arbitrary);
-//   * otherwise the first neighbour evaluated here (best <= C after it).
+// cost C, and best <= C already holds.  The planes compared against are
+//   * the current plane (n, cur_d), cur_d the d_plane it was costed with:
+//     every plane is stored together with min(C, 1) for its cost C here at
+//     that d_plane (by the init sweep, an accepted candidate or the learned
+//     path's proposal costs; stereo.rs:263-266, 355-366), so best <=
+//     cur_cost <= C from the start.  Only with costs_exact: planted states
+//     carry arbitrary costs;
+//   * the first neighbour evaluated here (best <= C after it).
 // No draws are involved.  Neighbours mostly repeat the current plane: the
 // init's fronto-parallel planes persist wherever no candidate wins, and a
 // fronto-parallel neighbour carries d_plane = z * 1 exactly.
 template <class RayAt>
 __device__ __forceinline__ unsigned nb_codes(int x, int y, int w, int h, const gplane* prev, RayAt ray_at, float2 ri,
-                                             float4 cur, float zlo, float zhi, bool costs_exact) {
-    const bool fronto = costs_exact && __float_as_uint(cur.x) == 0u && __float_as_uint(cur.y) == 0u &&
-                        __float_as_uint(cur.z) == 0x3f800000u;
-    uint32_t f0 = 0u, f1 = 0u, f2 = 0x3f800000u, f3 = __float_as_uint(cur.w);
-    bool have_first = fronto;
+                                             float4 cur, float cur_d, float zlo, float zhi, bool costs_exact) {
+    const uint32_t c0 = __float_as_uint(cur.x), c1 = __float_as_uint(cur.y), c2 = __float_as_uint(cur.z),
+                   c3 = __float_as_uint(cur_d);
+    uint32_t f0 = 0u, f1 = 0u, f2 = 0u, f3 = 0u;
+    bool have_first = false;
     unsigned codes = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -941,7 +941,8 @@ __device__ __forceinline__ unsigned nb_codes(int x, int y, int w, int h, const g
         if (!nb_candidate(q, x, y, w, h, prev, ray_at, ri, zlo, zhi, cnx, cny, cnz, d_plane)) continue;
         const uint32_t u0 = __float_as_uint(cnx), u1 = __float_as_uint(cny), u2 = __float_as_uint(cnz),
                        u3 = __float_as_uint(d_plane);
-        if (have_first && u0 == f0 && u1 == f1 && u2 == f2 && u3 == f3) {
+        if ((costs_exact && u0 == c0 && u1 == c1 && u2 == c2 && u3 == c3) ||
+            (have_first && u0 == f0 && u1 == f1 && u2 == f2 && u3 == f3)) {
             codes |= 1u << (2 * q);
             continue;
         }
@@ -1019,8 +1020,14 @@ __device__ __forceinline__ void pass_pixel(const JobDesc& J, const SrcDesc* __re
     const float zlo = J.zmin * 0.5f, zhi = J.zmax * 2.0f;
     int rows = 0;
     auto ray_at = [&](int dx, int dy) { return s_ray[c + dy * PW + dx]; };
+    // the d_plane the current plane was costed with: z for (+0, +0, 1, z),
+    // else stored beside it (JobDesc.dplane)
+    const bool exact = J.costs_exact != 0;
+    const bool fronto = __float_as_uint(cur.x) == 0u && __float_as_uint(cur.y) == 0u &&
+                        __float_as_uint(cur.z) == 0x3f800000u;
+    const float cur_d = fronto || !exact ? cur.w : J.dplane[i];
     const unsigned codes =
-        codes_in >= 0 ? (unsigned)codes_in : nb_codes(x, y, w, h, prev, ray_at, ri, cur, zlo, zhi, J.costs_exact != 0);
+        codes_in >= 0 ? (unsigned)codes_in : nb_codes(x, y, w, h, prev, ray_at, ri, cur, cur_d, zlo, zhi, exact);
 
     // Seven candidates in the reference's order: the 4 neighbour planes
     // L, R, U, D (stereo.rs:293-321), then 3 perturbations of the current
@@ -1065,6 +1072,7 @@ __device__ __forceinline__ void pass_pixel(const JobDesc& J, const SrcDesc* __re
         if (a_here > 0.05f) {
             outp = make_float4(b0, b1, b2, bd / a_here);
             outc = fminf(best_c, 1.0f);
+            J.dplane[i] = bd;   // this pixel's own entry: no other pixel reads it
         }
     }
     J.plane[par ^ 1][i] = outp;
@@ -1632,14 +1640,17 @@ __global__ __launch_bounds__(256) void proposal_cost_kernel(const JobDesc* __res
         const float2 ri = J.rays[i];
         const float a = (pl.x * ri.x + pl.y * ri.y) + pl.z;
         const GlobRef R{J.census, J.rays, x, y, J.w, J.h};
-        const float cst = fminf(eval_plane<false>(R, srcs + J.src_off, J.nsrc, pl.x, pl.y, pl.z, pl.w * a,
+        const float dp = pl.w * a;
+        const float cst = fminf(eval_plane<false>(R, srcs + J.src_off, J.nsrc, pl.x, pl.y, pl.z, dp,
                                                   __builtin_inff(), nullptr),
                                 1.0f);
         if (!prop) {
             J.cost[0][i] = cst;
+            J.dplane[i] = dp;
         } else if (cst < J.cost[0][i]) {
             J.plane[0][i] = pl;
             J.cost[0][i] = cst;
+            J.dplane[i] = dp;
         }
     }
 }
