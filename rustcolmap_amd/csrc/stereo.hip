@@ -913,17 +913,18 @@ __device__ __forceinline__ bool nb_candidate(int q, int x, int y, int w, int h, 
 //
 // A repeat is a candidate whose plane's cost here is already accounted
 // for; it cannot change the result.  plane_cost is a function of
-// (n, d_plane) and the hint, and the best cost only falls: a repeat's
-// threshold is no higher, so it aborts wherever the earlier evaluation did
-// (1e9 exceeds every stored cost, which is at most 1) or returns the same
-// cost C, and best <= C already holds.  The planes compared against are
+// (n, d_plane) and the hint, returning the plane's cost C or 1e9 when the
+// hint aborts it, and the best cost only falls: a repeat's threshold is no
+// higher than the earlier evaluation's, so its result is never below that
+// one's, and best is already at most that one's result.  So the repeat is
+// never accepted.  The planes compared against are
 //   * the current plane (n, cur_d), cur_d the d_plane it was costed with:
 //     every plane is stored together with min(C, 1) for its cost C here at
 //     that d_plane (by the init sweep, an accepted candidate or the learned
 //     path's proposal costs; stereo.rs:263-266, 355-366), so best <=
-//     cur_cost <= C from the start.  Only with costs_exact: planted states
-//     carry arbitrary costs;
-//   * the first neighbour evaluated here (best <= C after it).
+//     cur_cost <= C <= 1 < 1e9 from the start.  Only with costs_exact:
+//     planted states carry arbitrary costs;
+//   * the first neighbour evaluated here (best <= its result after it).
 // No draws are involved.  Neighbours mostly repeat the current plane: the
 // init's fronto-parallel planes persist wherever no candidate wins, and a
 // fronto-parallel neighbour carries d_plane = z * 1 exactly.
