@@ -755,7 +755,13 @@ static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_op
             // the projected coordinates above)
             J.pinhole_ok = ok ? (J.zmin >= 1e-6f ? 3 : 1) : 0;
         }
-        J.costs_exact = 1;   // cleared by mvs_debug_run_passes, whose planes and costs are the caller's
+        // cleared by mvs_debug_run_passes, whose planes and costs are the
+        // caller's; MVS_COSTS_EXACT=0 clears it everywhere (tests: the repeat
+        // rule that relies on it must change no result)
+        {
+            const char* env = getenv("MVS_COSTS_EXACT");
+            J.costs_exact = (env && env[0] == '0') ? 0 : 1;
+        }
         J.tiles_x = (rv.w + TW - 1) / TW;
         J.tiles_y = (rv.h + TH - 1) / TH;
         J.tile_off = tile_off;
@@ -786,6 +792,7 @@ static int plan_batch(mvs_ctx* c, const mvs_job_t* jobs, int njobs, const mvs_op
     P.sig.push_back((uint64_t)(P.cfg.tw * 10000 + P.cfg.th * 100 + P.cfg.persistent * 10 + P.cfg.eval));
     P.grid_p = std::min((int)P.tiles.size(), c->n_cu * persistent_blocks_per_cu(P.cfg));
     P.sig.push_back((uint64_t)P.grid_p);
+    P.sig.push_back(P.jobs.empty() ? 0u : (uint64_t)P.jobs[0].costs_exact);
     P.sig.push_back((uint64_t)(uintptr_t)c->jobs.p);
     return MVS_OK;
 }
