@@ -997,13 +997,12 @@ __device__ __forceinline__ bool make_candidate(int q, int x, int y, int w, int h
 }
 
 // One pixel of one pass: pixel (x, y) at window centre c of the LDS tile of
-// pitch PW.  `codes` are its neighbour candidates' codes (nb_codes), or -1
-// to have them made here.  With WORK, the window rows its evaluations read,
-// halved and capped at 255, go to J.work (scheduling information only).
+// pitch PW.  With WORK, the window rows its evaluations read, halved and
+// capped at 255, go to J.work (scheduling information only).
 template <int PW, int EV, bool WORK = false>
 __device__ __forceinline__ void pass_pixel(const JobDesc& J, const SrcDesc* __restrict__ S, int x, int y, int c,
                                           int par, u64 salt, float sig_inv, float sig_n, u64* s_cen, float2* s_ray,
-                                          unsigned& ne, const float* s_tab, int codes_in = -1) {
+                                          unsigned& ne, const float* s_tab) {
     const int w = J.w, h = J.h;
     const LdsRef R{s_cen, s_ray, c, PW};
     const int nsrc = J.nsrc;
@@ -1027,8 +1026,7 @@ __device__ __forceinline__ void pass_pixel(const JobDesc& J, const SrcDesc* __re
     const bool fronto = __float_as_uint(cur.x) == 0u && __float_as_uint(cur.y) == 0u &&
                         __float_as_uint(cur.z) == 0x3f800000u;
     const float cur_d = fronto || !exact ? cur.w : J.dplane[i];
-    const unsigned codes =
-        codes_in >= 0 ? (unsigned)codes_in : nb_codes(x, y, w, h, prev, ray_at, ri, cur, cur_d, zlo, zhi, exact);
+    const unsigned codes = nb_codes(x, y, w, h, prev, ray_at, ri, cur, cur_d, zlo, zhi, exact);
 
     // Seven candidates in the reference's order: the 4 neighbour planes
     // L, R, U, D (stereo.rs:293-321), then 3 perturbations of the current
