@@ -429,14 +429,49 @@ __device__ __forceinline__ void two_min_dpp(int& u1, int& u2) {
     u2 = min(a, b);
 }
 
-__device__ __forceinline__ void two_min_row16(float& u1, float& u2) {
-    int a = __float_as_int(u1), b = __float_as_int(u2);
-    two_min_dpp<0xB1>(a, b);
-    two_min_dpp<0x4E>(a, b);
-    two_min_dpp<0x141>(a, b);
-    two_min_dpp<0x140>(a, b);
-    u1 = __int_as_float(a);
-    u2 = __int_as_float(b);
+// One step of a transposed two-min tree over the 16 lanes of a row: the
+// lane holds the pairs of 2H rows (bit patterns, as in two_min_dpp), keeps
+// rows [0, H) or [H, 2H) (hi), sends the other half to its partner under
+// DPP pattern CTRL (which has the opposite hi and the same kept rows) and
+// merges the partner's half into its own.  The kept pairs end in a, b [0, H).
+template <int CTRL, int H>
+__device__ __forceinline__ void two_min_tstep(int* a, int* b, bool hi) {
+#pragma unroll
+    for (int t = 0; t < H; ++t) {
+        const int ka = hi ? a[t + H] : a[t], kb = hi ? b[t + H] : b[t];
+        const int sa = hi ? a[t] : a[t + H], sb = hi ? b[t] : b[t + H];
+        const int pa = dpp_i<CTRL>(sa), pb = dpp_i<CTRL>(sb);
+        a[t] = min(ka, pa);
+        b[t] = min(max(ka, pa), min(kb, pb));
+    }
+}
+
+// The 16 pairs a lane holds (row j = 0..15 of its 16-lane row group), each
+// merged over the 16 lanes of the group: row_mirror pairs lane r with
+// 15 - r, row_half_mirror then r with r ^ 7, the quad permutes with r ^ 2
+// and r ^ 1, so lane r ends with row j = r, merged over all 16 lanes, in
+// 15 merges instead of 16 x 4.
+__device__ __forceinline__ void two_min_rows16(int* a, int* b, int r16) {
+    two_min_tstep<0x140, 8>(a, b, (r16 & 8) != 0);
+    two_min_tstep<0x141, 4>(a, b, (r16 & 4) != 0);
+    two_min_tstep<0x4E, 2>(a, b, (r16 & 2) != 0);
+    two_min_tstep<0xB1, 1>(a, b, (r16 & 1) != 0);
+}
+
+// two_min_merge of lane l's pair with lane (l ^ 16)'s (SPAN 16) or
+// (l ^ 32)'s (SPAN 32): a permlane swap of a register with a copy of itself
+// leaves each lane holding its own value and its partner's, in some order,
+// and the merge is symmetric.
+template <int SPAN>
+__device__ __forceinline__ void two_min_xlane(float& u1, float& u2) {
+    const int a = __float_as_int(u1), b = __float_as_int(u2);
+    const auto sa = SPAN == 16 ? __builtin_amdgcn_permlane16_swap(a, a, false, false)
+                               : __builtin_amdgcn_permlane32_swap(a, a, false, false);
+    const auto sb = SPAN == 16 ? __builtin_amdgcn_permlane16_swap(b, b, false, false)
+                               : __builtin_amdgcn_permlane32_swap(b, b, false, false);
+    u1 = __int_as_float(sa[0]);
+    u2 = __int_as_float(sb[0]);
+    two_min_merge(u1, u2, __int_as_float(sa[1]), __int_as_float(sb[1]));
 }
 
 constexpr int LROW = 40;   // LDS row stride (u16): 32 components + 8 pad
@@ -449,8 +484,13 @@ constexpr int LROW = 40;   // LDS row stride (u16): 32 components + 8 pad
 // PASS 3: (check) max over the tile of |exact - estimate| / (|a|^2 + |b|^2)
 //         into *fb.ovf_n as float bits.
 // 4 waves in a 2 x 2 grid, each 64 x 64 = 4 x 4 MFMA 16x16x32 fragments.
+// Three workgroups per CU (42 KB of LDS each) need <= 168 registers per
+// lane: the compiler then keeps about 14 values of the set-up and the
+// epilogue in scratch (none in the product loop), which costs less than
+// the third wave per SIMD gains (1.11 -> 0.99 ms per batch of 190 pairs of
+// 2048 descriptors).
 template <int PASS>
-__global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict__ desc,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void filt_tile_kernel(const float* __restrict__ desc,
                                                         const float* __restrict__ norm,
                                                         const uint16_t* __restrict__ hi,
                                                         const uint16_t* __restrict__ lo,
@@ -596,17 +636,23 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
                     two_min(ru1[mf][i], ru2[mf][i], U);
                     two_min(cu1[nf], cu2[nf], U);
                 }
-        // rows: over the 16 lanes of a kq group, then the two column waves
+        // rows: over the 16 lanes of a kq group (lane r16 ends with row
+        // mf = r16 / 4, i = r16 % 4), then the two column waves
+        int ra[16], rb[16];
 #pragma unroll
         for (int mf = 0; mf < 4; ++mf)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) two_min_row16(ru1[mf][i], ru2[mf][i]);
+            for (int i = 0; i < 4; ++i) {
+                ra[mf * 4 + i] = __float_as_int(ru1[mf][i]);
+                rb[mf * 4 + i] = __float_as_int(ru2[mf][i]);
+            }
+        two_min_rows16(ra, rb, r16);
         // columns: over the 4 kq groups, then the two row waves
 #pragma unroll
-        for (int nf = 0; nf < 4; ++nf)
-#pragma unroll
-            for (int m = 16; m < 64; m <<= 1)
-                two_min_merge(cu1[nf], cu2[nf], __shfl_xor(cu1[nf], m), __shfl_xor(cu2[nf], m));
+        for (int nf = 0; nf < 4; ++nf) {
+            two_min_xlane<16>(cu1[nf], cu2[nf]);
+            two_min_xlane<32>(cu1[nf], cu2[nf]);
+        }
         __syncthreads();   // the staging buffers are reused below
         float2* srow = reinterpret_cast<float2*>(sAh);               // [2 column waves][128 rows]
         float2* scol = reinterpret_cast<float2*>(sAh) + 2 * FTILE;   // [2 row waves][128 columns]
@@ -615,15 +661,8 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
         uint32_t* srm = reinterpret_cast<uint32_t*>(sBh);            // [128 rows][4 words]: columns with L <= u2
         uint32_t* scm = srm + 4 * FTILE;                             // [128 columns][4 words]: rows
         uint32_t* snmax = reinterpret_cast<uint32_t*>(sBl);          // [2]: max scaled norm of rows, columns
-        if (r16 == 0) {
-#pragma unroll
-            for (int mf = 0; mf < 4; ++mf)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int R = wr * 64 + mf * 16 + kq * 4 + i;
-                    srow[wc * FTILE + R] = make_float2(ru1[mf][i], ru2[mf][i]);
-                }
-        }
+        srow[wc * FTILE + wr * 64 + (r16 >> 2) * 16 + kq * 4 + (r16 & 3)] =
+            make_float2(__int_as_float(ra[0]), __int_as_float(rb[0]));
         if (kq == 0) {
 #pragma unroll
             for (int nf = 0; nf < 4; ++nf) {
@@ -709,8 +748,11 @@ __global__ __launch_bounds__(256) void filt_tile_kernel(const float* __restrict_
         for (int nf = 0; nf < 4; ++nf) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                cw[nf][h] |= (uint32_t)__shfl_xor((int)cw[nf][h], 16);
-                cw[nf][h] |= (uint32_t)__shfl_xor((int)cw[nf][h], 32);
+                const int v = (int)cw[nf][h];
+                const auto s16 = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+                const int v2 = s16[0] | s16[1];
+                const auto s32 = __builtin_amdgcn_permlane32_swap(v2, v2, false, false);
+                cw[nf][h] = (uint32_t)(s32[0] | s32[1]);
             }
             if (kq == 0) {
                 const int C = wc * 64 + nf * 16 + r16;
