@@ -511,13 +511,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     const int i0 = rt * FTILE, j0 = ct * FTILE;
     const int nra = min(FTILE, P.na - i0), nrb = min(FTILE, P.nb - j0);
     // squared norms of the tile's rows (threads 0..127) and columns (128..255),
-    // loaded now and put in LDS after the products (0 past a set)
+    // pre-scaled by 1 + EPS_REL in pass 1; rows / columns past a set hold +inf
+    // in pass 1, 0 in pass 3.  Written to LDS before the products (the
+    // product loop's first barrier orders them), so that no register holds
+    // them across the loop.
     __shared__ float s_norm[2 * FTILE];
-    float my_norm = 0.0f;
     {
         const int e = threadIdx.x & (FTILE - 1);
         const bool is_row = threadIdx.x < FTILE;
-        if (e < (is_row ? nra : nrb)) my_norm = norm[is_row ? P.a + i0 + e : P.b + j0 + e];
+        float my_norm = PASS == 1 ? INFINITY : 0.0f;
+        if (e < (is_row ? nra : nrb))
+            my_norm = (PASS == 1 ? 1.0f + EPS_REL : 1.0f) * norm[is_row ? P.a + i0 + e : P.b + j0 + e];
+        s_norm[threadIdx.x] = my_norm;
     }
 
     __shared__ __align__(16) uint16_t sAh[FTILE * LROW], sAl[FTILE * LROW], sBh[FTILE * LROW], sBl[FTILE * LROW];
@@ -600,23 +605,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     // |b|^2), U = tU - 2G, L = U - k tU with k = 2 EPS / (1 + EPS).  Rows /
     // columns past a set get +inf: U = +inf never enters the two smallest and
     // L = NaN fails every compare.
-    const float scale = PASS == 1 ? 1.0f + EPS_REL : 1.0f;
-    const float pad = PASS == 1 ? INFINITY : 0.0f;
-    s_norm[threadIdx.x] = my_norm;
-    __syncthreads();
     float nrow[4][4], ncol[4];
 #pragma unroll
     for (int mf = 0; mf < 4; ++mf)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int R = wr * 64 + mf * 16 + kq * 4 + i;
-            nrow[mf][i] = R < nra ? scale * s_norm[R] : pad;
-        }
+        for (int i = 0; i < 4; ++i) nrow[mf][i] = s_norm[wr * 64 + mf * 16 + kq * 4 + i];
 #pragma unroll
-    for (int nf = 0; nf < 4; ++nf) {
-        const int C = wc * 64 + nf * 16 + r16;
-        ncol[nf] = C < nrb ? scale * s_norm[FTILE + C] : pad;
-    }
+    for (int nf = 0; nf < 4; ++nf) ncol[nf] = s_norm[FTILE + wc * 64 + nf * 16 + r16];
 
     if constexpr (PASS == 1) {
         float ru1[4][4], ru2[4][4], cu1[4], cu2[4];
@@ -682,7 +677,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
             // scaled norms are >= 0, so their bit patterns order as the values;
             // a wave holds 64 rows or 64 columns: its maximum, then one atomic
             // per wave (256 atomics on two words serialised in the LDS)
-            unsigned nm = e < (is_row ? nra : nrb) ? __float_as_uint((1.0f + EPS_REL) * s_norm[q]) : 0u;
+            unsigned nm = e < (is_row ? nra : nrb) ? __float_as_uint(s_norm[q]) : 0u;
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) nm = max(nm, (unsigned)__shfl_xor((int)nm, off));
             if ((threadIdx.x & 63) == 0) atomicMax(&snmax[is_row ? 0 : 1], nm);
@@ -690,32 +685,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
         __syncthreads();
         // Second sweep over the accumulators: bit masks of the elements with
         // L <= the tile's u2, per row (columns) and per column (rows).  The
-        // empty asm makes the norms and accumulators opaque, so tU and U are
-        // recomputed from the accumulator registers instead of being kept
-        // live (64 more registers) across the reductions.
-#pragma unroll
-        for (int mf = 0; mf < 4; ++mf)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) asm volatile("" : "+v"(nrow[mf][i]));
-#pragma unroll
-        for (int nf = 0; nf < 4; ++nf) asm volatile("" : "+v"(ncol[nf]));
+        // empty asm makes the accumulators opaque, so tU and U are recomputed
+        // from the accumulator registers instead of being kept live (64 more
+        // registers) across the reductions; the norms are read again from
+        // LDS (the same values) rather than held across them, and the row /
+        // column bases are made opaque too, so that the row indices are
+        // recomputed here rather than kept from the first sweep.
 #pragma unroll
         for (int mf = 0; mf < 4; ++mf)
 #pragma unroll
             for (int nf = 0; nf < 4; ++nf) asm volatile("" : "+a"(acc[mf][nf]));
+        int rbase = wr * 64 + kq * 4, cbase = wc * 64 + r16;
+        asm volatile("" : "+v"(rbase), "+v"(cbase));
+#pragma unroll
+        for (int mf = 0; mf < 4; ++mf)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) nrow[mf][i] = s_norm[rbase + mf * 16 + i];
+#pragma unroll
+        for (int nf = 0; nf < 4; ++nf) ncol[nf] = s_norm[FTILE + cbase + nf * 16];
         const float kb = 2.0f * EPS_REL / (1.0f + EPS_REL);
         float tc[4];
         uint32_t cw[4][2];   // per column block: rows wr*64 + (0..31), wr*64 + (32..63)
 #pragma unroll
         for (int nf = 0; nf < 4; ++nf) {
-            tc[nf] = suc[wc * 64 + nf * 16 + r16].y;
+            tc[nf] = suc[cbase + nf * 16].y;
             cw[nf][0] = cw[nf][1] = 0u;
         }
 #pragma unroll
         for (int mf = 0; mf < 4; ++mf)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int R = wr * 64 + mf * 16 + kq * 4 + i;
+                const int R = rbase + mf * 16 + i;
                 const float tr = sur[R].y;
                 // the row test's wave mask holds, for each of the four rows
                 // of this (mf, i) (kq = 0..3), its 16 columns of block nf at
@@ -761,12 +761,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
             }
         }
         __syncthreads();
-        // records: threads 0..127 rows, 128..255 columns
-        const int q = threadIdx.x, e = q & (FTILE - 1);
+        // records: threads 0..127 rows, 128..255 columns (the thread index
+        // made opaque, so that e is recomputed here rather than kept, or
+        // spilled, from the top of the kernel)
+        int q = threadIdx.x;
+        asm volatile("" : "+v"(q));
+        const int e = q & (FTILE - 1);
         const bool is_row = q < FTILE;
         if (e < (is_row ? nra : nrb)) {
             const float2 u = is_row ? sur[e] : suc[e];
-            const float own = (1.0f + EPS_REL) * s_norm[q];
+            const float own = s_norm[q];
             const float other = __uint_as_float(snmax[is_row ? 1 : 0]);
             // every element's L >= u1 - kb * max tU; the factor and the
             // constant cover the rounding of L
