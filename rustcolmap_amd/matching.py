@@ -79,11 +79,16 @@ class FrameMatcher:
 
     def slots(self, pairs: Sequence[Tuple[int, int]]) -> np.ndarray:
         """First output slot of each pair (min(n_i, n_j) slots per pair)."""
+        pr = np.asarray(pairs, np.int64).reshape(-1, 2)
         n = len(self.counts)
+        off = np.zeros(len(pr) + 1, np.int64)
+        if n == 0 or len(pr) == 0:
+            return off
         # out-of-range indices get no slots here; the library rejects them
-        sizes = [min(int(self.counts[i]), int(self.counts[j])) if 0 <= i < n and 0 <= j < n else 0
-                 for i, j in pairs]
-        return np.concatenate([[0], np.cumsum(sizes, dtype=np.int64)])
+        ok = (pr >= 0).all(axis=1) & (pr < n).all(axis=1)
+        c = self.counts.astype(np.int64)[np.where(ok[:, None], pr, 0)]
+        np.cumsum(np.where(ok, c.min(axis=1), 0), out=off[1:])
+        return off
 
     def match(self, pairs: Sequence[Tuple[int, int]], ratio: float = DEFAULT_RATIO,
               download: bool = True) -> List[PairMatches]:
@@ -104,8 +109,15 @@ class FrameMatcher:
         if not download:
             return [PairMatches(np.zeros(int(cnt[p]), np.uint32), np.zeros(int(cnt[p]), np.uint32))
                     for p in range(len(pr))]
-        return [PairMatches(oa[off[p]:off[p] + cnt[p]].copy(), ob[off[p]:off[p] + cnt[p]].copy())
-                for p in range(len(pr))]
+        # one gather of every pair's matches out of its slots into dense
+        # arrays, then a view per pair (no per-pair copies)
+        m = cnt[:len(pr)]
+        doff = np.zeros(len(pr) + 1, np.int64)
+        np.cumsum(m, out=doff[1:])
+        idx = np.repeat(off[:-1] - doff[:-1], m) + np.arange(doff[-1], dtype=np.int64)
+        a, b = oa[idx], ob[idx]
+        d = doff.tolist()
+        return [PairMatches(a[d[p]:d[p + 1]], b[d[p]:d[p + 1]]) for p in range(len(pr))]
 
     def stats(self) -> dict:
         return match_stats(self.ctx)
