@@ -485,10 +485,12 @@ constexpr int LROW = 40;   // LDS row stride (u16): 32 components + 8 pad
 //         into *fb.ovf_n as float bits.
 // 4 waves in a 2 x 2 grid, each 64 x 64 = 4 x 4 MFMA 16x16x32 fragments.
 // Three workgroups per CU (42 KB of LDS each) need <= 168 registers per
-// lane: the compiler then keeps about 14 values of the set-up and the
-// epilogue in scratch (none in the product loop), which costs less than
-// the third wave per SIMD gains (1.11 -> 0.99 ms per batch of 190 pairs of
-// 2048 descriptors).
+// lane: the compiler then keeps two lane coordinates in scratch across the
+// product loop (nothing inside it), which costs far less than the third
+// wave per SIMD gains (1.11 -> 0.89 ms per batch of 190 pairs of 2048
+// descriptors, with the norms written to LDS before the loop).  The budget
+// has no slack: changes to the loop's address arithmetic have made the
+// allocator spill an accumulator inside the loop.
 template <int PASS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void filt_tile_kernel(const float* __restrict__ desc,
                                                         const float* __restrict__ norm,
