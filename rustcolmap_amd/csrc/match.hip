@@ -835,15 +835,27 @@ __global__ __launch_bounds__(256) void filt_collect_kernel(const PairDesc* __res
     const int64_t base = (cols ? P.cpart : P.rpart) + local;
     const TileMeta* meta = (cols ? fb.cmeta : fb.rmeta) + base;
     const uint4* mask = (cols ? fb.cmask : fb.rmask) + base;
+    // the lower bounds of the first 16 tiles (2048 columns) stay in
+    // registers from the first read of their records
+    constexpr int NCACHE = 16;
+    float lbs[NCACHE];
     float u1 = INFINITY, u2 = INFINITY;
-    for (int t = 0; t < nt; ++t) {
+#pragma unroll
+    for (int t = 0; t < NCACHE; ++t) {
+        lbs[t] = NAN;
+        if (t < nt) {
+            const TileMeta m = meta[(int64_t)t * n];
+            two_min_merge(u1, u2, m.u1, m.u2);
+            lbs[t] = m.lb;
+        }
+    }
+    for (int t = NCACHE; t < nt; ++t) {
         const TileMeta m = meta[(int64_t)t * n];
         two_min_merge(u1, u2, m.u1, m.u2);
     }
     uint32_t c = 0;
     uint32_t* cand = (cols ? fb.ccand : fb.rcand) + (size_t)q * CAP;
-    for (int t = 0; t < nt; ++t) {
-        if (!(meta[(int64_t)t * n].lb <= u2)) continue;
+    auto take = [&](int t) {
         const uint4 m4 = mask[(int64_t)t * n];
         const uint32_t w[4] = {m4.x, m4.y, m4.z, m4.w};
 #pragma unroll
@@ -856,7 +868,18 @@ __global__ __launch_bounds__(256) void filt_collect_kernel(const PairDesc* __res
                 ++c;
             }
         }
+    };
+    // tiles in increasing order, as one loop over all of them would take them
+    uint32_t pass = 0;
+#pragma unroll
+    for (int t = 0; t < NCACHE; ++t) pass |= lbs[t] <= u2 ? 1u << t : 0u;   // NaN (no tile) fails
+    while (pass) {
+        const int t = __builtin_ctz(pass);
+        pass &= pass - 1u;
+        take(t);
     }
+    for (int t = NCACHE; t < nt; ++t)
+        if (meta[(int64_t)t * n].lb <= u2) take(t);
     if (c > (uint32_t)CAP) {
         c = CAP + 1;
         fb.ovf[atomicAdd(fb.ovf_n, 1u)] = (cols ? 0x80000000u : 0u) | (uint32_t)q;
