@@ -474,7 +474,11 @@ __device__ __forceinline__ void two_min_xlane(float& u1, float& u2) {
     two_min_merge(u1, u2, __int_as_float(sa[1]), __int_as_float(sb[1]));
 }
 
-constexpr int LROW = 40;   // LDS row stride (u16): 32 components + 8 pad
+// LDS layout of a staged chunk: [128 rows][4 segments of 8 components],
+// 64 bytes a row with no padding; segment s of row R sits in slot
+// s ^ ((R >> 2) & 3), so the 16 lanes that read segment kq of 16 consecutive
+// rows hit 16 distinct 16-byte bank groups (all 64 banks once).
+__device__ __forceinline__ int lds_slot(int row, int seg) { return row * 32 + ((seg ^ ((row >> 2) & 3)) << 3); }
 
 // PASS 1: per row and per column of a 128 x 128 tile, the two smallest
 //         upper bounds U (u1 <= u2), a lower bound of the tile's L, and a
@@ -527,7 +531,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
         s_norm[threadIdx.x] = my_norm;
     }
 
-    __shared__ __align__(16) uint16_t sAh[FTILE * LROW], sAl[FTILE * LROW], sBh[FTILE * LROW], sBl[FTILE * LROW];
+    __shared__ __align__(16) uint16_t sAh[FTILE * 32], sAl[FTILE * 32], sBh[FTILE * 32], sBl[FTILE * 32];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
     const int r16 = lane & 15, kq = lane >> 4;
     f32x4 acc[4][4];
@@ -536,69 +540,65 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 #pragma unroll
         for (int nf = 0; nf < 4; ++nf) acc[mf][nf] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    // Staging: thread item = 128 rows x 4 segments of 8 components, two per
-    // thread.  Rows past a set re-read its last row (in bounds; masked in the
-    // epilogue).  The next chunk's global loads are issued before this
-    // chunk's MFMAs and land in LDS after them.
+    // Staging: each chunk of 32 components goes from memory straight into
+    // LDS (global_load_lds_dwordx4: lane L of a wave writes the 16 bytes at
+    // the wave's LDS base + 16 L).  A wave's instruction fills 16 rows of one
+    // array, so the lane loads the segment that belongs in slot L & 3 of row
+    // L >> 2.  Rows past a set re-read its last row (in bounds; masked in the
+    // epilogue).  A chunk's fragments are read into registers first; after a
+    // barrier the next chunk's loads refill the buffer while the MFMAs run,
+    // with no staging registers.
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const uint16_t* ga[2];
     const uint16_t* gb[2];
-    int soff[2];
 #pragma unroll
     for (int it = 0; it < 2; ++it) {
-        const int item = threadIdx.x + 256 * it;
-        const int row = item >> 2, seg = item & 3;
-        ga[it] = hi + (size_t)(P.a + i0 + min(row, nra - 1)) * DIM + seg * 8;
-        gb[it] = hi + (size_t)(P.b + j0 + min(row, nrb - 1)) * DIM + seg * 8;
-        soff[it] = row * LROW + seg * 8;
+        const int row = (it * 4 + wv) * 16 + (lane >> 2);
+        const int gseg = (lane & 3) ^ ((row >> 2) & 3);
+        ga[it] = hi + (size_t)(P.a + i0 + min(row, nra - 1)) * DIM + gseg * 8;
+        gb[it] = hi + (size_t)(P.b + j0 + min(row, nrb - 1)) * DIM + gseg * 8;
     }
     const ptrdiff_t lo_off = lo - hi;
-    us8 pf[2][4];
-#pragma unroll
-    for (int it = 0; it < 2; ++it) {
-        pf[it][0] = *reinterpret_cast<const us8*>(ga[it]);
-        pf[it][1] = *reinterpret_cast<const us8*>(ga[it] + lo_off);
-        pf[it][2] = *reinterpret_cast<const us8*>(gb[it]);
-        pf[it][3] = *reinterpret_cast<const us8*>(gb[it] + lo_off);
-    }
-#pragma unroll
-    for (int kc = 0; kc < DIM; kc += 32) {
-        if (kc) __syncthreads();   // every wave is done with the previous chunk
+    typedef __attribute__((address_space(3))) void* lds_ptr;
+    auto issue = [&](int kc) {
 #pragma unroll
         for (int it = 0; it < 2; ++it) {
-            *reinterpret_cast<us8*>(sAh + soff[it]) = pf[it][0];
-            *reinterpret_cast<us8*>(sAl + soff[it]) = pf[it][1];
-            *reinterpret_cast<us8*>(sBh + soff[it]) = pf[it][2];
-            *reinterpret_cast<us8*>(sBl + soff[it]) = pf[it][3];
+            const int base = (it * 4 + wv) * 16 * 32;   // first u16 of the wave's 16 rows
+            __builtin_amdgcn_global_load_lds((const void*)(ga[it] + kc), (lds_ptr)(sAh + base), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)(ga[it] + lo_off + kc), (lds_ptr)(sAl + base), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)(gb[it] + kc), (lds_ptr)(sBh + base), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)(gb[it] + lo_off + kc), (lds_ptr)(sBl + base), 16, 0, 0);
         }
-        __syncthreads();
-        if (kc + 32 < DIM) {
+    };
+    issue(0);
 #pragma unroll
-            for (int it = 0; it < 2; ++it) {
-                pf[it][0] = *reinterpret_cast<const us8*>(ga[it] + kc + 32);
-                pf[it][1] = *reinterpret_cast<const us8*>(ga[it] + lo_off + kc + 32);
-                pf[it][2] = *reinterpret_cast<const us8*>(gb[it] + kc + 32);
-                pf[it][3] = *reinterpret_cast<const us8*>(gb[it] + lo_off + kc + 32);
-            }
+    for (int kc = 0; kc < DIM; kc += 32) {
+        // the previous chunk's MFMAs stay between its loads' issue and this wait
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's loads of the chunk have landed
+        __syncthreads();                 // and every other wave's
+        us8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const int row = wr * 64 + f * 16 + r16, col = wc * 64 + f * 16 + r16;
+            ah[f] = *reinterpret_cast<const us8*>(sAh + lds_slot(row, kq));
+            al[f] = *reinterpret_cast<const us8*>(sAl + lds_slot(row, kq));
+            bh[f] = *reinterpret_cast<const us8*>(sBh + lds_slot(col, kq));
+            bl[f] = *reinterpret_cast<const us8*>(sBl + lds_slot(col, kq));
         }
-        us8 bh[4], bl[4];
+        __syncthreads();   // every wave holds its fragments: the buffer may be refilled
+        if (kc + 32 < DIM) issue(kc + 32);
+        // the loads go out before the MFMAs (the scheduler would otherwise
+        // sink them to the end of the chunk, where nothing hides them)
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int nf = 0; nf < 4; ++nf) {
-            const int col = wc * 64 + nf * 16 + r16;
-            bh[nf] = *reinterpret_cast<const us8*>(sBh + col * LROW + kq * 8);
-            bl[nf] = *reinterpret_cast<const us8*>(sBl + col * LROW + kq * 8);
-        }
-#pragma unroll
-        for (int mf = 0; mf < 4; ++mf) {
-            const int row = wr * 64 + mf * 16 + r16;
-            const us8 ah = *reinterpret_cast<const us8*>(sAh + row * LROW + kq * 8);
-            const us8 al = *reinterpret_cast<const us8*>(sAl + row * LROW + kq * 8);
+        for (int mf = 0; mf < 4; ++mf)
 #pragma unroll
             for (int nf = 0; nf < 4; ++nf) {
-                acc[mf][nf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah), as_bf(bh[nf]), acc[mf][nf], 0, 0, 0);
-                acc[mf][nf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah), as_bf(bl[nf]), acc[mf][nf], 0, 0, 0);
-                acc[mf][nf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(al), as_bf(bh[nf]), acc[mf][nf], 0, 0, 0);
+                acc[mf][nf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah[mf]), as_bf(bh[nf]), acc[mf][nf], 0, 0, 0);
+                acc[mf][nf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(ah[mf]), as_bf(bl[nf]), acc[mf][nf], 0, 0, 0);
+                acc[mf][nf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf(al[mf]), as_bf(bh[nf]), acc[mf][nf], 0, 0, 0);
             }
-        }
     }
 
     // element (mf, nf, i) of this lane: tile row R = wr*64 + mf*16 + kq*4 + i,
