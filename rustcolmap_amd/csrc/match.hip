@@ -488,8 +488,8 @@ __device__ __forceinline__ int lds_slot(int row, int seg) { return row * 32 + ((
 // PASS 3: (check) max over the tile of |exact - estimate| / (|a|^2 + |b|^2)
 //         into *fb.ovf_n as float bits.
 // 4 waves in a 2 x 2 grid, each 64 x 64 = 4 x 4 MFMA 16x16x32 fragments.
-// Three workgroups per CU (42 KB of LDS each) need <= 168 registers per
-// lane: the compiler then keeps two lane coordinates in scratch across the
+// Three workgroups per CU (34 KB of LDS each, which would allow four) need
+// <= 168 registers per lane: the compiler then keeps two lane coordinates in scratch across the
 // product loop (nothing inside it), which costs far less than the third
 // wave per SIMD gains (1.11 -> 0.89 ms per batch of 190 pairs of 2048
 // descriptors, with the norms written to LDS before the loop).  The budget
